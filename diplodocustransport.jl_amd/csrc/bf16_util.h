@@ -1,0 +1,35 @@
+// bf16_util.h -- round-to-nearest-even float <-> bfloat16 bit conversions, host and device.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define DTX_HD __host__ __device__
+#else
+#define DTX_HD
+#endif
+
+namespace dtx {
+
+DTX_HD inline uint32_t f32_bits(float x) { return __builtin_bit_cast(uint32_t, x); }
+
+DTX_HD inline float bits_f32(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+// float -> bf16 with round-to-nearest-even (finite inputs).
+DTX_HD inline uint16_t bf16_rne(float x) {
+    uint32_t u = f32_bits(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+DTX_HD inline float bf16_to_f32(uint16_t h) { return bits_f32((uint32_t)h << 16); }
+
+// Split a double into hi + lo bf16 parts: hi = bf16(x), lo = bf16(x - hi).
+DTX_HD inline void split_bf16(double x, uint16_t* hi, uint16_t* lo) {
+    uint16_t h = bf16_rne((float)x);
+    double r = x - (double)bf16_to_f32(h);
+    *hi = h;
+    *lo = bf16_rne((float)r);
+}
+
+}  // namespace dtx

@@ -1,0 +1,1267 @@
+// runtime.hip -- host runtime behind the dtx C ABI (include/dtx.h).
+//
+// Replaces the device placement of ForwardEulerStruct / ForwardSymplecticEulerStruct
+// (src/SteppingStructs.jl:33-100, 142-227), the functor bodies (src/SteppingMethods.jl:12-107,
+// 120-243) and the Solve loop (src/Solver.jl:1-64).  Operators are packed once on the host
+// (binary channels in symmetric pair form, CSC -> column-sorted ELL), uploaded, and every step
+// is two (ForwardEuler) or three (symplectic) kernel launches on one HIP stream with no host
+// synchronisation; per-step diagnostics land in a device ring that the host reads when it
+// synchronises.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "bf16_util.h"
+#include "dtx_internal.h"
+
+using namespace dtx;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_create_error;
+
+struct HostChannel {
+    int32_t s_out, s1, s2, kind;
+    int32_t No, N1, N2;
+    int64_t npairs, npairs_pad;
+    std::vector<double> S;  // [pair][row] FP64
+    std::vector<uint32_t> pairs;
+};
+
+struct TimedEvent {
+    hipEvent_t start, stop;
+    int cls;  // 0 binary, 1 update
+};
+
+}  // namespace
+
+struct dtx_handle {
+    dtx_config cfg;
+    int32_t N = 0, C = 0, S = 0;
+    int64_t n = 0;
+    std::vector<int32_t> sp_off, sp_size;
+    int32_t precision = 0, method = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // host staging of the binary operator
+    bool binary = false;
+    bool binary_dirty = false;
+    std::map<std::tuple<int, int, int>, HostChannel> hch;
+    std::vector<double> loss_host;  // N*N column-major
+    bool loss_nonzero = false;
+    bool synth_binary = false;      // channels generated on the device
+
+    // device: binary operator and its work table
+    std::vector<PackedChannel> chans;
+    double* d_loss = nullptr;
+    BinWork* d_items = nullptr;
+    int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0;
+    double* d_part = nullptr;
+    int64_t part_stride = 0;
+    int32_t* d_nslot = nullptr;
+    int64_t pairs_total = 0;
+    double bin_flops = 0, gain_bytes = 0;
+
+    // device: state and vectors
+    double* d_f[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* d_fT = nullptr;
+    double *d_invap = nullptr, *d_dfinj = nullptr, *d_vol = nullptr, *d_finit = nullptr;
+    double* d_eweight = nullptr;
+    int32_t *d_state_species = nullptr, *d_species_off = nullptr;
+    double *d_moments = nullptr, *d_block_part = nullptr, *d_status = nullptr;
+    uint32_t* d_counter = nullptr;
+    uint8_t* d_active = nullptr;
+    EllMatrix ell[3];
+    double* d_emi = nullptr;
+    double *d_fmom = nullptr, *d_dfmom = nullptr, *d_dpflux = nullptr, *d_dbin = nullptr,
+           *d_demi = nullptr;
+    bool vectors_set = false;
+    bool emission = false;
+
+    int64_t step_counter = 0;
+    int64_t reported = 0;
+    std::vector<int8_t> step_verbose;  // verbose of steps not yet reported (ring order)
+    int64_t device_bytes = 0;
+    std::vector<void*> allocations;
+
+    bool timing = false;
+    std::vector<TimedEvent> events;
+    dtx_kernel_times times{};
+};
+
+// --------------------------------------------------------------------------- helpers ------
+
+static int fail(dtx_handle* h, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    else {
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_create_error = buf;
+    }
+    return code;
+}
+
+#define HIP_CHECK(h, expr)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((h), DTX_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                  \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                        \
+    } while (0)
+
+template <typename T>
+static int dalloc(dtx_handle* h, T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    size_t bytes = (size_t)count * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(h, DTX_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    h->device_bytes += (int64_t)bytes;
+    h->allocations.push_back(*p);
+    return DTX_OK;
+}
+
+template <typename T>
+static void dfree(dtx_handle* h, T** p, int64_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        auto it = std::find(h->allocations.begin(), h->allocations.end(), (void*)*p);
+        if (it != h->allocations.end()) h->allocations.erase(it);
+        h->device_bytes -= (int64_t)(count > 0 ? count : 1) * (int64_t)sizeof(T);
+        *p = nullptr;
+    }
+}
+
+template <typename T>
+static int upload(dtx_handle* h, T* dst, const T* src, int64_t count) {
+    if (count <= 0) return DTX_OK;
+    HIP_CHECK(h, hipMemcpyAsync(dst, src, (size_t)count * sizeof(T), hipMemcpyHostToDevice,
+                                h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return DTX_OK;
+}
+
+static void pair_of(const HostChannel& ch, int64_t p, int* i, int* j) {
+    uint32_t v = ch.pairs[p];
+    *i = (int)(v & 0xFFFFu);
+    *j = (int)(v >> 16);
+}
+
+uint64_t dtx::splitmix_key(uint64_t seed, uint64_t stream) {
+    uint64_t z = seed + (stream + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Build the pair enumeration of a channel: TRI (i <= j, i outer) or RECT (i in s1 outer).
+// Entries are cell-local state indices; the table is padded with PAIR_PAD to PAIR_ALIGN.
+static void build_pairs(const dtx_handle* h, HostChannel& ch) {
+    ch.pairs.clear();
+    const int o1 = h->sp_off[ch.s1], o2 = h->sp_off[ch.s2];
+    if (ch.kind == CH_TRI) {
+        for (int i = 0; i < ch.N1; ++i)
+            for (int j = i; j < ch.N1; ++j)
+                ch.pairs.push_back((uint32_t)(o1 + i) | ((uint32_t)(o1 + j) << 16));
+    } else {
+        for (int i = 0; i < ch.N1; ++i)
+            for (int j = 0; j < ch.N2; ++j)
+                ch.pairs.push_back((uint32_t)(o1 + i) | ((uint32_t)(o2 + j) << 16));
+    }
+    ch.npairs = (int64_t)ch.pairs.size();
+    ch.npairs_pad = (ch.npairs + PAIR_ALIGN - 1) / PAIR_ALIGN * PAIR_ALIGN;
+    ch.pairs.resize((size_t)ch.npairs_pad, PAIR_PAD);
+}
+
+static HostChannel& host_channel(dtx_handle* h, int s_out, int s1, int s2) {
+    auto key = std::make_tuple(s_out, s1, s2);
+    auto it = h->hch.find(key);
+    if (it != h->hch.end()) return it->second;
+    HostChannel ch;
+    ch.s_out = s_out;
+    ch.s1 = s1;
+    ch.s2 = s2;
+    ch.kind = (s1 == s2) ? CH_TRI : CH_RECT;
+    ch.No = h->sp_size[s_out];
+    ch.N1 = h->sp_size[s1];
+    ch.N2 = h->sp_size[s2];
+    build_pairs(h, ch);
+    ch.S.assign((size_t)ch.npairs_pad * ch.No, 0.0);
+    return h->hch.emplace(key, std::move(ch)).first->second;
+}
+
+static void release_binary_device(dtx_handle* h) {
+    for (auto& c : h->chans) {
+        if (c.A) {
+            (void)hipFree(c.A);
+            h->device_bytes -= c.a_bytes;
+            auto it = std::find(h->allocations.begin(), h->allocations.end(), c.A);
+            if (it != h->allocations.end()) h->allocations.erase(it);
+        }
+        if (c.pairs) {
+            (void)hipFree(c.pairs);
+            h->device_bytes -= c.npairs_pad * 4;
+            auto it = std::find(h->allocations.begin(), h->allocations.end(), (void*)c.pairs);
+            if (it != h->allocations.end()) h->allocations.erase(it);
+        }
+    }
+    h->chans.clear();
+}
+
+static int64_t packed_elems(int precision, int64_t npairs_pad, int nktiles) {
+    // FP64/FP32: [ktile][pair][TK];  BF16X3: [ktile][chunk][hi TK*32 | lo TK*32]
+    int64_t per_tile = npairs_pad * TK * (precision == DTX_PREC_BF16X3 ? 2 : 1);
+    return per_tile * nktiles;
+}
+
+static int64_t elem_bytes(int precision) {
+    return precision == DTX_PREC_FP64 ? 8 : (precision == DTX_PREC_FP32 ? 4 : 2);
+}
+
+// Pack an FP64 [pair][row] channel into the device layout of the handle's precision.
+static int pack_upload_channel(dtx_handle* h, const HostChannel& hc) {
+    PackedChannel pc;
+    pc.s_out = hc.s_out;
+    pc.s1 = hc.s1;
+    pc.s2 = hc.s2;
+    pc.kind = hc.kind;
+    pc.No = hc.No;
+    pc.N1 = hc.N1;
+    pc.N2 = hc.N2;
+    pc.npairs = hc.npairs;
+    pc.npairs_pad = hc.npairs_pad;
+    pc.nktiles = (hc.No + TK - 1) / TK;
+    const int64_t elems = packed_elems(h->precision, pc.npairs_pad, pc.nktiles);
+    const int64_t eb = elem_bytes(h->precision);
+    pc.a_bytes = elems * eb;
+    std::vector<uint8_t> buf((size_t)pc.a_bytes, 0);
+    const int64_t P = pc.npairs_pad;
+    for (int kt = 0; kt < pc.nktiles; ++kt) {
+        for (int64_t p = 0; p < P; ++p) {
+            for (int r = 0; r < TK; ++r) {
+                const int row = kt * TK + r;
+                const double v = (row < hc.No) ? hc.S[(size_t)p * hc.No + row] : 0.0;
+                if (h->precision == DTX_PREC_FP64) {
+                    ((double*)buf.data())[(kt * P + p) * TK + r] = v;
+                } else if (h->precision == DTX_PREC_FP32) {
+                    ((float*)buf.data())[(kt * P + p) * TK + r] = (float)v;
+                } else {
+                    uint16_t hi, lo;
+                    split_bf16(v, &hi, &lo);
+                    const int64_t chunk = p / 32, pp = p % 32;
+                    uint16_t* base = (uint16_t*)buf.data() + kt * P * TK * 2 + chunk * TK * 64;
+                    base[r * 32 + pp] = hi;
+                    base[TK * 32 + r * 32 + pp] = lo;
+                }
+            }
+        }
+    }
+    hipError_t e = hipMalloc(&pc.A, (size_t)pc.a_bytes);
+    if (e != hipSuccess)
+        return fail(h, DTX_ERR_OOM, "hipMalloc(%lld) for a gain channel failed: %s",
+                    (long long)pc.a_bytes, hipGetErrorString(e));
+    h->device_bytes += pc.a_bytes;
+    h->allocations.push_back(pc.A);
+    e = hipMalloc((void**)&pc.pairs, (size_t)P * 4);
+    if (e != hipSuccess) {
+        h->chans.push_back(pc);
+        return fail(h, DTX_ERR_OOM, "hipMalloc for a pair table failed: %s",
+                    hipGetErrorString(e));
+    }
+    h->device_bytes += P * 4;
+    h->allocations.push_back(pc.pairs);
+    h->chans.push_back(pc);
+    HIP_CHECK(h, hipMemcpy(pc.A, buf.data(), (size_t)pc.a_bytes, hipMemcpyHostToDevice));
+    HIP_CHECK(h, hipMemcpy(pc.pairs, hc.pairs.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+    return DTX_OK;
+}
+
+// Build the grouped work table for the handle's channels and cell count.
+static int build_work(dtx_handle* h) {
+    dfree(h, &h->d_items, h->nitems);
+    dfree(h, &h->d_part, h->part_stride * h->max_slots);
+    dfree(h, &h->d_nslot, h->N);
+    h->nitems = 0;
+    h->max_slots = 0;
+    const int C = h->C;
+    h->tc = (C <= 16) ? 16 : 64;
+    const int ctiles = (C + h->tc - 1) / h->tc;
+    h->ldc = ctiles * h->tc;
+    double total = 0;
+    h->pairs_total = 0;
+    h->bin_flops = 0;
+    h->gain_bytes = 0;
+    for (auto& c : h->chans) {
+        total += (double)c.nktiles * ctiles * c.npairs_pad;
+        h->pairs_total += c.npairs;
+        h->bin_flops += 2.0 * c.No * (double)c.npairs * C;
+        h->gain_bytes += (double)c.a_bytes;
+    }
+    const int target_items = 1024;
+    int64_t chunk = (int64_t)std::ceil(total / target_items);
+    chunk = std::max<int64_t>(512, (chunk + PAIR_ALIGN - 1) / PAIR_ALIGN * PAIR_ALIGN);
+    std::vector<int> next_slot(h->S, 0);
+    std::vector<BinWork> items;
+    const int64_t cellN = (int64_t)C * h->N;
+    h->part_stride = cellN;
+    // slots are assigned first; the partial pointer is patched once the buffer exists
+    struct Pending { BinWork w; int slot; };
+    std::vector<Pending> pend;
+    for (auto& c : h->chans) {
+        for (int64_t p0 = 0; p0 < c.npairs_pad; p0 += chunk) {
+            const int64_t p1 = std::min<int64_t>(p0 + chunk, c.npairs_pad);
+            const int slot = next_slot[c.s_out]++;
+            for (int kt = 0; kt < c.nktiles; ++kt)
+                for (int ct = 0; ct < ctiles; ++ct) {
+                    BinWork w{};
+                    w.A = c.A;
+                    w.pairs = c.pairs;
+                    w.fT = h->d_fT;
+                    w.a_ktile_stride = packed_elems(h->precision, c.npairs_pad, 1);
+                    w.ktile = kt;
+                    w.ctile = ct;
+                    w.p_begin = (int32_t)p0;
+                    w.p_end = (int32_t)p1;
+                    w.row_base = h->sp_off[c.s_out];
+                    w.rows = c.No;
+                    w.cells = C;
+                    w.ldc = h->ldc;
+                    w.N = h->N;
+                    pend.push_back({w, slot});
+                }
+        }
+    }
+    for (int s = 0; s < h->S; ++s) h->max_slots = std::max(h->max_slots, next_slot[s]);
+    if (h->max_slots > 0) {
+        int rc = dalloc(h, &h->d_part, h->part_stride * h->max_slots);
+        if (rc) return rc;
+    }
+    for (auto& pd : pend) {
+        pd.w.part = h->d_part + (int64_t)pd.slot * h->part_stride;
+        items.push_back(pd.w);
+    }
+    h->nitems = (int32_t)items.size();
+    if (h->nitems) {
+        int rc = dalloc(h, &h->d_items, h->nitems);
+        if (rc) return rc;
+        rc = upload(h, h->d_items, items.data(), h->nitems);
+        if (rc) return rc;
+    }
+    std::vector<int32_t> nslot(h->N, 0);
+    for (int s = 0; s < h->S; ++s)
+        for (int k = 0; k < h->sp_size[s]; ++k) nslot[h->sp_off[s] + k] = next_slot[s];
+    int rc = dalloc(h, &h->d_nslot, h->N);
+    if (rc) return rc;
+    return upload(h, h->d_nslot, nslot.data(), h->N);
+}
+
+static int finalize_binary(dtx_handle* h) {
+    if (!h->binary_dirty) return DTX_OK;
+    if (!h->synth_binary) {
+        release_binary_device(h);
+        for (auto& kv : h->hch) {
+            const HostChannel& hc = kv.second;
+            bool any = false;
+            for (double v : hc.S)
+                if (v != 0.0) { any = true; break; }
+            if (!any) continue;
+            int rc = pack_upload_channel(h, hc);
+            if (rc) return rc;
+        }
+        h->hch.clear();
+        dfree(h, &h->d_loss, (int64_t)h->N * h->N);
+        if (h->loss_nonzero) {
+            int rc = dalloc(h, &h->d_loss, (int64_t)h->N * h->N);
+            if (rc) return rc;
+            rc = upload(h, h->d_loss, h->loss_host.data(), (int64_t)h->N * h->N);
+            if (rc) return rc;
+        }
+        h->loss_host.clear();
+        h->loss_host.shrink_to_fit();
+    }
+    int rc = build_work(h);
+    if (rc) return rc;
+    h->binary_dirty = false;
+    return DTX_OK;
+}
+
+// fT[k * ldc + c] = f[c * N + k]
+__global__ void transpose_state_kernel(const double* f, double* fT, int N, int C, int ldc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)N * C) return;
+    int c = (int)(i / N), k = (int)(i % N);
+    fT[(int64_t)k * ldc + c] = f[i];
+}
+
+static int refresh_fT(dtx_handle* h) {
+    int64_t total = (int64_t)h->N * h->C;
+    int blocks = (int)((total + 255) / 256);
+    transpose_state_kernel<<<blocks, 256, 0, h->stream>>>(h->d_f[h->cur], h->d_fT, h->N, h->C,
+                                                          h->ldc);
+    HIP_CHECK(h, hipGetLastError());
+    return DTX_OK;
+}
+
+static int ensure_ready(dtx_handle* h) {
+    if (!h->vectors_set)
+        return fail(h, DTX_ERR_STATE, "dtx_set_vectors must be called before stepping");
+    if (h->binary && h->binary_dirty) {
+        int rc = finalize_binary(h);
+        if (rc) return rc;
+        rc = refresh_fT(h);  // ldc may have been set now
+        if (rc) return rc;
+    }
+    return DTX_OK;
+}
+
+static void print_shortest(char* out, size_t cap, double v) {
+    if (std::isinf(v)) { snprintf(out, cap, v > 0 ? "Inf" : "-Inf"); return; }
+    if (std::isnan(v)) { snprintf(out, cap, "NaN"); return; }
+    for (int prec = 1; prec <= 17; ++prec) {
+        snprintf(out, cap, "%.*g", prec, v);
+        if (strtod(out, nullptr) == v) return;
+    }
+}
+
+static void print_status(const dtx_handle* h, const double* st, int verbose) {
+    char a[40], b[40], c[40], d[40], e[40], f[40], g[40], t[40];
+    if (h->method == DTX_METHOD_FORWARD_EULER && st[ST_FINITE] == 0.0)
+        printf("non-finite value in df calculation\n");
+    if (verbose < 1 || verbose > 3) return;
+    const double cr = st[ST_CR];
+    print_shortest(a, sizeof a, cr);
+    print_shortest(t, sizeof t, st[ST_TSTART]);
+    print_shortest(b, sizeof b, st[ST_DT]);
+    if (verbose == 1 && cr > 1.0) {
+        printf("Cr = %s, t=%s, dt=%s, system may be unstable\n", a, t, b);
+    } else if (verbose == 2) {
+        printf("\rCr = %s, t=%s, dt=%s\n", a, t, b);
+    } else if (verbose == 3) {
+        print_shortest(c, sizeof c, st[ST_CR_BIN]);
+        print_shortest(d, sizeof d, st[ST_CR_EMI]);
+        print_shortest(e, sizeof e, st[ST_CR_FLUX]);
+        if (h->method == DTX_METHOD_FORWARD_EULER) {
+            printf("Cr = %s,Cr_Bin = %s, Cr_Emi = %s, Cr_Flux = %s, t=%s, dt=%s\n", a, c, d, e,
+                   t, b);
+        } else {
+            print_shortest(f, sizeof f, st[ST_CR_MOM]);
+            print_shortest(g, sizeof g, st[ST_CR_SPACE]);
+            printf("Cr = %s, Cr_Bin = %s, Cr_Emi = %s, Cr_PFlux = %s, Cr_Momentum = %s, "
+                   "Cr_Space = %s,  t=%s, dt=%s\n", a, c, d, e, f, g, t, b);
+        }
+    }
+}
+
+// Read and print the diagnostics of every step not yet reported (the stream must be idle).
+static int drain_status(dtx_handle* h) {
+    const int64_t pending = h->step_counter - h->reported;
+    if (pending <= 0) return DTX_OK;
+    std::vector<double> ring((size_t)STATUS_RING * STATUS_WORDS);
+    HIP_CHECK(h, hipMemcpy(ring.data(), h->d_status, ring.size() * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    const int64_t first = std::max<int64_t>(h->reported, h->step_counter - STATUS_RING);
+    for (int64_t s = first; s < h->step_counter; ++s) {
+        const double* st = ring.data() + (s % STATUS_RING) * STATUS_WORDS;
+        print_status(h, st, h->step_verbose[(size_t)(s % STATUS_RING)]);
+    }
+    fflush(stdout);
+    h->reported = h->step_counter;
+    return DTX_OK;
+}
+
+static void fill_stats(const double* st, dtx_step_stats* out) {
+    memset(out, 0, sizeof *out);
+    out->step_index = (int32_t)st[ST_STEP];
+    out->finite = (int32_t)st[ST_FINITE];
+    out->cr_evaluated = (int32_t)st[ST_CR_EVAL];
+    out->cr = st[ST_CR];
+    out->cr_bin = st[ST_CR_BIN];
+    out->cr_emi = st[ST_CR_EMI];
+    out->cr_flux = st[ST_CR_FLUX];
+    out->cr_momentum = st[ST_CR_MOM];
+    out->cr_space = st[ST_CR_SPACE];
+    out->sum_f = st[ST_SUMF];
+    out->dt = st[ST_DT];
+    out->t_start = st[ST_TSTART];
+    out->save = (int32_t)st[ST_SAVE];
+}
+
+// Largest number of steps Solve accepts for one save interval (see dtx_solve).
+constexpr double kMaxIntervalSteps = 1e8;
+
+static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start, double dt,
+                            int save, int which_ell) {
+    UpdateArgs a{};
+    a.f = h->d_f[h->cur];
+    a.f_out = h->d_f[h->cur ^ 1];
+    a.fT = h->d_fT;
+    a.inv_ap = h->d_invap;
+    a.df_inj = h->d_dfinj;
+    a.vol = h->d_vol;
+    a.part = h->d_part;
+    a.nslot = h->d_nslot;
+    a.part_stride = h->part_stride;
+    a.loss = h->d_loss;
+    a.emi = h->d_emi;
+    a.active = h->d_active;
+    const EllMatrix& E = h->ell[which_ell];
+    a.ell_col = E.col;
+    a.ell_val = E.val;
+    a.ell_width = E.width;
+    a.eweight = h->d_eweight;
+    a.state_species = h->d_state_species;
+    a.species_off = h->d_species_off;
+    a.moments = h->d_moments;
+    a.block_part = h->d_block_part;
+    a.status = h->d_status;
+    a.counter = h->d_counter;
+    a.f_mom = h->d_fmom;
+    a.df_mom = h->d_dfmom;
+    a.dpflux = h->d_dpflux;
+    a.dbin = h->d_dbin;
+    a.demi = h->d_demi;
+    a.n = h->n;
+    a.N = h->N;
+    a.C = h->C;
+    a.S = h->S;
+    a.ldc = h->ldc;
+    a.binary = h->binary && (h->nitems > 0 || h->d_loss != nullptr);
+    a.emission = h->emission;
+    a.verbose = verbose;
+    a.step_index = (int32_t)(h->step_counter % 0x7FFFFFFF);
+    a.ring = STATUS_RING;
+    a.s = s;
+    a.t_start = t_start;
+    a.dt = dt;
+    a.save = save;
+    return a;
+}
+
+static int timed_begin(dtx_handle* h, int cls, TimedEvent* ev) {
+    if (!h->timing) return DTX_OK;
+    ev->cls = cls;
+    HIP_CHECK(h, hipEventCreate(&ev->start));
+    HIP_CHECK(h, hipEventCreate(&ev->stop));
+    HIP_CHECK(h, hipEventRecord(ev->start, h->stream));
+    return DTX_OK;
+}
+
+static int timed_end(dtx_handle* h, TimedEvent* ev) {
+    if (!h->timing) return DTX_OK;
+    HIP_CHECK(h, hipEventRecord(ev->stop, h->stream));
+    h->events.push_back(*ev);
+    return DTX_OK;
+}
+
+// Enqueue one functor call with an already clipped dt.
+static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save) {
+    const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
+    const bool bin = h->binary && h->nitems > 0;
+    TimedEvent ev{};
+    if (bin) {
+        int rc = timed_begin(h, 0, &ev);
+        if (rc) return rc;
+        HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
+        rc = timed_end(h, &ev);
+        if (rc) return rc;
+    }
+    int rc = timed_begin(h, 1, &ev);
+    if (rc) return rc;
+    if (h->method == DTX_METHOD_FORWARD_EULER) {
+        UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_F);
+        HIP_CHECK(h, launch_euler_update(a, h->stream));
+    } else {
+        UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_P);
+        HIP_CHECK(h, launch_symplectic_momentum(a, h->stream));
+        UpdateArgs b = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_X);
+        HIP_CHECK(h, launch_symplectic_space(b, h->stream));
+    }
+    rc = timed_end(h, &ev);
+    if (rc) return rc;
+    if (h->step_verbose.size() != (size_t)STATUS_RING) h->step_verbose.assign(STATUS_RING, 0);
+    h->step_verbose[(size_t)(h->step_counter % STATUS_RING)] = (int8_t)verbose;
+    h->cur ^= 1;
+    h->step_counter++;
+    if (h->step_counter - h->reported >= STATUS_RING / 2) {
+        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+        return drain_status(h);
+    }
+    return DTX_OK;
+}
+
+// CSC (1-based) -> ELL with slots sorted by column inside each row.
+static int set_ell(dtx_handle* h, EllMatrix& E, int64_t n, const int64_t* colptr,
+                   const int64_t* rowval, const double* nzval) {
+    std::vector<std::vector<std::pair<int64_t, double>>> rows((size_t)n);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t k0 = colptr[j] - 1, k1 = colptr[j + 1] - 1;
+        if (k0 < 0 || k1 < k0)
+            return fail(h, DTX_ERR_INVALID, "colptr is not a valid 1-based CSC pointer");
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t r = rowval[k] - 1;
+            if (r < 0 || r >= n) return fail(h, DTX_ERR_INVALID, "rowval out of range");
+            rows[(size_t)r].push_back({j, nzval[k]});
+        }
+    }
+    int64_t width = 0, nnz = 0;
+    for (auto& r : rows) {
+        width = std::max<int64_t>(width, (int64_t)r.size());
+        nnz += (int64_t)r.size();
+    }
+    dfree(h, &E.col, E.n * E.width);
+    dfree(h, &E.val, E.n * E.width);
+    E.n = n;
+    E.width = (int32_t)width;
+    E.nnz = nnz;
+    if (width == 0) return DTX_OK;
+    std::vector<int32_t> col((size_t)(width * n), -1);
+    std::vector<double> val((size_t)(width * n), 0.0);
+    for (int64_t r = 0; r < n; ++r) {
+        // column order is already ascending (columns were visited in order)
+        auto& v = rows[(size_t)r];
+        for (size_t s = 0; s < v.size(); ++s) {
+            col[(size_t)s * n + r] = (int32_t)v[s].first;
+            val[(size_t)s * n + r] = v[s].second;
+        }
+    }
+    int rc = dalloc(h, &E.col, width * n);
+    if (rc) return rc;
+    rc = dalloc(h, &E.val, width * n);
+    if (rc) return rc;
+    rc = upload(h, E.col, col.data(), width * n);
+    if (rc) return rc;
+    return upload(h, E.val, val.data(), width * n);
+}
+
+// ------------------------------------------------------------------------ C ABI ----------
+
+extern "C" {
+
+int dtx_create(const dtx_config* cfg, dtx_handle** out) {
+    if (!cfg || !out) return fail(nullptr, DTX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->num_species < 1 || cfg->num_species > DTX_MAX_SPECIES)
+        return fail(nullptr, DTX_ERR_INVALID, "num_species must be in 1..%d", DTX_MAX_SPECIES);
+    if (cfg->x_num < 1 || cfg->y_num < 1 || cfg->z_num < 1)
+        return fail(nullptr, DTX_ERR_INVALID, "x_num, y_num, z_num must be positive");
+    if (cfg->precision < DTX_PREC_FP64 || cfg->precision > DTX_PREC_BF16X3)
+        return fail(nullptr, DTX_ERR_INVALID, "unknown precision %d", cfg->precision);
+    if (cfg->method != DTX_METHOD_FORWARD_EULER &&
+        cfg->method != DTX_METHOD_FORWARD_SYMPLECTIC_EULER)
+        return fail(nullptr, DTX_ERR_INVALID, "unknown method %d", cfg->method);
+    if (!(cfg->dt0 > 0.0)) return fail(nullptr, DTX_ERR_INVALID, "dt0 must be positive");
+    std::unique_ptr<dtx_handle> h(new dtx_handle());
+    h->cfg = *cfg;
+    h->S = cfg->num_species;
+    h->sp_off.assign(h->S + 1, 0);
+    h->sp_size.assign(h->S, 0);
+    int64_t N = 0;
+    for (int s = 0; s < h->S; ++s) {
+        if (cfg->px_num[s] < 1 || cfg->py_num[s] < 1 || cfg->pz_num[s] < 1)
+            return fail(nullptr, DTX_ERR_INVALID, "species %d has an empty momentum grid", s);
+        h->sp_off[s] = (int32_t)N;
+        h->sp_size[s] = cfg->px_num[s] * cfg->py_num[s] * cfg->pz_num[s];
+        N += h->sp_size[s];
+    }
+    h->sp_off[h->S] = (int32_t)N;
+    // The update kernels keep two N-length FP64 rows of a cell in LDS (plus 3 KB of
+    // reduction scratch) within the 64 KB per-workgroup budget.
+    if (N > 3840)
+        return fail(nullptr, DTX_ERR_UNSUPPORTED, "N = %lld states per cell exceeds 3840",
+                    (long long)N);
+    h->N = (int32_t)N;
+    const int64_t C = (int64_t)cfg->x_num * cfg->y_num * cfg->z_num;
+    if (C > (1 << 30)) return fail(nullptr, DTX_ERR_UNSUPPORTED, "too many cells");
+    h->C = (int32_t)C;
+    h->n = N * C;
+    if (h->n > 0x7FFFFFFF)
+        return fail(nullptr, DTX_ERR_UNSUPPORTED, "n = %lld exceeds the 32-bit column index",
+                    (long long)h->n);
+    h->precision = cfg->precision;
+    h->method = cfg->method;
+    h->tc = (h->C <= 16) ? 16 : 64;
+    h->ldc = (h->C + h->tc - 1) / h->tc * h->tc;
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess)
+        return fail(nullptr, DTX_ERR_HIP, "hipSetDevice(%d): %s", cfg->device,
+                    hipGetErrorString(e));
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return fail(nullptr, DTX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    dtx_handle* hp = h.get();
+    int rc = 0;
+    const int64_t n = h->n;
+    rc |= dalloc(hp, &h->d_f[0], n);
+    rc |= dalloc(hp, &h->d_f[1], n);
+    rc |= dalloc(hp, &h->d_fT, (int64_t)h->ldc * h->N);
+    rc |= dalloc(hp, &h->d_invap, n);
+    rc |= dalloc(hp, &h->d_dfinj, n);
+    rc |= dalloc(hp, &h->d_vol, C);
+    rc |= dalloc(hp, &h->d_finit, n);
+    rc |= dalloc(hp, &h->d_state_species, N);
+    rc |= dalloc(hp, &h->d_species_off, h->S + 1);
+    rc |= dalloc(hp, &h->d_moments, C * h->S * 2);
+    rc |= dalloc(hp, &h->d_block_part, C * BP_WORDS);
+    rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
+    rc |= dalloc(hp, &h->d_counter, 1);
+    if (h->method == DTX_METHOD_FORWARD_SYMPLECTIC_EULER) {
+        rc |= dalloc(hp, &h->d_fmom, n);
+        rc |= dalloc(hp, &h->d_dfmom, n);
+        rc |= dalloc(hp, &h->d_dpflux, n);
+        rc |= dalloc(hp, &h->d_dbin, n);
+        rc |= dalloc(hp, &h->d_demi, n);
+    }
+    if (rc) {
+        std::string msg = h->err;
+        dtx_destroy(h.release());
+        return fail(nullptr, DTX_ERR_OOM, "%s", msg.c_str());
+    }
+    std::vector<int32_t> ss((size_t)N);
+    for (int s = 0; s < h->S; ++s)
+        for (int k = 0; k < h->sp_size[s]; ++k) ss[(size_t)(h->sp_off[s] + k)] = s;
+    rc = upload(hp, h->d_state_species, ss.data(), N);
+    if (!rc) rc = upload(hp, h->d_species_off, h->sp_off.data(), h->S + 1);
+    if (!rc) {
+        e = hipMemsetAsync(h->d_counter, 0, sizeof(uint32_t), h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * N * 8, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->d_status, 0,
+                                                (size_t)STATUS_RING * STATUS_WORDS * 8,
+                                                h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->d_moments, 0, (size_t)C * h->S * 16,
+                                                h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(hp, DTX_ERR_HIP, "init: %s", hipGetErrorString(e));
+    }
+    if (rc) {
+        std::string msg = h->err;
+        dtx_destroy(h.release());
+        return fail(nullptr, rc, "%s", msg.c_str());
+    }
+    h->step_verbose.assign(STATUS_RING, 0);
+    *out = h.release();
+    return DTX_OK;
+}
+
+void dtx_destroy(dtx_handle* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& ev : h->events) {
+        (void)hipEventDestroy(ev.start);
+        (void)hipEventDestroy(ev.stop);
+    }
+    release_binary_device(h);
+    for (void* p : h->allocations) (void)hipFree(p);
+    h->allocations.clear();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char* dtx_last_error(const dtx_handle* h) {
+    if (h) return h->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_create_error.c_str();
+}
+
+int dtx_set_flux_csc(dtx_handle* h, int32_t which, int64_t n, const int64_t* colptr,
+                     const int64_t* rowval, const double* nzval) {
+    if (!h) return DTX_ERR_INVALID;
+    if (which < DTX_FLUX_F || which > DTX_FLUX_P)
+        return fail(h, DTX_ERR_INVALID, "unknown flux matrix %d", which);
+    if (n != h->n)
+        return fail(h, DTX_ERR_INVALID, "flux matrix is %lld x %lld, expected n = %lld",
+                    (long long)n, (long long)n, (long long)h->n);
+    if (!colptr || (colptr[n] > 1 && (!rowval || !nzval)))
+        return fail(h, DTX_ERR_INVALID, "null CSC array");
+    return set_ell(h, h->ell[which], n, colptr, rowval, nzval);
+}
+
+int dtx_set_binary_dense(dtx_handle* h, int64_t N, const double* M) {
+    if (!h) return DTX_ERR_INVALID;
+    if (N != h->N)
+        return fail(h, DTX_ERR_INVALID, "M_Bin is (%lld^2 x %lld), expected N = %d",
+                    (long long)N, (long long)N, h->N);
+    if (!M) return fail(h, DTX_ERR_INVALID, "null M_Bin");
+    h->hch.clear();
+    h->synth_binary = false;
+    h->loss_host.assign((size_t)N * N, 0.0);
+    h->loss_nonzero = false;
+    const int64_t NN = N * N;
+    // Fold M[k,i,j] and M[k,j,i] into one pair coefficient; entries with k == i or k == j
+    // (the loss diagonal written by LossMatrix_to_M_Bin!) go to the FP64 loss matrix.
+    for (int so = 0; so < h->S; ++so)
+        for (int s1 = 0; s1 < h->S; ++s1)
+            for (int s2 = s1; s2 < h->S; ++s2) {
+                HostChannel& ch = host_channel(h, so, s1, s2);
+                const int o = h->sp_off[so];
+                for (int64_t p = 0; p < ch.npairs; ++p) {
+                    int gi, gj;
+                    pair_of(ch, p, &gi, &gj);
+                    const double* Mij = M + (int64_t)gi * N + (int64_t)gj * NN;
+                    const double* Mji = M + (int64_t)gj * N + (int64_t)gi * NN;
+                    double* dst = ch.S.data() + (size_t)p * ch.No;
+                    for (int k = 0; k < ch.No; ++k) {
+                        const int gk = o + k;
+                        double v = (gi == gj) ? Mij[gk] : Mij[gk] + Mji[gk];
+                        if (v != 0.0 && (gk == gi || gk == gj)) {
+                            const int other = (gk == gi) ? gj : gi;
+                            h->loss_host[(size_t)gk + (size_t)N * other] += v;
+                            h->loss_nonzero = true;
+                            v = 0.0;
+                        }
+                        dst[k] = v;
+                    }
+                }
+            }
+    h->binary = true;
+    h->binary_dirty = true;
+    return DTX_OK;
+}
+
+int dtx_add_binary_gain(dtx_handle* h, int32_t s_out, int32_t s1, int32_t s2, const double* G) {
+    if (!h) return DTX_ERR_INVALID;
+    if (s_out < 0 || s_out >= h->S || s1 < 0 || s1 >= h->S || s2 < 0 || s2 >= h->S)
+        return fail(h, DTX_ERR_INVALID, "species index out of range");
+    if (!G) return fail(h, DTX_ERR_INVALID, "null gain block");
+    if (h->synth_binary) {
+        release_binary_device(h);
+        h->synth_binary = false;
+    }
+    if (h->loss_host.empty()) h->loss_host.assign((size_t)h->N * h->N, 0.0);
+    const int No = h->sp_size[s_out], N1 = h->sp_size[s1], N2 = h->sp_size[s2];
+    const int a = std::min(s1, s2), b = std::max(s1, s2);
+    HostChannel& ch = host_channel(h, s_out, a, b);
+    const int oa = h->sp_off[a], ob = h->sp_off[b];
+    for (int64_t p = 0; p < ch.npairs; ++p) {
+        int gi, gj;
+        pair_of(ch, p, &gi, &gj);
+        const int i = gi - oa, j = gj - ob;  // i in a, j in b
+        double* dst = ch.S.data() + (size_t)p * No;
+        const double* g1;
+        const double* g2 = nullptr;
+        if (s1 == s2) {  // TRI: G[k,i,j] + G[k,j,i]  (i <= j)
+            g1 = G + (int64_t)No * (i + (int64_t)N1 * j);
+            if (i != j) g2 = G + (int64_t)No * (j + (int64_t)N1 * i);
+        } else if (s1 == a) {
+            g1 = G + (int64_t)No * (i + (int64_t)N1 * j);
+        } else {  // given as (s_out; b, a): G[k, j, i]
+            g1 = G + (int64_t)No * (j + (int64_t)N1 * i);
+        }
+        for (int k = 0; k < No; ++k) dst[k] += g2 ? g1[k] + g2[k] : g1[k];
+    }
+    (void)N2;
+    h->binary = true;
+    h->binary_dirty = true;
+    return DTX_OK;
+}
+
+int dtx_add_binary_loss(dtx_handle* h, int32_t s1, int32_t s2, const double* L) {
+    if (!h) return DTX_ERR_INVALID;
+    if (s1 < 0 || s1 >= h->S || s2 < 0 || s2 >= h->S)
+        return fail(h, DTX_ERR_INVALID, "species index out of range");
+    if (!L) return fail(h, DTX_ERR_INVALID, "null loss block");
+    if (h->synth_binary)
+        return fail(h, DTX_ERR_STATE, "cannot mix host channels with a synthetic operator");
+    if (h->loss_host.empty()) h->loss_host.assign((size_t)h->N * h->N, 0.0);
+    const int N1 = h->sp_size[s1], N2 = h->sp_size[s2];
+    const int o1 = h->sp_off[s1], o2 = h->sp_off[s2];
+    for (int c = 0; c < N2; ++c)
+        for (int a = 0; a < N1; ++a) {
+            const double v = L[a + (int64_t)N1 * c];
+            if (v != 0.0) {
+                h->loss_host[(size_t)(o1 + a) + (size_t)h->N * (o2 + c)] -= v;
+                h->loss_nonzero = true;
+            }
+        }
+    h->binary = true;
+    h->binary_dirty = true;
+    return DTX_OK;
+}
+
+int dtx_set_binary_domain(dtx_handle* h, int64_t count, const int64_t* domain) {
+    if (!h) return DTX_ERR_INVALID;
+    dfree(h, &h->d_active, h->C);
+    if (count < 0) return DTX_OK;
+    std::vector<uint8_t> act((size_t)h->C, 0);
+    for (int64_t d = 0; d < count; ++d) {
+        if (domain[d] < 0 || domain[d] >= h->C)
+            return fail(h, DTX_ERR_INVALID, "domain cell %lld out of range",
+                        (long long)domain[d]);
+        act[(size_t)domain[d]] = 1;
+    }
+    int rc = dalloc(h, &h->d_active, h->C);
+    if (rc) return rc;
+    return upload(h, h->d_active, act.data(), h->C);
+}
+
+int dtx_set_emission_blocks(dtx_handle* h, const double* blocks) {
+    if (!h) return DTX_ERR_INVALID;
+    if (!blocks) return fail(h, DTX_ERR_INVALID, "null emission blocks");
+    const int64_t count = (int64_t)h->C * h->N * h->N;
+    if (!h->d_emi) {
+        int rc = dalloc(h, &h->d_emi, count);
+        if (rc) return rc;
+    }
+    int rc = upload(h, h->d_emi, blocks, count);
+    if (rc) return rc;
+    h->emission = true;
+    return DTX_OK;
+}
+
+int dtx_set_emission_dense(dtx_handle* h, int64_t n, const double* M) {
+    if (!h) return DTX_ERR_INVALID;
+    if (n != h->n) return fail(h, DTX_ERR_INVALID, "M_Emi must be n x n with n = %lld",
+                               (long long)h->n);
+    if (!M) return fail(h, DTX_ERR_INVALID, "null M_Emi");
+    const int N = h->N, C = h->C;
+    std::vector<double> blocks((size_t)C * N * N);
+    for (int64_t j = 0; j < n; ++j) {
+        const int cj = (int)(j / N);
+        for (int64_t i = 0; i < n; ++i) {
+            const double v = M[i + n * j];
+            const int ci = (int)(i / N);
+            if (ci != cj) {
+                if (v != 0.0)
+                    return fail(h, DTX_ERR_UNSUPPORTED,
+                                "M_Emi has a non-zero outside the per-cell diagonal blocks "
+                                "(row %lld, column %lld)", (long long)i + 1, (long long)j + 1);
+                continue;
+            }
+            blocks[(size_t)ci * N * N + (size_t)(i % N) + (size_t)N * (j % N)] = v;
+        }
+    }
+    return dtx_set_emission_blocks(h, blocks.data());
+}
+
+int dtx_set_vectors(dtx_handle* h, const double* Ap, const double* Vol, const double* f_init,
+                    const double* df_inj) {
+    if (!h) return DTX_ERR_INVALID;
+    if (!Ap || !Vol || !f_init || !df_inj) return fail(h, DTX_ERR_INVALID, "null vector");
+    const int64_t n = h->n;
+    std::vector<double> inv((size_t)n);
+    for (int64_t i = 0; i < n; ++i) inv[(size_t)i] = 1.0 / Ap[i];  // 1 ./ Ap_Flux
+    int rc = upload(h, h->d_invap, inv.data(), n);
+    if (!rc) rc = upload(h, h->d_vol, Vol, h->C);
+    if (!rc) rc = upload(h, h->d_finit, f_init, n);
+    if (!rc) rc = upload(h, h->d_dfinj, df_inj, n);
+    if (rc) return rc;
+    h->vectors_set = true;
+    return dtx_reset(h);
+}
+
+int dtx_set_moment_weights(dtx_handle* h, const double* w) {
+    if (!h) return DTX_ERR_INVALID;
+    if (!w) return fail(h, DTX_ERR_INVALID, "null weights");
+    if (!h->d_eweight) {
+        int rc = dalloc(h, &h->d_eweight, h->N);
+        if (rc) return rc;
+    }
+    return upload(h, h->d_eweight, w, h->N);
+}
+
+int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
+    if (!h) return DTX_ERR_INVALID;
+    h->hch.clear();
+    h->loss_host.clear();
+    release_binary_device(h);
+    dfree(h, &h->d_loss, (int64_t)h->N * h->N);
+    int rc = dalloc(h, &h->d_loss, (int64_t)h->N * h->N);
+    if (rc) return rc;
+    HIP_CHECK(h, launch_fill_zero(h->d_loss, (int64_t)h->N * h->N, h->stream));
+    uint64_t stream = 0;
+    for (int a = 0; a < h->S; ++a)
+        for (int b = a; b < h->S; ++b) {
+            const int outs[2] = {a, b};
+            const int nout = (a == b) ? 1 : 2;
+            for (int q = 0; q < nout; ++q) {
+                const int so = outs[q];
+                const uint64_t key = splitmix_key(seed, stream++);
+                HostChannel hc;
+                hc.s_out = so;
+                hc.s1 = a;
+                hc.s2 = b;
+                hc.kind = (a == b) ? CH_TRI : CH_RECT;
+                hc.No = h->sp_size[so];
+                hc.N1 = h->sp_size[a];
+                hc.N2 = h->sp_size[b];
+                build_pairs(h, hc);
+                PackedChannel pc;
+                pc.s_out = so;
+                pc.s1 = a;
+                pc.s2 = b;
+                pc.kind = hc.kind;
+                pc.No = hc.No;
+                pc.N1 = hc.N1;
+                pc.N2 = hc.N2;
+                pc.npairs = hc.npairs;
+                pc.npairs_pad = hc.npairs_pad;
+                pc.nktiles = (hc.No + TK - 1) / TK;
+                pc.a_bytes = packed_elems(h->precision, pc.npairs_pad, pc.nktiles) *
+                             elem_bytes(h->precision);
+                hipError_t e = hipMalloc(&pc.A, (size_t)pc.a_bytes);
+                if (e != hipSuccess)
+                    return fail(h, DTX_ERR_OOM, "hipMalloc(%lld) for a synthetic channel: %s",
+                                (long long)pc.a_bytes, hipGetErrorString(e));
+                h->device_bytes += pc.a_bytes;
+                h->allocations.push_back(pc.A);
+                e = hipMalloc((void**)&pc.pairs, (size_t)pc.npairs_pad * 4);
+                if (e != hipSuccess) {
+                    h->chans.push_back(pc);
+                    return fail(h, DTX_ERR_OOM, "hipMalloc for a pair table: %s",
+                                hipGetErrorString(e));
+                }
+                h->device_bytes += pc.npairs_pad * 4;
+                h->allocations.push_back(pc.pairs);
+                h->chans.push_back(pc);
+                HIP_CHECK(h, hipMemcpy(pc.pairs, hc.pairs.data(), (size_t)pc.npairs_pad * 4,
+                                       hipMemcpyHostToDevice));
+                HIP_CHECK(h, launch_synth_gain_packed(h->precision, pc.A, pc.pairs,
+                                                      pc.npairs_pad, pc.No, pc.nktiles,
+                                                      pc.kind, pc.N1, pc.N2, key, scale,
+                                                      h->stream));
+                // loss of the species whose particle leaves: out a -> loss on a (i), out b ->
+                // loss on b (j); a == b -> one loss over (i, j)
+                const int oo = h->sp_off[so];
+                const int loss_on_first = (a == b) || (so == a);
+                HIP_CHECK(h, launch_synth_loss(h->d_loss, h->N, h->sp_off[a], hc.N1,
+                                               h->sp_off[b], hc.N2, hc.No, key, scale,
+                                               loss_on_first, h->stream));
+                (void)oo;
+            }
+        }
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    h->synth_binary = true;
+    h->binary = true;
+    h->binary_dirty = true;
+    return DTX_OK;
+}
+
+int dtx_reset(dtx_handle* h) {
+    if (!h) return DTX_ERR_INVALID;
+    if (!h->vectors_set) return fail(h, DTX_ERR_STATE, "dtx_set_vectors not called");
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    int rc = drain_status(h);
+    if (rc) return rc;
+    h->cur = 0;
+    HIP_CHECK(h, hipMemcpyAsync(h->d_f[0], h->d_finit, (size_t)h->n * 8,
+                                hipMemcpyDeviceToDevice, h->stream));
+    rc = refresh_fT(h);
+    if (rc) return rc;
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    h->step_counter = 0;
+    h->reported = 0;
+    return DTX_OK;
+}
+
+int dtx_get_f(dtx_handle* h, double* f) {
+    if (!h || !f) return DTX_ERR_INVALID;
+    HIP_CHECK(h, hipMemcpyAsync(f, h->d_f[h->cur], (size_t)h->n * 8, hipMemcpyDeviceToHost,
+                                h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return drain_status(h);
+}
+
+int dtx_set_f(dtx_handle* h, const double* f) {
+    if (!h || !f) return DTX_ERR_INVALID;
+    int rc = upload(h, h->d_f[h->cur], f, h->n);
+    if (rc) return rc;
+    rc = refresh_fT(h);
+    if (rc) return rc;
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return DTX_OK;
+}
+
+int dtx_moments(dtx_handle* h, double* number, double* energy) {
+    if (!h || !number || !energy) return DTX_ERR_INVALID;
+    std::vector<double> m((size_t)h->C * h->S * 2);
+    HIP_CHECK(h, hipMemcpyAsync(m.data(), h->d_moments, m.size() * 8, hipMemcpyDeviceToHost,
+                                h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < (size_t)h->C * h->S; ++i) {
+        number[i] = m[2 * i];
+        energy[i] = m[2 * i + 1];
+    }
+    return drain_status(h);
+}
+
+int dtx_synchronize(dtx_handle* h) {
+    if (!h) return DTX_ERR_INVALID;
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return drain_status(h);
+}
+
+int dtx_step(dtx_handle* h, double t_start, double t_stop, double dt, int32_t verbose,
+             double* dt_out, int32_t* save_out, dtx_step_stats* stats) {
+    if (!h) return DTX_ERR_INVALID;
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    int save;
+    if (t_start + dt >= t_stop) {  // src/SteppingMethods.jl:18-23
+        dt = t_stop - t_start;
+        save = 1;
+    } else {
+        save = 0;
+    }
+    const int64_t idx = h->step_counter;
+    rc = enqueue_step(h, t_start, dt, verbose, save);
+    if (rc) return rc;
+    if (dt_out) *dt_out = dt;
+    if (save_out) *save_out = save;
+    if (stats) {
+        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+        double st[STATUS_WORDS];
+        HIP_CHECK(h, hipMemcpy(st, h->d_status + (idx % STATUS_RING) * STATUS_WORDS,
+                               sizeof st, hipMemcpyDeviceToHost));
+        fill_stats(st, stats);
+        rc = drain_status(h);
+        if (rc) return rc;
+    }
+    return DTX_OK;
+}
+
+int dtx_solve(dtx_handle* h, double dt_initial, int64_t n_save, const double* t_save,
+              int32_t verbose, double* f_out, double* t_out, int64_t* n_steps) {
+    if (!h) return DTX_ERR_INVALID;
+    if (n_save < 1 || !t_save || !f_out || !t_out)
+        return fail(h, DTX_ERR_INVALID, "invalid Solve arguments");
+    int rc = ensure_ready(h);
+    if (rc) return rc;
+    rc = dtx_reset(h);  // src/Solver.jl:22
+    if (rc) return rc;
+    const int64_t n = h->n;
+    double* staging = nullptr;
+    HIP_CHECK(h, hipHostMalloc((void**)&staging, (size_t)n * n_save * 8, hipHostMallocDefault));
+    HIP_CHECK(h, hipMemcpyAsync(staging, h->d_finit, (size_t)n * 8, hipMemcpyDeviceToHost,
+                                h->stream));
+    t_out[0] = t_save[0];
+    double dt = dt_initial;
+    int64_t steps = 0;
+    if (!(dt > 0.0) || !std::isfinite(dt))
+        rc = fail(h, DTX_ERR_INVALID, "dt_initial must be positive and finite");
+    for (int64_t i = 1; i < n_save && rc == DTX_OK; ++i) {  // src/Solver.jl:27-44
+        double t_start = t_save[i - 1];
+        const double t_stop = t_save[i];
+        int save = 0;
+        // The clipped dt carries over between intervals (src/Solver.jl:35).  When rounding makes
+        // it collapse, the reference would loop (practically) forever; refuse instead.
+        if (!(dt > 0.0) || (t_stop - t_start) / dt > kMaxIntervalSteps) {
+            rc = fail(h, DTX_ERR_INVALID,
+                      "Solve: dt = %.17g carried into save interval %lld [%.17g, %.17g] would "
+                      "need more than %.0e steps", dt, (long long)i, t_start, t_stop,
+                      kMaxIntervalSteps);
+            break;
+        }
+        while (!save) {
+            if (t_start + dt >= t_stop) {
+                dt = t_stop - t_start;
+                save = 1;
+            }
+            rc = enqueue_step(h, t_start, dt, verbose, save);
+            if (rc) break;
+            ++steps;
+            t_start += dt;
+        }
+        if (rc) break;
+        hipError_t e = hipMemcpyAsync(staging + i * n, h->d_f[h->cur], (size_t)n * 8,
+                                      hipMemcpyDeviceToHost, h->stream);
+        if (e != hipSuccess) rc = fail(h, DTX_ERR_HIP, "snapshot copy: %s", hipGetErrorString(e));
+        t_out[i] = t_stop;
+    }
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (rc == DTX_OK && e != hipSuccess)
+        rc = fail(h, DTX_ERR_HIP, "solve: %s", hipGetErrorString(e));
+    if (rc == DTX_OK) {
+        memcpy(f_out, staging, (size_t)n * n_save * 8);
+        rc = drain_status(h);
+    }
+    (void)hipHostFree(staging);
+    if (n_steps) *n_steps = steps;
+    return rc;
+}
+
+int dtx_recent_stats(dtx_handle* h, dtx_step_stats* out, int32_t max_steps, int32_t* count) {
+    if (!h || !out || !count) return DTX_ERR_INVALID;
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> ring((size_t)STATUS_RING * STATUS_WORDS);
+    HIP_CHECK(h, hipMemcpy(ring.data(), h->d_status, ring.size() * 8, hipMemcpyDeviceToHost));
+    int64_t avail = std::min<int64_t>(h->step_counter, STATUS_RING);
+    int64_t take = std::min<int64_t>(avail, max_steps);
+    for (int64_t q = 0; q < take; ++q) {
+        int64_t s = h->step_counter - take + q;
+        fill_stats(ring.data() + (s % STATUS_RING) * STATUS_WORDS, &out[q]);
+    }
+    *count = (int32_t)take;
+    return drain_status(h);
+}
+
+int64_t dtx_device_bytes(const dtx_handle* h) { return h ? h->device_bytes : 0; }
+
+int dtx_work(const dtx_handle* hc, dtx_work_info* out) {
+    dtx_handle* h = const_cast<dtx_handle*>(hc);
+    if (!h || !out) return DTX_ERR_INVALID;
+    if (h->binary && h->binary_dirty && h->vectors_set) {
+        int rc = ensure_ready(h);
+        if (rc) return rc;
+    }
+    memset(out, 0, sizeof *out);
+    const double C = h->C, N = h->N;
+    out->pairs = h->pairs_total;
+    out->bin_launch_items = h->nitems;
+    out->gain_bytes = h->gain_bytes;
+    if (h->binary) {
+        out->bin_flops = h->bin_flops + (h->d_loss ? 2.0 * N * N * C : 0.0);
+        // packed gain once + pair tables + f reads (fT) + partial writes
+        double pair_bytes = 0;
+        for (auto& c : h->chans) pair_bytes += c.npairs_pad * 4.0;
+        out->bin_bytes = h->gain_bytes + pair_bytes + C * N * 8.0 +
+                         (double)h->max_slots * C * N * 8.0;
+    }
+    // update: f, f_out, fT, invAp, df_inj, ELL (col+val), partial reads, loss (L2), moments
+    double ell_bytes = 0;
+    for (int w = 0; w < 3; ++w) ell_bytes += (double)h->ell[w].width * h->ell[w].n * 12.0;
+    out->update_bytes = h->n * 8.0 * 5 + ell_bytes + (double)h->max_slots * C * N * 8.0;
+    return DTX_OK;
+}
+
+int dtx_kernel_timing(dtx_handle* h, int32_t enable, int32_t reset, dtx_kernel_times* out) {
+    if (!h) return DTX_ERR_INVALID;
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    for (auto& ev : h->events) {
+        float ms = 0;
+        HIP_CHECK(h, hipEventElapsedTime(&ms, ev.start, ev.stop));
+        if (ev.cls == 0) {
+            h->times.bin_ms += ms;
+            h->times.bin_launches++;
+        } else {
+            h->times.update_ms += ms;
+            h->times.update_launches++;
+        }
+        (void)hipEventDestroy(ev.start);
+        (void)hipEventDestroy(ev.stop);
+    }
+    h->events.clear();
+    if (out) *out = h->times;
+    if (reset) h->times = dtx_kernel_times{};
+    h->timing = enable != 0;
+    return drain_status(h);
+}
+
+}  // extern "C"

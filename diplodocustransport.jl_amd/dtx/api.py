@@ -1,0 +1,302 @@
+"""Python mirror of DiplodocusTransport.jl's stepping API on the HIP backend.
+
+Same names, argument meaning and error behaviour as the reference's Julia surface:
+
+* ``ForwardEulerStruct(PhaseSpace, Initial, Injection, BinM, EmiM, FluxM; Adaptive)``
+  (src/SteppingStructs.jl:33) and ``ForwardSymplecticEulerStruct`` (:142),
+* the functor ``method(t_start, t_stop, dt, Verbose) -> (dt, save)``
+  (src/SteppingMethods.jl:12, 120),
+* ``Solve(method, dt_initial, t_save; Verbose)`` (src/Solver.jl:1) returning ``OutputStruct``
+  (src/StructsAndDictionaries.jl:362-377),
+* the matrix containers BinaryMatricesStruct / EmissionMatricesStruct / FluxMatricesStruct
+  (src/StructsAndDictionaries.jl:316-359),
+* number N^0 and energy T^00 moments (src/DistributionMoments.jl:8-42, 94-212).
+
+Every numerical call goes through libdtx.so (the C ABI of include/dtx.h); this module only
+marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .grids import PhaseSpace
+
+
+class BackendType:
+    pass
+
+
+class CPUBackend(BackendType):
+    """Present for API parity; this package only implements the HIP backend."""
+
+
+class HIPBackend(BackendType):
+    """The MI355X backend: state and operators live in HBM, steps run as gfx950 kernels."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+
+
+@dataclass
+class BinaryMatricesStruct:
+    """M_Bin as the (N^2 x N) column-major tensor (any array whose Fortran-order memory is
+    M_Bin, e.g. a 3-D M[k,i,j] Fortran array), and/or the channel form (gains, losses).
+    Binary_list non-empty <=> binary interactions are on (src/SteppingStructs.jl:46)."""
+    M_Bin: Optional[np.ndarray] = None
+    Binary_list: list = field(default_factory=list)
+    Domain: Optional[list] = None
+    gains: list = field(default_factory=list)     # dtx.synth.GainChannel
+    losses: list = field(default_factory=list)    # dtx.synth.LossChannel
+    synth_seed: Optional[int] = None              # generate on the device instead
+    synth_scale: float = 1.0
+
+
+@dataclass
+class EmissionMatricesStruct:
+    """M_Emi dense n x n (column-major, block diagonal per cell) or per-cell blocks."""
+    M_Emi: Optional[np.ndarray] = None
+    Emission_list: list = field(default_factory=list)
+    blocks: Optional[np.ndarray] = None
+
+
+@dataclass
+class FluxMatricesStruct:
+    """Ap_Flux (n), X_Flux / P_Flux as (colptr, rowval, nzval) 1-based CSC tuples, Vol (C)."""
+    Ap_Flux: np.ndarray
+    X_Flux: Optional[tuple]
+    P_Flux: Optional[tuple]
+    Vol: np.ndarray
+
+
+@dataclass
+class OutputStruct:
+    f: list
+    t: np.ndarray
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _csc_args(csc):
+    colptr, rowval, nzval = csc
+    colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+    rowval = np.ascontiguousarray(rowval, dtype=np.int64)
+    nzval = _f64(nzval)
+    return colptr, rowval, nzval
+
+
+class _SteppingStruct:
+    METHOD = _lib.METHOD_FORWARD_EULER
+
+    def __init__(self, PhaseSpace: PhaseSpace, Initial, Injection, BinM: BinaryMatricesStruct,
+                 EmiM: EmissionMatricesStruct, FluxM: FluxMatricesStruct,
+                 Adaptive: bool = False, Backend: BackendType = None,
+                 Precision: str = "fp64"):
+        if Backend is None:
+            Backend = HIPBackend()
+        if not isinstance(Backend, HIPBackend):
+            raise ValueError("Backend type not recognized.")  # src/SteppingStructs.jl:74
+        self.lib = _lib.load()
+        self.PhaseSpace = PhaseSpace
+        self.Adaptive = Adaptive
+        self.Implicit = False
+        self.dt0 = float(PhaseSpace.dt0)
+        self.Binary_Interactions = bool(BinM.Binary_list)
+        self.Emission_Interactions = bool(EmiM.Emission_list)
+        self.Bin_Domain = BinM.Domain
+        self.f_init = _f64(Initial).copy()
+        self.precision = Precision
+        n = PhaseSpace.n
+        if self.f_init.size != n:
+            raise ValueError(f"Initial has length {self.f_init.size}, expected {n}")
+        self._h = C.c_void_p()
+        cfg = _lib.make_config(PhaseSpace, Precision, self.METHOD, Backend.device)
+        _lib.check(self.lib.dtx_create(C.byref(cfg), C.byref(self._h)), None, "dtx_create")
+        h = self._h
+        # fluxes
+        if self.METHOD == _lib.METHOD_FORWARD_EULER:
+            F = self._flux_sum(n, FluxM)
+            if F is not None:
+                cp, rv, nz = _csc_args(F)
+                _lib.check(self.lib.dtx_set_flux_csc(h, _lib.FLUX_F, n, _lib.i64ptr(cp),
+                                                     _lib.i64ptr(rv), _lib.dptr(nz)), h,
+                           "dtx_set_flux_csc")
+        else:
+            for which, M in ((_lib.FLUX_X, FluxM.X_Flux), (_lib.FLUX_P, FluxM.P_Flux)):
+                if M is not None:
+                    cp, rv, nz = _csc_args(M)
+                    _lib.check(self.lib.dtx_set_flux_csc(h, which, n, _lib.i64ptr(cp),
+                                                         _lib.i64ptr(rv), _lib.dptr(nz)), h,
+                               "dtx_set_flux_csc")
+        # binary
+        if self.Binary_Interactions:
+            if BinM.synth_seed is not None:
+                _lib.check(self.lib.dtx_synth_binary(h, int(BinM.synth_seed),
+                                                     float(BinM.synth_scale)), h,
+                           "dtx_synth_binary")
+            elif BinM.M_Bin is not None:
+                M = np.asarray(BinM.M_Bin, dtype=np.float64)
+                Mf = M.reshape(-1, order="F") if M.flags.f_contiguous else \
+                    np.asfortranarray(M).reshape(-1, order="F")
+                N = PhaseSpace.n_momentum
+                if Mf.size != N ** 3:
+                    raise ValueError("M_Bin is not the correct size")  # src/SteppingMethods.jl:349
+                _lib.check(self.lib.dtx_set_binary_dense(h, N, _lib.dptr(Mf)), h,
+                           "dtx_set_binary_dense")
+            else:
+                for g in BinM.gains:
+                    G = np.asfortranarray(g.G, dtype=np.float64).reshape(-1, order="F")
+                    _lib.check(self.lib.dtx_add_binary_gain(h, g.s_out, g.s1, g.s2,
+                                                            _lib.dptr(G)), h,
+                               "dtx_add_binary_gain")
+                for l in BinM.losses:
+                    L = np.asfortranarray(l.L, dtype=np.float64).reshape(-1, order="F")
+                    _lib.check(self.lib.dtx_add_binary_loss(h, l.s1, l.s2, _lib.dptr(L)), h,
+                               "dtx_add_binary_loss")
+            if BinM.Domain is not None:
+                dom = np.ascontiguousarray(BinM.Domain, dtype=np.int64)
+                _lib.check(self.lib.dtx_set_binary_domain(h, dom.size, _lib.i64ptr(dom)), h,
+                           "dtx_set_binary_domain")
+        # emission
+        if self.Emission_Interactions:
+            if EmiM.blocks is not None:
+                B = np.ascontiguousarray(EmiM.blocks, dtype=np.float64)
+                _lib.check(self.lib.dtx_set_emission_blocks(h, _lib.dptr(B)), h,
+                           "dtx_set_emission_blocks")
+            else:
+                M = np.asfortranarray(EmiM.M_Emi, dtype=np.float64).reshape(-1, order="F")
+                _lib.check(self.lib.dtx_set_emission_dense(h, n, _lib.dptr(M)), h,
+                           "dtx_set_emission_dense")
+        ap, vol = _f64(FluxM.Ap_Flux), _f64(FluxM.Vol)
+        inj = _f64(Injection)
+        _lib.check(self.lib.dtx_set_vectors(h, _lib.dptr(ap), _lib.dptr(vol),
+                                            _lib.dptr(self.f_init), _lib.dptr(inj)), h,
+                   "dtx_set_vectors")
+        w = _f64(PhaseSpace.energy_weights())
+        _lib.check(self.lib.dtx_set_moment_weights(h, _lib.dptr(w)), h,
+                   "dtx_set_moment_weights")
+
+    @staticmethod
+    def _flux_sum(n, FluxM):
+        from .synth import csc_add
+        X, P = FluxM.X_Flux, FluxM.P_Flux
+        if X is None and P is None:
+            return None
+        if X is None:
+            return P
+        if P is None:
+            return X
+        return csc_add(n, X, P)  # F_Flux = X_Flux + P_Flux (src/SteppingStructs.jl:60)
+
+    # ---- functor ----------------------------------------------------------------------
+    def __call__(self, t_start, t_stop, dt, Verbose: int, stats: bool = False):
+        dt_out = C.c_double()
+        save = C.c_int32()
+        st = _lib.StepStats() if stats else None
+        _lib.check(self.lib.dtx_step(self._h, float(t_start), float(t_stop), float(dt),
+                                     int(Verbose), C.byref(dt_out), C.byref(save),
+                                     C.byref(st) if st is not None else None), self._h,
+                   "dtx_step")
+        if stats:
+            return dt_out.value, bool(save.value), st
+        return dt_out.value, bool(save.value)
+
+    # ---- state --------------------------------------------------------------------------
+    @property
+    def f(self) -> np.ndarray:
+        out = np.empty(self.PhaseSpace.n)
+        _lib.check(self.lib.dtx_get_f(self._h, _lib.dptr(out)), self._h, "dtx_get_f")
+        return out
+
+    @f.setter
+    def f(self, value):
+        v = _f64(value)
+        _lib.check(self.lib.dtx_set_f(self._h, _lib.dptr(v)), self._h, "dtx_set_f")
+
+    def reset(self):
+        _lib.check(self.lib.dtx_reset(self._h), self._h, "dtx_reset")
+
+    def synchronize(self):
+        _lib.check(self.lib.dtx_synchronize(self._h), self._h, "dtx_synchronize")
+
+    def moments(self):
+        """(number, energy), each (C, num_species): N^0 and T^00 of the current f."""
+        ps = self.PhaseSpace
+        num = np.empty(ps.n_space * ps.num_species)
+        en = np.empty_like(num)
+        _lib.check(self.lib.dtx_moments(self._h, _lib.dptr(num), _lib.dptr(en)), self._h,
+                   "dtx_moments")
+        return num.reshape(ps.n_space, ps.num_species), en.reshape(ps.n_space, ps.num_species)
+
+    def recent_stats(self, count: int):
+        arr = (_lib.StepStats * count)()
+        got = C.c_int32()
+        _lib.check(self.lib.dtx_recent_stats(self._h, arr, count, C.byref(got)), self._h,
+                   "dtx_recent_stats")
+        return list(arr)[:got.value]
+
+    def work(self) -> _lib.WorkInfo:
+        w = _lib.WorkInfo()
+        _lib.check(self.lib.dtx_work(self._h, C.byref(w)), self._h, "dtx_work")
+        return w
+
+    def kernel_timing(self, enable: bool, reset: bool = True) -> _lib.KernelTimes:
+        t = _lib.KernelTimes()
+        _lib.check(self.lib.dtx_kernel_timing(self._h, int(enable), int(reset), C.byref(t)),
+                   self._h, "dtx_kernel_timing")
+        return t
+
+    def device_bytes(self) -> int:
+        return int(self.lib.dtx_device_bytes(self._h))
+
+    def solve_native(self, dt_initial, t_save, Verbose=1):
+        """Solve entirely inside the library (no per-step host synchronisation)."""
+        t_save = _f64(t_save)
+        n = self.PhaseSpace.n
+        fout = np.empty((t_save.size, n))
+        tout = np.empty(t_save.size)
+        steps = C.c_int64()
+        _lib.check(self.lib.dtx_solve(self._h, float(dt_initial), t_save.size,
+                                      _lib.dptr(t_save), int(Verbose), _lib.dptr(fout),
+                                      _lib.dptr(tout), C.byref(steps)), self._h, "dtx_solve")
+        return OutputStruct([fout[i] for i in range(t_save.size)], tout), steps.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.dtx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ForwardEulerStruct(_SteppingStruct):
+    METHOD = _lib.METHOD_FORWARD_EULER
+
+
+class ForwardSymplecticEulerStruct(_SteppingStruct):
+    METHOD = _lib.METHOD_FORWARD_SYMPLECTIC_EULER
+
+
+def Solve(method: _SteppingStruct, dt_initial: float, t_save, progress: bool = False,
+          fileName=None, fileLocation=None, Verbose: int = 1) -> OutputStruct:
+    """Solve (src/Solver.jl:1-64): the time loop runs inside libdtx (dtx_solve) with the same
+    dt clipping and carry-over; the JLD2 write (:53-59) is not part of this package."""
+    if fileName is not None:
+        raise NotImplementedError("SolutionFileSave (JLD2) is not provided by this package")
+    out, _ = method.solve_native(dt_initial, t_save, Verbose)
+    return out
+
+
+def FourFlow_N0(state: np.ndarray, ps: PhaseSpace, species: int, x_idx=1, y_idx=1, z_idx=1):
+    """N^0 of FourFlow (src/DistributionMoments.jl:33) for a host state vector."""
+    return float(np.sum(ps.view(state, species, x_idx, y_idx, z_idx)))

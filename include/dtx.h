@@ -1,0 +1,251 @@
+/*
+ * dtx.h -- C ABI of the MI355X-native explicit-Euler stepping library (libdtx.so).
+ *
+ * This is the drop-in boundary for DiplodocusTransport.jl's hot path.  The Julia side keeps its
+ * API (ForwardEulerStruct / ForwardSymplecticEulerStruct, the functor call, Solve, the moments)
+ * and replaces the body of the step with `ccall`s into these entry points.  All arguments are
+ * plain C types; host arrays are owned by the caller and copied during the call; no host pointer
+ * is retained.  Device memory belongs to the handle and is released by dtx_destroy.
+ *
+ * Arrays follow Julia's layouts: column-major dense matrices, SparseMatrixCSC with 1-based Int64
+ * colptr/rowval, state vectors indexed as in GlobalIndices_To_StateIndex
+ * (src/GlobalToStateIndices.jl:7-25): cell-major, species blocks in name_list order, px fastest.
+ *
+ * Errors: every int-returning call returns DTX_OK (0) or a negative DTX_ERR_* code; the message
+ * is available from dtx_last_error(h) (or dtx_last_error(NULL) when creation failed).  As in the
+ * reference, a non-finite df or a CFL number above one is not an error: it is reported in the
+ * step statistics and printed (src/SteppingMethods.jl:48-50, 87-93).
+ *
+ * Threading: a handle is not thread-safe; its calls are ordered on one HIP stream.  Different
+ * handles may be used from different threads.  The dtx_server_* API is thread-safe.
+ */
+#ifndef DTX_H
+#define DTX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTX_MAX_SPECIES 16
+
+enum dtx_status {
+    DTX_OK = 0,
+    DTX_ERR_INVALID = -1,     /* bad argument or shape                                   */
+    DTX_ERR_OOM = -2,         /* device or host allocation failed                        */
+    DTX_ERR_HIP = -3,         /* a HIP runtime call failed                               */
+    DTX_ERR_STATE = -4,       /* call out of order (e.g. stepping before set_vectors)    */
+    DTX_ERR_UNSUPPORTED = -5, /* valid input the library does not support                */
+    DTX_ERR_BUSY = -6         /* server: request table or memory budget full             */
+};
+
+/* Arithmetic of the binary contraction.  f, df, the Euler update, losses and the moments are
+ * FP64 in every mode. */
+enum dtx_precision {
+    DTX_PREC_FP64 = 0,   /* FP64 MFMA                                                      */
+    DTX_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulation within a pair block, FP64 across   */
+    DTX_PREC_BF16X3 = 2  /* split-BF16 hi/lo, 3 MFMA products, same accumulation as FP32    */
+};
+
+enum dtx_method {
+    DTX_METHOD_FORWARD_EULER = 0,            /* src/SteppingMethods.jl:12-107   */
+    DTX_METHOD_FORWARD_SYMPLECTIC_EULER = 1  /* src/SteppingMethods.jl:120-243  */
+};
+
+enum dtx_flux_which {
+    DTX_FLUX_F = 0,  /* F_Flux = X_Flux + P_Flux (ForwardEuler, src/SteppingStructs.jl:60)  */
+    DTX_FLUX_X = 1,  /* X_Flux (space; symplectic, src/SteppingStructs.jl:170)             */
+    DTX_FLUX_P = 2   /* P_Flux (momentum; symplectic, src/SteppingStructs.jl:171)          */
+};
+
+typedef struct dtx_handle dtx_handle;
+
+/* Shape and mode of one simulation.  Replaces the globals the reference reads from Main
+ * (Backend, Precision, px/py/pz_num_list, x/y/z_num; src/StructsAndDictionaries.jl:70-121,
+ * src/SteppingStructs.jl:35-36) and PhaseSpace.Time.dt0 (src/StructsAndDictionaries.jl:55). */
+typedef struct {
+    int32_t num_species;
+    int32_t px_num[DTX_MAX_SPECIES];
+    int32_t py_num[DTX_MAX_SPECIES];
+    int32_t pz_num[DTX_MAX_SPECIES];
+    int32_t x_num, y_num, z_num;
+    int32_t precision;   /* enum dtx_precision */
+    int32_t method;      /* enum dtx_method    */
+    int32_t device;      /* HIP device ordinal */
+    double dt0;
+} dtx_config;
+
+/* Per-step diagnostics of the functor (src/SteppingMethods.jl:48-94, 175-229). */
+typedef struct {
+    int32_t finite;        /* isfinite(sum(df))                     (ForwardEuler only; 1 else) */
+    int32_t cr_evaluated;  /* Verbose in 1..3 and sum(f) != 0                                   */
+    int32_t save;          /* the functor's `save` result                                       */
+    int32_t step_index;    /* 0-based step counter since dtx_reset                              */
+    double cr;             /* Cr                                                                */
+    double cr_bin;         /* Cr_Bin   (Verbose == 3)                                           */
+    double cr_emi;         /* Cr_Emi   (Verbose == 3)                                           */
+    double cr_flux;        /* Cr_Flux (ForwardEuler) / Cr_PFlux (symplectic) (Verbose == 3)      */
+    double cr_momentum;    /* Cr_Momentum (symplectic, Verbose == 3)                            */
+    double cr_space;       /* Cr_Space    (symplectic, Verbose == 3)                            */
+    double dt;             /* dt used (after the clip, src/SteppingMethods.jl:18-23)            */
+    double t_start;
+    double sum_f;          /* sum(f) before the update                                          */
+} dtx_step_stats;
+
+/* ---- lifetime -------------------------------------------------------------------------- */
+
+/* Allocate a handle (replaces the device placement in ForwardEulerStruct /
+ * ForwardSymplecticEulerStruct, src/SteppingStructs.jl:33-100, 142-227). */
+int dtx_create(const dtx_config* cfg, dtx_handle** out);
+/* Release all device memory of the handle. */
+void dtx_destroy(dtx_handle* h);
+/* Message of the last failed call on h; h == NULL gives the last failure of dtx_create. */
+const char* dtx_last_error(const dtx_handle* h);
+
+/* ---- operators (inputs of the struct constructors) ------------------------------------- */
+
+/* A flux matrix as Julia's SparseMatrixCSC{Float64,Int64}: n x n, 1-based colptr (n+1) and
+ * rowval (nnz), nzval (nnz).  Stored on the device as column-sorted ELL rows.
+ * Replaces CuSparseMatrixCSC(FluxM.X_Flux + FluxM.P_Flux) (src/SteppingStructs.jl:68) and the
+ * symplectic X_Flux / P_Flux uploads (:188-189).  A method that needs an absent matrix treats it
+ * as zero. */
+int dtx_set_flux_csc(dtx_handle* h, int32_t which, int64_t n, const int64_t* colptr,
+                     const int64_t* rowval, const double* nzval);
+
+/* The monolithic binary tensor BinM.M_Bin: (N^2 x N) column-major Float64 with
+ * M_Bin[(i-1)N+k, j] = M[k,i,j] (src/Collisions/BuildBinaryMatrices.jl:54, 181-196, 261-276).
+ * Replaces CuArray(BinM.M_Bin) (src/SteppingStructs.jl:66). */
+int dtx_set_binary_dense(dtx_handle* h, int64_t N, const double* M_Bin);
+
+/* Channel form of the same tensor, for sizes where the monolithic array does not fit.
+ * Gain: dB[k] += sum_{i in s1, j in s2} G[k + No*(i + N1*j)] f_i f_j for k in s_out, which is
+ * what GainMatrix_to_M_Bin! contributes (G/2 at (k,i,j) and (k,j,i),
+ * src/Collisions/BuildBinaryMatrices.jl:114-204).  Species are 0-based. */
+int dtx_add_binary_gain(dtx_handle* h, int32_t s_out, int32_t s1, int32_t s2, const double* G);
+/* Loss: dB[a] -= f_a * sum_{c in s2} L[a + N1*c] f_c for a in s1, which is what
+ * LossMatrix_to_M_Bin! contributes (-L/2 at (a,a,c) and (a,c,a), :206-284). */
+int dtx_add_binary_loss(dtx_handle* h, int32_t s1, int32_t s2, const double* L);
+/* BinM.Domain: 0-based cell offsets that carry binary interactions; count < 0 means `nothing`
+ * (all cells).  (src/Collisions/CollisionDomains.jl:6-38, src/SteppingMethods.jl:369) */
+int dtx_set_binary_domain(dtx_handle* h, int64_t count, const int64_t* domain);
+
+/* EmiM.M_Emi: dense n x n column-major Float64 (src/Collisions/BuildEmissionMatrices.jl:43).
+ * It must be block diagonal per cell (the structure BuildEmissionMatrices produces); the
+ * library keeps only the C diagonal N x N blocks and rejects off-block non-zeros.
+ * Replaces CuArray(EmiM.M_Emi) (src/SteppingStructs.jl:67). */
+int dtx_set_emission_dense(dtx_handle* h, int64_t n, const double* M_Emi);
+/* The same operator given directly as C blocks of N x N column-major (cell-major order). */
+int dtx_set_emission_blocks(dtx_handle* h, const double* blocks);
+
+/* FluxM.Ap_Flux (n; the library forms 1 ./ Ap as src/SteppingStructs.jl:55 does), FluxM.Vol
+ * (C), Initial (n, f_init) and Injection (n, df_Inj).  f is set to f_init. */
+int dtx_set_vectors(dtx_handle* h, const double* Ap, const double* Vol, const double* f_init,
+                    const double* df_inj);
+
+/* Per-state T^00 weights E_bar(p) of one cell (N), i.e. Grids.dE_list expanded over (u, phi)
+ * (src/StructsAndDictionaries.jl:259; src/DistributionMoments.jl:145,162).  Optional: without
+ * them the energy moment is reported as 0. */
+int dtx_set_moment_weights(dtx_handle* h, const double* energy_weight);
+
+/* Synthetic workload support: generate lognormal(0,1)*scale gain channels for every species
+ * pair (a <= b) -> (a, b) directly in device memory, with number-conserving losses
+ * (sum_k G[k,i,j] = L[i,j]).  No counterpart in the reference, which reads DiplodocusCollisions
+ * files; used for the benchmark workloads whose tensors are too large to build on the host. */
+int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale);
+
+/* ---- stepping ---------------------------------------------------------------------------- */
+
+/* One functor call `(method)(t_start, t_stop, dt, Verbose) -> (dt, save)`
+ * (src/SteppingMethods.jl:12-107 / 120-243).  Returns the clipped dt and save.  With
+ * stats != NULL the call waits for the step and fills *stats; otherwise it only enqueues the
+ * step (its diagnostics are printed when the handle next synchronises). */
+int dtx_step(dtx_handle* h, double t_start, double t_stop, double dt, int32_t verbose,
+             double* dt_out, int32_t* save_out, dtx_step_stats* stats);
+
+/* Solve(method, dt_initial, t_save; Verbose) (src/Solver.jl:1-64) without the JLD2 write:
+ * resets f to f_init, steps through every save interval without host synchronisation and writes
+ * the n_save states to f_out (n_save x n, row i = output.f[i+1]) and t_out (n_save).  The
+ * clipped dt carries over between intervals as in the reference (:35).  *n_steps receives the
+ * number of steps taken (may be NULL). */
+int dtx_solve(dtx_handle* h, double dt_initial, int64_t n_save, const double* t_save,
+              int32_t verbose, double* f_out, double* t_out, int64_t* n_steps);
+
+/* `@. method.f = method.f_init` (src/Solver.jl:22) and clears the step counter. */
+int dtx_reset(dtx_handle* h);
+/* copyto!(f, method.f) (src/Solver.jl:40) and the reverse, for resume. */
+int dtx_get_f(dtx_handle* h, double* f);
+int dtx_set_f(dtx_handle* h, const double* f);
+/* Number N^0 and energy T^00 of the current f per (cell, species): arrays of C*num_species,
+ * cell-major (src/DistributionMoments.jl:33, 145/162), computed in FP64 by the step kernel. */
+int dtx_moments(dtx_handle* h, double* number, double* energy);
+/* Wait for all enqueued work of h and print pending step diagnostics. */
+int dtx_synchronize(dtx_handle* h);
+/* Statistics of the most recent max_steps steps (oldest first) as kept on the device. */
+int dtx_recent_stats(dtx_handle* h, dtx_step_stats* out, int32_t max_steps, int32_t* count);
+
+/* ---- introspection ----------------------------------------------------------------------- */
+
+/* Bytes of device memory the handle holds (operators, state and scratch). */
+int64_t dtx_device_bytes(const dtx_handle* h);
+/* Algorithmic work of one step: binary FLOPs (math, counted once) and HBM bytes per part. */
+typedef struct {
+    double bin_flops;      /* 2 * No * pairs * cells summed over channels (+ loss GEMV)   */
+    double bin_bytes;      /* packed gain bytes + loss matrix + f reads + partial traffic */
+    double update_bytes;   /* fused update kernel: f, partials, invAp, df_inj, ELL, out   */
+    double gain_bytes;     /* packed gain operator bytes (streamed once per step)         */
+    int64_t pairs;         /* total packed pairs over channels                            */
+    int32_t bin_launch_items;
+} dtx_work_info;
+int dtx_work(const dtx_handle* h, dtx_work_info* out);
+/* Device time of the last synchronised window, per kernel class, in milliseconds (HIP events
+ * on the handle's stream); reset by passing reset != 0. */
+typedef struct {
+    double bin_ms, update_ms;
+    int64_t bin_launches, update_launches;
+} dtx_kernel_times;
+int dtx_kernel_timing(dtx_handle* h, int32_t enable, int32_t reset, dtx_kernel_times* out);
+
+/* ---- serving ----------------------------------------------------------------------------- */
+
+typedef struct dtx_server dtx_server;
+
+typedef struct {
+    int32_t device;
+    int32_t precision;       /* enum dtx_precision used for every request            */
+    int64_t hbm_budget;      /* bytes of device memory the admitted requests may hold  */
+    int32_t max_active;      /* request slots stepped together                        */
+} dtx_server_config;
+
+typedef struct {
+    dtx_config cfg;          /* shape of the simulation                               */
+    uint64_t seed;           /* synthetic workload seed (operators, f0)               */
+    int64_t steps;           /* Euler steps to run                                    */
+    double theta;            /* Maxwell-Juttner temperature of f0                     */
+    int32_t with_flux;       /* x advection + synchrotron cooling when x_num > 1      */
+} dtx_request;
+
+typedef struct {
+    int64_t id;
+    int32_t status;          /* 0 queued, 1 running, 2 done, negative = failed        */
+    double submit_ms, start_ms, finish_ms;  /* server clock                           */
+    double number_total, energy_total;       /* moments of the final f               */
+    int64_t steps_done;
+} dtx_request_result;
+
+int dtx_server_create(const dtx_server_config* cfg, dtx_server** out);
+/* Enqueue a request; returns its id through *id.  Thread-safe. */
+int dtx_server_submit(dtx_server* s, const dtx_request* req, int64_t* id);
+/* Result of a request (status 2 when done).  Thread-safe. */
+int dtx_server_poll(dtx_server* s, int64_t id, dtx_request_result* out);
+/* Final f of a finished request (n doubles). */
+int dtx_server_fetch(dtx_server* s, int64_t id, double* f, int64_t n);
+/* Block until every submitted request has finished. */
+int dtx_server_drain(dtx_server* s);
+void dtx_server_destroy(dtx_server* s);
+const char* dtx_server_last_error(const dtx_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DTX_H */

@@ -1,0 +1,136 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Bar: FP64 GPU vs oracle within 1e-12 relative (L-inf over the state, relative to the state's
+L-inf norm) after whole Solve runs; elementwise semantics (clamps, Cr, finite flag) exact.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dtx import workloads
+from dtx.api import Solve
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+def rel_linf(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def _unpack(raw, rowstep):
+    """Place each lane's accumulator element at its (row, col) of the 16x16 tile."""
+    D = np.full((16, 16), np.nan)
+    for l in range(64):
+        for e in range(4):
+            row = 4 * (l // 16) + e if rowstep == 1 else l // 16 + 4 * e
+            D[row, l % 16] = raw[l * 4 + e]
+    return D
+
+
+def test_mfma_lane_layouts():
+    """The result layouts the contraction kernels' epilogues assume: FP64 16x16x4 holds
+    D[(l/16)+4e][l%16]; FP32 16x16x4 and BF16 16x16x32 hold D[4(l/16)+e][l%16]."""
+    lib = C.CDLL(str(Path(__file__).parent / "gpu_probe" / "libmfma_probe.so"))
+    lib.mfma_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    raw = np.zeros(256)
+    assert lib.mfma_probe(0, A.ctypes.data, B.ctypes.data, raw.ctypes.data) == 0
+    np.testing.assert_allclose(_unpack(raw, 4), A @ B, rtol=1e-14, atol=1e-14)
+    A32, B32 = A.astype(np.float32), B.astype(np.float32)
+    raw32 = np.zeros(256, np.float32)
+    assert lib.mfma_probe(1, A32.ctypes.data, B32.ctypes.data, raw32.ctypes.data) == 0
+    np.testing.assert_allclose(_unpack(raw32, 1), A32.astype(np.float64) @ B32.astype(np.float64),
+                               rtol=1e-5, atol=1e-5)
+    Ab = rng.standard_normal((16, 32)).astype(np.float32)
+    Bb = rng.standard_normal((32, 16)).astype(np.float32)
+    to_bits = lambda x: (x.view(np.uint32) >> 16).astype(np.uint16)  # noqa: E731
+    Ab = (to_bits(Ab).astype(np.uint32) << 16).view(np.float32)  # exactly representable
+    Bb = (to_bits(Bb).astype(np.uint32) << 16).view(np.float32)
+    rawb = np.zeros(256, np.float32)
+    Abits = np.ascontiguousarray(to_bits(Ab))
+    Bbits = np.ascontiguousarray(to_bits(Bb).T)  # column-major B
+    assert lib.mfma_probe(2, Abits.ctypes.data, Bbits.ctypes.data, rawb.ctypes.data) == 0
+    np.testing.assert_allclose(_unpack(rawb, 1), Ab.astype(np.float64) @ Bb.astype(np.float64),
+                               rtol=1e-5, atol=1e-4)
+
+
+def _solve_both(w, n_save=3, verbose=1, method="euler", mono=None, precision="fp64"):
+    m = workloads.make_struct(w, precision=precision, method=method, use_monolithic=mono)
+    out = Solve(m, w.dt, w.t_save(n_save), Verbose=verbose)
+    prob = O.problem_from_workload(w, use_monolithic=mono)
+    fo, to, stats, nsteps = prob.solve(w.f0, w.dt, w.t_save(n_save), verbose=verbose,
+                                       method=0 if method == "euler" else 1)
+    return m, out, fo, to, stats
+
+
+@pytest.mark.parametrize("mono", [True, False])
+def test_tiny_solve_parity_fp64(mono):
+    w = workloads.tiny(steps=30)
+    m, out, fo, to, _ = _solve_both(w, mono=mono)
+    np.testing.assert_array_equal(out.t, to)
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, i
+
+
+def test_tiny_single_step_binary_and_stats():
+    w = workloads.tiny(steps=5)
+    m = workloads.make_struct(w)
+    prob = O.problem_from_workload(w)
+    st = O.Stepper(prob, w.f0)
+    t = 0.0
+    for k in range(5):
+        dt_g, save_g, sg = m(t, 1.0, w.dt, 3, stats=True)
+        dt_o, save_o, so = st(t, 1.0, w.dt, 3)
+        assert dt_g == dt_o and save_g == save_o
+        assert sg.finite == so.finite
+        for name in ("cr", "cr_bin", "cr_flux"):
+            a, b = getattr(sg, name), getattr(so, name)
+            assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), (k, name, a, b)
+        assert rel_linf(m.f, st.f) < TOL64
+        t += dt_g
+
+
+@pytest.mark.parametrize("kw", [dict(emission=True), dict(injection=True),
+                                dict(domain=[0, 2]), dict(num_species=1),
+                                dict(num_species=3, x_num=2)])
+def test_tiny_variants_parity(kw):
+    w = workloads.tiny(steps=25, **kw)
+    m, out, fo, to, _ = _solve_both(w)
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, (kw, i)
+
+
+def test_tiny_symplectic_parity():
+    w = workloads.tiny(steps=25, injection=True)
+    m, out, fo, to, _ = _solve_both(w, method="symplectic")
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, i
+
+
+def test_moments_match_oracle():
+    w = workloads.tiny(steps=10)
+    m, out, fo, to, _ = _solve_both(w)
+    num, en = m.moments()
+    N = w.ps.n_momentum
+    for c in range(w.ps.n_space):
+        for s, (off, sp) in enumerate(zip(w.ps.species_offset, w.ps.species)):
+            blk = fo[-1][c * N + off: c * N + off + sp.size]
+            n0, e0 = O.moments(blk, sp)
+            assert abs(num[c, s] - n0) <= 1e-12 * abs(n0)
+            assert abs(en[c, s] - e0) <= 1e-12 * abs(e0)
+
+
+def test_medium_transport_parity_fp64():
+    # 2 species x (8 p x 4 u) = 64 states per cell, 24 cells, 200 steps, x advection + cooling
+    w = workloads.transport_1d(p_num=8, u_num=4, x_num=24, steps=200, monolithic=True)
+    m, out, fo, to, _ = _solve_both(w, n_save=5)
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, i
