@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit-Euler step on one MI355X.
+
+Workload (default): 1D transport -- 2 species (electron, photon) x (32 log p bins on
+[1e-3, 1e3] x 8 u bins x 1 phi bin) = 512 states per cell, 128 periodic x cells (65,536
+unknowns), upwind x advection + synchrotron cooling, per-cell binary collisions with lognormal
+gain tensors, plain FP64.  A step is one ForwardEuler functor call over the whole state; inputs
+are resident in HBM before the timed region.
+
+Prints ONE JSON line: steps/s of the FP64 path (value), with the FP32-MFMA and split-BF16 paths
+and their relative error against FP64 beside it, the live roofline of the dominant kernel (the
+binary contraction, HIP events on the library's stream) and the FP64 CPU oracle timed on a
+bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "diplodocustransport.jl_amd"))
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def run_gpu(w, precision, steps, warmup, verbose=1):
+    from dtx import workloads
+    m = workloads.make_struct(w, precision=precision, use_monolithic=False)
+    work = m.work()
+    m.reset()
+    t, big = 0.0, 1e300
+    for _ in range(warmup):
+        dt, _ = m(t, big, w.dt, verbose)
+        t += dt
+    m.synchronize()
+    m.kernel_timing(enable=True, reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt, _ = m(t, big, w.dt, verbose)
+        t += dt
+    m.synchronize()
+    t1 = time.perf_counter()
+    kt = m.kernel_timing(enable=False, reset=True)
+    f_end = m.f
+    num, en = m.moments()
+    res = dict(ms_per_step=(t1 - t0) * 1e3 / steps, steps_per_s=steps / (t1 - t0),
+               bin_ms=kt.bin_ms / max(1, kt.bin_launches),
+               update_ms=kt.update_ms / max(1, kt.update_launches),
+               bin_flops=work.bin_flops, gain_bytes=work.gain_bytes, bin_bytes=work.bin_bytes,
+               update_bytes=work.update_bytes, items=work.bin_launch_items,
+               device_gb=m.device_bytes() / 1e9, f=f_end, number=float(num.sum()),
+               energy=float(en.sum()))
+    m.close()
+    return res
+
+
+def run_cpu_baseline(w, sample_steps):
+    """The oracle in reference structure (per cell two GEMVs on the monolithic (N^2 x N) FP64
+    tensor, CSC SpMV, elementwise passes) on the host cores, over a bounded number of steps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as O
+    from dtx import synth
+    M = synth.assemble_M_Bin(w.ps, w.gains, w.losses)
+    prob = O.OracleProblem(w.ps, w.Ap, w.Vol, w.inj, F=w.F, M_Bin=M)
+    st = O.Stepper(prob, w.f0)
+    times, t = [], 0.0
+    for _ in range(sample_steps):
+        t0 = time.perf_counter()
+        dt, _, _ = st(t, 1e300, w.dt, 1)
+        times.append(time.perf_counter() - t0)
+        t += dt
+    med = float(np.median(times))
+    return dict(value=1.0 / med, unit="steps/s", cores=O.num_threads(), kind="port",
+                sample=f"{sample_steps} ForwardEuler steps of the same workload, reference "
+                       f"structure (per-cell dgemv pair on the monolithic {w.ps.n_momentum}^2 x "
+                       f"{w.ps.n_momentum} FP64 tensor), median step {med:.3f} s on "
+                       f"{O.num_threads()} threads of {cpu_model()}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the path shards by cells across GPUs only by design, see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from dtx import workloads
+    t0 = time.perf_counter()
+    w = workloads.transport_1d()
+    log(f"workload built in {time.perf_counter() - t0:.1f}s: N={w.ps.n_momentum} "
+        f"C={w.ps.n_space} n={w.ps.n} dt={w.dt:.4e}")
+    res = {}
+    for prec in args.precisions.split(","):
+        log(f"[bench] {prec}: {args.warmup} warmup + {args.steps} timed steps")
+        res[prec] = run_gpu(w, prec, args.steps, args.warmup)
+        r = res[prec]
+        log(f"[bench] {prec}: {r['steps_per_s']:.1f} steps/s, binary kernel {r['bin_ms']*1e3:.1f}"
+            f" us, update kernel {r['update_ms']*1e3:.1f} us, {r['items']} work items")
+    base = res["fp64"]
+    fp = base["f"]
+    prec_report = {}
+    for prec, r in res.items():
+        if prec == "fp64":
+            continue
+        mask = np.abs(fp) > 1e-8 * np.max(np.abs(fp))
+        prec_report[prec] = dict(
+            steps_per_s=round(r["steps_per_s"], 2),
+            speedup_vs_fp64=round(r["steps_per_s"] / base["steps_per_s"], 3),
+            rel_linf_f=float(np.max(np.abs(r["f"][mask] - fp[mask]) / np.abs(fp[mask]))),
+            number_drift=abs(r["number"] - base["number"]) / abs(base["number"]),
+            energy_drift=abs(r["energy"] - base["energy"]) / abs(base["energy"]),
+            binary_kernel_us=round(r["bin_ms"] * 1e3, 2),
+            binary_tflops=round(r["bin_flops"] / (r["bin_ms"] * 1e-3) / 1e12, 2))
+    achieved = base["bin_flops"] / (base["bin_ms"] * 1e-3) / 1e12
+    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["fp64"],
+                unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["fp64"], 4), traffic=None,
+                kernel="bin_f64_kernel<64>",
+                flops_per_launch=base["bin_flops"], kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: {args.cpu_steps} oracle steps")
+        cpu = run_cpu_baseline(w, args.cpu_steps)
+    line = {
+        "metric": "steps/s + solves/s, p50/p99 latency; FP32-MFMA vs FP64 at <=1e-6 rel err",
+        "value": round(base["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(base["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal gain tensors, Maxwell-Juttner f0)",
+        "config": {"workload": "transport_1d: 2 species x 256 states (32 p x 8 u x 1 phi), "
+                               "128 periodic x cells, 65536 unknowns, upwind x advection + "
+                               "synchrotron cooling, per-cell binary collisions, plain FP64",
+                   "solve_steps": w.steps,
+                   "solves_per_s": round(base["steps_per_s"] / w.steps, 4),
+                   "kernels_per_step": 2},
+        "precision_modes": prec_report,
+        "update_kernel_us": round(base["update_ms"] * 1e3, 2),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
