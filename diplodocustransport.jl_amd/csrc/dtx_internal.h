@@ -12,46 +12,53 @@
 namespace dtx {
 
 // ---- binary contraction tiling --------------------------------------------------------------
-// One work item multiplies a TK-row tile of a packed gain channel with the pair products of a
-// TC-cell tile over a range of pairs, and writes a partial sum for its (slot, cell, row).
+// The contraction follows the two-GEMV association of update_Big_Bin!: for every pair row i,
+// T_i[k, c] = sum_j S[k, (i, j)] f_j^c is a GEMM whose B operand is the state itself, and
+// dB[k, c] += f_i^c * T_i[k, c] is folded into an FP64 accumulator when the row ends.  Rows are
+// cut into chunks of KC consecutive j; a work item multiplies a TK-row tile of one channel with
+// a TC-cell tile over a range of chunks and writes its partial sum to its own slot.
 constexpr int TK = 64;          // output rows (k) per work item
-constexpr int KC64 = 16;        // pairs per LDS stage, FP64 and FP32 kernels
-constexpr int KC16 = 32;        // pairs per LDS stage, split-BF16 kernel
-constexpr int PAIR_ALIGN = 32;  // channel pair lists are padded to this multiple
-constexpr int FLUSH_PAIRS = 512;  // FP32 accumulators are folded into FP64 every this many pairs
-constexpr uint32_t PAIR_PAD = 0xFFFFFFFFu;  // padded pair: product forced to zero
+constexpr int KC = 32;          // j per chunk (one LDS stage)
+constexpr int SPECIES_ALIGN = 32;  // species blocks of the B operand start on this multiple
+constexpr int MAX_RANGE = 256;     // most chunks one work item covers (its table slice lives in LDS)
 
-// Channel kinds: TRI = one species with itself, pairs i <= j; RECT = s1 < s2, all (i, j).
-enum ChannelKind : int32_t { CH_TRI = 0, CH_RECT = 1 };
+// Channel kinds: TRI = one species with itself, pairs i <= j; RECT = s1 < s2, all (i, j);
+// LOSS = rows of one species against all states, multiplied by the row's own f at the end.
+enum ChannelKind : int32_t { CH_TRI = 0, CH_RECT = 1, CH_LOSS = 2 };
 
 // A work item of the grouped contraction kernel (one workgroup).  Items of several problems
 // (serving) can be mixed in one launch; everything an item needs is reachable from it.
 struct BinWork {
-    const void* A;          // packed gain of the channel: [ktile][...] (see pack layout)
-    const uint32_t* pairs;  // pair table of the channel: (i | j << 16), cell-local state index
-    const double* fT;       // state-major f of the problem: fT[state * ldc + cell]
-    double* part;           // partial sums of this slot: part[cell * N + row]
-    int64_t a_ktile_stride; // elements between consecutive k-tiles of A
-    int32_t ktile;          // k-tile index inside the channel
-    int32_t ctile;          // first cell of the tile = ctile * TC
-    int32_t p_begin, p_end; // pair range, multiples of the stage size
-    int32_t row_base;       // state index of the channel's first output row inside a cell
-    int32_t rows;           // valid output rows of the channel (No)
-    int32_t cells;          // valid cells of the problem (C)
-    int32_t ldc;            // leading dimension of fT
-    int32_t N;              // states per cell (partial buffer row pitch)
-    int32_t pad_;
+    const void* A;            // packed A of the channel, k-tile 0: [ktile][chunk][...]
+    const uint32_t* chunks;   // chunk table: i_pad | (j0_pad << 16)  (padded state indices)
+    const double* fT;         // FP64 f, state-major over padded states: fT[kp * ldc + cell]
+    const void* B;            // B operand: FP64 / FP32 -> fT (FP64 f is read directly),
+                              // BF16 -> hi bits, cell-major: B[cell * ldb + kp]
+    int64_t b_lo;             // BF16: elements from the hi to the lo array
+    double* part;             // partial sums of this slot: part[cell * N + row]
+    int64_t a_ktile_stride;   // elements between consecutive k-tiles of A
+    int32_t ktile;            // k-tile inside the channel
+    int32_t ctile;            // first cell of the tile = ctile * TC
+    int32_t c_begin, c_end;   // chunk range
+    int32_t row_base;         // state index (unpadded) of the channel's first output row
+    int32_t row_pad_base;     // padded state index of the first output row (LOSS multiplier)
+    int32_t rows;             // valid output rows of the channel (No)
+    int32_t cells;            // valid cells of the problem (C)
+    int32_t ldc;              // leading dimension of fT / fT32 (cells, padded)
+    int32_t ldb;              // row pitch of the BF16 B arrays (padded states)
+    int32_t N;                // states per cell (partial buffer row pitch)
+    int32_t kind;             // ChannelKind
 };
 
 // Packed operator of one channel on the device.
 struct PackedChannel {
-    int32_t s_out, s1, s2, kind;
+    int32_t s_out, s1, s2, kind, precision;
     int32_t No, N1, N2;
-    int64_t npairs;         // real pairs
-    int64_t npairs_pad;     // padded to PAIR_ALIGN
+    int64_t npairs;         // real (i, j) pairs
+    int64_t nchunks;        // chunks of KC
     int32_t nktiles;        // ceil(No / TK)
-    void* A = nullptr;      // device: packed gain (precision dependent)
-    uint32_t* pairs = nullptr;  // device pair table
+    void* A = nullptr;      // device: packed A
+    uint32_t* chunks = nullptr;  // device chunk table
     int64_t a_bytes = 0;
 };
 
@@ -69,14 +76,18 @@ struct EllMatrix {
 struct UpdateArgs {
     const double* f;        // current state (cell-major)
     double* f_out;          // next state
-    double* fT;             // state-major copy of f_out for the next contraction
+    double* fT;             // FP64 state-major copy of f_out (padded states) for the contraction
+    float* fT32;            // FP32 copy of the same (FP32 contraction) or null
+    uint16_t* fhi;          // BF16 hi/lo split, cell-major over padded states, or null
+    uint16_t* flo;
+    const int32_t* pad_index;  // padded index of every state (N)
+    int32_t ldb;            // row pitch of fhi / flo
     const double* inv_ap;
     const double* df_inj;
     const double* vol;
     const double* part;     // binary partial sums [slot][cell][N]
     const int32_t* nslot;   // slots per state (N)
     int64_t part_stride;    // doubles between slots
-    const double* loss;     // FP64 N x N diagonal/loss matrix (column-major) or null
     const double* emi;      // emission blocks [cell][N x N] or null
     const uint8_t* active;  // binary domain per cell (1 = evaluated) or null = all
     const int32_t* ell_col; const double* ell_val; int32_t ell_width;
@@ -84,6 +95,7 @@ struct UpdateArgs {
     const int32_t* state_species;  // species of each state (N)
     const int32_t* species_off;    // (S+1)
     double* moments;        // [cell][S][2] (number, energy)
+    double* mom_part;       // per-block moment partials [cell][segment][S][2]
     double* block_part;     // per-block partials [nblocks][8]
     double* status;         // status ring [ring][16]
     uint32_t* counter;      // last-block counter
@@ -116,15 +128,20 @@ enum BlockSlot { BP_SUMDF = 0, BP_SUMF, BP_CR, BP_CR_BIN, BP_CR_EMI, BP_CR_FLUX,
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
+hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,
+                            int rows, int nktiles, double* A, hipStream_t stream);
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream);
+hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
+                                int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
+                                hipStream_t stream);
 
 // synthetic generator (synth.hip)
-hipError_t launch_synth_gain_packed(int precision, void* A, const uint32_t* pairs,
-                                    int64_t npairs_pad, int No, int nktiles, int kind,
-                                    int N1, int N2, uint64_t key, double scale,
+hipError_t launch_synth_gain_packed(int precision, void* A, const uint32_t* chunks,
+                                    int64_t nchunks, int No, int nktiles, int kind, int N1,
+                                    int N2, int pad1, int pad2, uint64_t key, double scale,
                                     hipStream_t stream);
 hipError_t launch_synth_loss(double* loss, int N, int off1, int N1, int off2, int N2,
                              int No, uint64_t key, double scale, int kind_same,

@@ -1,12 +1,17 @@
 // kernels_binary.hip -- grouped binary-collision contraction for gfx950.
 //
-// Replaces the per-cell pair of GEMVs of update_Big_Bin! (src/SteppingMethods.jl:360-386):
-//     dB[k, c] = sum_{(i,j)} S[k, (i,j)] * f_i^c * f_j^c
-// as one GEMM per channel with K = the channel's (i,j) pairs.  The A operand is the packed gain
-// S (streamed from HBM once per step); the B operand, the per-cell pair products f_i f_j, is
-// generated on the fly into LDS from the state-major copy fT and never materialised in HBM.
-// Every workgroup takes one work item (channel, 64-row tile, cell tile, pair range) from a work
-// table and writes its partial sum to its own slot, so the reduction order is fixed by the
+// Replaces the per-cell pair of GEMVs of update_Big_Bin! (src/SteppingMethods.jl:360-386),
+//     T[k,i] = sum_j M[k,i,j] f_j ;  dB[k] = sum_i T[k,i] f_i ,
+// for all cells at once.  The tensor is packed per channel in symmetric pair rows: row i holds
+// S[k,(i,j)] for its j range, cut into chunks of KC = 32 consecutive j.  For each chunk the
+// kernel runs an MFMA GEMM  acc[k,c] += sum_j S[k,(i,j)] f_j^c  whose B operand is the state
+// itself, scaled by f_i^c: FP64 and FP32 fold f_i into the staged B tile, split-BF16 keeps B
+// as the bf16 split of f and folds tot[k,c] += f_i^c * acc[k,c] in FP64 when the row ends.  LOSS
+// channels (the diagonal part, FP64 in every mode) accumulate sum_j D[k,j] f_j^c over all chunks
+// and are multiplied by the row's own f_k^c.
+//
+// Every workgroup takes one work item (channel, 64-row tile, cell tile, chunk range) from a work
+// table and writes its FP64 partial sum to its own slot, so the reduction order is fixed by the
 // item list alone: a simulation gives bit-identical results alone or grouped with others.
 //
 // Wave tiling: 4 waves = WK (rows) x WC (cells); each wave owns MR x MC 16x16 MFMA tiles.
@@ -14,7 +19,6 @@
 // The 16x16 result tile is held as D[4*(l/16)+e][l%16] (e = 0..3) by the FP32 and BF16
 // instructions, and as D[(l/16)+4*e][l%16] by v_mfma_f64_16x16x4_f64 (measured on gfx950).
 #include "dtx_internal.h"
-#include "bf16_util.h"
 
 namespace dtx {
 
@@ -33,88 +37,123 @@ struct Tiling {
 };
 
 struct ItemView {
-    const void* A;
-    const uint32_t* pairs;
+    const char* A;
+    const uint32_t* chunks;
     const double* fT;
+    const void* B;
+    int64_t b_lo;
     double* part;
-    int p_begin, p_end, krow0, row_base, rows, cells, ldc, N, c0;
+    int c_begin, c_end, krow0, row_base, row_pad_base, rows, cells, ldc, ldb, N, c0, kind;
 };
 
-__device__ inline ItemView load_item(const BinWork* items, int tc) {
+__device__ inline ItemView load_item(const BinWork* items, int tc, int elem_bytes) {
     const BinWork& w = items[blockIdx.x];
     ItemView v;
-    v.A = w.A;
-    v.pairs = w.pairs;
+    v.A = (const char*)w.A + (int64_t)w.ktile * w.a_ktile_stride * elem_bytes;
+    v.chunks = w.chunks;
     v.fT = w.fT;
+    v.B = w.B;
+    v.b_lo = w.b_lo;
     v.part = w.part;
-    v.p_begin = w.p_begin;
-    v.p_end = w.p_end;
+    v.c_begin = w.c_begin;
+    v.c_end = w.c_end;
     v.krow0 = w.ktile * TK;
     v.row_base = w.row_base;
+    v.row_pad_base = w.row_pad_base;
     v.rows = w.rows;
     v.cells = w.cells;
     v.ldc = w.ldc;
+    v.ldb = w.ldb;
     v.N = w.N;
     v.c0 = w.ctile * tc;
+    v.kind = w.kind;
     return v;
 }
 
-// Store one 16x16 tile of partial sums: rows krow0 + row, cells c0 + col.  ROWSTEP is the row
-// distance between a lane's consecutive accumulator elements (1 for FP32/BF16, 4 for FP64).
+// Row (within the 64-row tile) of accumulator element e of a 16x16 tile at row16.
+template <int ROWSTEP>
+__device__ inline int tile_row(int row16, int lane, int e) {
+    return row16 + (ROWSTEP == 1 ? 4 * (lane >> 4) + e : (lane >> 4) + 4 * e);
+}
+
+// Write one tile of FP64 partial sums (LOSS items are first multiplied by f of their row).
 template <int ROWSTEP>
 __device__ inline void store_tile(const ItemView& v, int row16, int col16, int lane,
-                                  const double* acc4) {
-    const int col = col16 + (lane & 15);
-    const int cell = v.c0 + col;
+                                  const double* t4) {
+    const int cell = v.c0 + col16 + (lane & 15);
     if (cell >= v.cells) return;
     double* dst = v.part + (int64_t)cell * v.N + v.row_base;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const int krow = v.krow0 + row16 + (ROWSTEP == 1 ? 4 * (lane >> 4) + e
-                                                         : (lane >> 4) + 4 * e);
-        if (krow < v.rows) dst[krow] = acc4[e];
+        const int krow = v.krow0 + tile_row<ROWSTEP>(row16, lane, e);
+        if (krow < v.rows) {
+            double x = t4[e];
+            if (v.kind == CH_LOSS) x = x * v.fT[(int64_t)(v.row_pad_base + krow) * v.ldc + cell];
+            dst[krow] = x;
+        }
     }
 }
 
+// f_i of the lane's column in the cell tile (row-end multiplier).
+__device__ inline double f_at(const ItemView& v, int ip, int col) {
+    return v.fT[(int64_t)ip * v.ldc + v.c0 + col];
+}
+
+// Copy the item's slice of the chunk table into LDS, so the main loop never waits on a global
+// load for its control flow.
+__device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
+    for (int q = threadIdx.x; q < v.c_end - v.c_begin; q += 256) ctab[q] = v.chunks[v.c_begin + q];
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------------------- FP64 -----
+// The row multiplier f_i is folded into the B tile while it is staged: B'[t][c] = f_i^c f_j^c
+// (LOSS channels stage f_j^c and multiply by their row's f at the end), so the accumulators
+// hold the whole item's contribution and no epilogue runs inside the loop.  Each 32-j chunk is
+// staged through LDS in two halves of 16 j, double-buffered (32 KB of LDS).
 template <int TC>
 __global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict__ items) {
     using T = Tiling<TC>;
-    constexpr int KC = KC64;
-    constexpr int AP = KC * TK / 256;     // A elements per thread
-    constexpr int BP = KC * TC / 256;     // pair products per thread (consecutive cells)
-    constexpr int BT = TC / BP;           // threads per pair row
-    __shared__ double As[2][KC * TK];
-    __shared__ double Bs[2][KC * TC];
+    constexpr int HK = KC / 2;          // j per LDS stage
+    constexpr int AP = HK * TK / 256;   // 4 A elements per thread
+    constexpr int BP = HK * TC / 256;   // B elements per thread (consecutive cells)
+    constexpr int BT = TC / BP;         // threads per B row
+    __shared__ double As[2][HK * TK];
+    __shared__ double Bs[2][HK * TC];
+    __shared__ uint32_t ctab[MAX_RANGE];
 
-    const ItemView v = load_item(items, TC);
+    const ItemView v = load_item(items, TC, 8);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
-    const double* A = (const double*)v.A + (int64_t)items[blockIdx.x].a_ktile_stride *
-                                               items[blockIdx.x].ktile;
-    const int bpp = tid / BT, bcc = (tid % BT) * BP;
+    const double* A = (const double*)v.A + (int64_t)v.c_begin * KC * TK;
+    const double* B = (const double*)v.B;
+    const int bq = tid / BT, bcc = (tid % BT) * BP;
+    const bool loss = v.kind == CH_LOSS;
+    load_ctab(v, ctab);
 
-    double ra[AP], rbi[BP], rbj[BP];
-    auto load_chunk = [&](int p) {
-        const double* src = A + (int64_t)p * TK + tid * AP;
+    double ra[AP], rb[BP], rf[BP];
+    // stage st = half (st & 1) of local chunk (st >> 1)
+    auto load_stage = [&](int st) {
+        const int ch = st >> 1, h = st & 1;
+        const uint32_t ce = ctab[ch];
+        const double* src = A + (int64_t)ch * KC * TK + h * HK * TK + tid * AP;
 #pragma unroll
         for (int q = 0; q < AP; ++q) ra[q] = src[q];
-        const uint32_t pr = v.pairs[p + bpp];
-        if (pr != PAIR_PAD) {
-            const double* fi = v.fT + (int64_t)(pr & 0xFFFFu) * v.ldc + v.c0 + bcc;
-            const double* fj = v.fT + (int64_t)(pr >> 16) * v.ldc + v.c0 + bcc;
+        const int j0 = (int)(ce >> 16) + h * HK;
+        const double* b = B + (int64_t)(j0 + bq) * v.ldc + v.c0 + bcc;
 #pragma unroll
-            for (int q = 0; q < BP; ++q) { rbi[q] = fi[q]; rbj[q] = fj[q]; }
-        } else {
+        for (int q = 0; q < BP; ++q) rb[q] = b[q];
+        if (!loss) {
+            const double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
 #pragma unroll
-            for (int q = 0; q < BP; ++q) { rbi[q] = 0.0; rbj[q] = 0.0; }
+            for (int q = 0; q < BP; ++q) rf[q] = fi[q];
         }
     };
-    auto store_chunk = [&](int buf) {
+    auto store_stage = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < AP; ++q) As[buf][tid * AP + q] = ra[q];
 #pragma unroll
-        for (int q = 0; q < BP; ++q) Bs[buf][bpp * TC + bcc + q] = rbi[q] * rbj[q];
+        for (int q = 0; q < BP; ++q) Bs[buf][bq * TC + bcc + q] = loss ? rb[q] : rf[q] * rb[q];
     };
 
     d4 acc[T::MR][T::MC];
@@ -122,18 +161,18 @@ __global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict_
     for (int r = 0; r < T::MR; ++r)
 #pragma unroll
         for (int c = 0; c < T::MC; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
-
-    if (v.p_begin < v.p_end) {
-        load_chunk(v.p_begin);
-        store_chunk(0);
+    const int nst = 2 * (v.c_end - v.c_begin);
+    if (nst > 0) {
+        load_stage(0);
+        store_stage(0);
     }
     __syncthreads();
     int buf = 0;
-    for (int p = v.p_begin; p < v.p_end; p += KC) {
-        const bool more = p + KC < v.p_end;
-        if (more) load_chunk(p + KC);
+    for (int st = 0; st < nst; ++st) {
+        const bool more = st + 1 < nst;
+        if (more) load_stage(st + 1);
 #pragma unroll
-        for (int kk = 0; kk < KC; kk += 4) {
+        for (int kk = 0; kk < HK; kk += 4) {
             const int kr = kk + (lane >> 4);
             double a[T::MR], b[T::MC];
 #pragma unroll
@@ -148,7 +187,7 @@ __global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict_
                 for (int c = 0; c < T::MC; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
-        if (more) store_chunk(buf ^ 1);
+        if (more) store_stage(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
@@ -156,73 +195,76 @@ __global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict_
     for (int r = 0; r < T::MR; ++r)
 #pragma unroll
         for (int c = 0; c < T::MC; ++c) {
-            double t[4] = {acc[r][c][0], acc[r][c][1], acc[r][c][2], acc[r][c][3]};
-            store_tile<4>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, t);
+            const double t4[4] = {acc[r][c][0], acc[r][c][1], acc[r][c][2], acc[r][c][3]};
+            store_tile<4>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, t4);
         }
 }
 
 // ------------------------------------------------------------------------------- FP32 -----
+// As FP64, with B'[t][c] = (float)(f_i^c * f_j^c) formed in FP64 and rounded once, FP32 MFMA
+// accumulation, and the FP32 accumulators folded into FP64 every FLUSH_CHUNKS chunks.
+constexpr int FLUSH_CHUNKS = 16;
+
 template <int TC>
 __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict__ items) {
     using T = Tiling<TC>;
-    constexpr int KC = KC64;
-    constexpr int AP = KC * TK / 256;
+    constexpr int AP = KC * TK / 256;   // 8
     constexpr int BP = KC * TC / 256;
     constexpr int BT = TC / BP;
-    constexpr int FLUSH = FLUSH_PAIRS / KC;
     __shared__ float As[2][KC * TK];
     __shared__ float Bs[2][KC * TC];
+    __shared__ uint32_t ctab[MAX_RANGE];
 
-    const ItemView v = load_item(items, TC);
+    const ItemView v = load_item(items, TC, 4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
-    const float* A = (const float*)v.A + (int64_t)items[blockIdx.x].a_ktile_stride *
-                                             items[blockIdx.x].ktile;
-    const int bpp = tid / BT, bcc = (tid % BT) * BP;
+    const float* A = (const float*)v.A + (int64_t)v.c_begin * KC * TK;
+    const double* B = v.fT;
+    const int bq = tid / BT, bcc = (tid % BT) * BP;
+    load_ctab(v, ctab);
 
     float ra[AP];
-    double rbi[BP], rbj[BP];
-    auto load_chunk = [&](int p) {
-        const float* src = A + (int64_t)p * TK + tid * AP;
+    double rb[BP], rf[BP];
+    auto load_chunk = [&](int ch) {
+        const uint32_t ce = ctab[ch];
+        const float* src = A + (int64_t)ch * KC * TK + tid * AP;
 #pragma unroll
         for (int q = 0; q < AP; ++q) ra[q] = src[q];
-        const uint32_t pr = v.pairs[p + bpp];
-        if (pr != PAIR_PAD) {
-            const double* fi = v.fT + (int64_t)(pr & 0xFFFFu) * v.ldc + v.c0 + bcc;
-            const double* fj = v.fT + (int64_t)(pr >> 16) * v.ldc + v.c0 + bcc;
+        const double* b = B + (int64_t)((int)(ce >> 16) + bq) * v.ldc + v.c0 + bcc;
+        const double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
 #pragma unroll
-            for (int q = 0; q < BP; ++q) { rbi[q] = fi[q]; rbj[q] = fj[q]; }
-        } else {
-#pragma unroll
-            for (int q = 0; q < BP; ++q) { rbi[q] = 0.0; rbj[q] = 0.0; }
+        for (int q = 0; q < BP; ++q) {
+            rb[q] = b[q];
+            rf[q] = fi[q];
         }
     };
     auto store_chunk = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < AP; ++q) As[buf][tid * AP + q] = ra[q];
 #pragma unroll
-        for (int q = 0; q < BP; ++q) Bs[buf][bpp * TC + bcc + q] = (float)(rbi[q] * rbj[q]);
+        for (int q = 0; q < BP; ++q) Bs[buf][bq * TC + bcc + q] = (float)(rf[q] * rb[q]);
     };
 
     f4 acc[T::MR][T::MC];
-    double acc64[T::MR][T::MC][4];
+    double tot[T::MR][T::MC][4];
 #pragma unroll
     for (int r = 0; r < T::MR; ++r)
 #pragma unroll
         for (int c = 0; c < T::MC; ++c) {
             acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc64[r][c][e] = 0.0;
+            for (int e = 0; e < 4; ++e) tot[r][c][e] = 0.0;
         }
-    if (v.p_begin < v.p_end) {
-        load_chunk(v.p_begin);
+    const int nch = v.c_end - v.c_begin;
+    if (nch > 0) {
+        load_chunk(0);
         store_chunk(0);
     }
     __syncthreads();
-    int buf = 0, it = 0;
-    for (int p = v.p_begin; p < v.p_end; p += KC) {
-        const bool more = p + KC < v.p_end;
-        if (more) load_chunk(p + KC);
+    int buf = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const bool more = ch + 1 < nch;
+        if (more) load_chunk(ch + 1);
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             const int kr = kk + (lane >> 4);
@@ -239,14 +281,13 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
                 for (int c = 0; c < T::MC; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], b[c], acc[r][c], 0, 0, 0);
         }
-        if (++it == FLUSH || !more) {
-            it = 0;
+        if ((ch + 1) % FLUSH_CHUNKS == 0 || !more) {
 #pragma unroll
             for (int r = 0; r < T::MR; ++r)
 #pragma unroll
                 for (int c = 0; c < T::MC; ++c) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) acc64[r][c][e] += (double)acc[r][c][e];
+                    for (int e = 0; e < 4; ++e) tot[r][c][e] += (double)acc[r][c][e];
                     acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
                 }
         }
@@ -258,92 +299,82 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
     for (int r = 0; r < T::MR; ++r)
 #pragma unroll
         for (int c = 0; c < T::MC; ++c)
-            store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, acc64[r][c]);
+            store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, tot[r][c]);
 }
 
 // ------------------------------------------------------------------------ split BF16 x3 -----
-// Packed A per k-tile and 32-pair chunk: hi[TK][32] then lo[TK][32] (bf16 bits).
-// LDS rows are padded to 40 elements so the 16-byte fragment reads of 16 rows hit distinct
-// banks.
+// B is the cell-major hi/lo split of f (no per-element work while staging); the row multiplier
+// f_i is staged in LDS beside each chunk and folded into FP64 when the row ends.  Packed A per
+// k-tile and chunk: hi[TK][32] then lo[TK][32] (bf16 bits).  LDS rows are padded to 40 elements
+// so the 16-byte fragment reads of 16 rows hit distinct banks.
 constexpr int BPITCH = 40;
 
 template <int TC>
 __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restrict__ items) {
     using T = Tiling<TC>;
-    constexpr int KC = KC16;                 // 32 pairs = one MFMA K step
-    constexpr int FLUSH = FLUSH_PAIRS / KC;
-    constexpr int BG = KC * TC / 8;          // groups of 8 products
     __shared__ __attribute__((aligned(16))) uint16_t Ah[2][TK * BPITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Al[2][TK * BPITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bh[2][TC * BPITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bl[2][TC * BPITCH];
+    __shared__ double Fs[2][TC];
+    __shared__ uint32_t ctab[MAX_RANGE];
 
-    const ItemView v = load_item(items, TC);
+    const ItemView v = load_item(items, TC, 2);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
-    const uint16_t* A = (const uint16_t*)v.A + (int64_t)items[blockIdx.x].a_ktile_stride *
-                                                   items[blockIdx.x].ktile;
-    const bool bthread = tid < BG;
-    const int bcc = tid % TC, bk8 = (tid / TC) * 8;
+    const uint16_t* A = (const uint16_t*)v.A + (int64_t)v.c_begin * TK * KC * 2;
+    const uint16_t* Bhi = (const uint16_t*)v.B;
+    const uint16_t* Blo = Bhi + v.b_lo;
+    const bool bthread = tid < TC * 4;
+    const int bcc = tid >> 2, bk8 = (tid & 3) * 8;
     const int arow = tid >> 2, ak8 = (tid & 3) * 8;
+    load_ctab(v, ctab);
 
-    uint4 rah, ral;
-    double rbi[8], rbj[8];
-    auto load_chunk = [&](int p) {
-        const uint4* src = (const uint4*)(A + (int64_t)p * TK * 2);  // chunk = 2*TK*32 bf16
+    uint4 rah, ral, rbh, rbl;
+    double rfi = 0.0;
+    auto load_chunk = [&](int ch) {
+        const uint32_t ce = ctab[ch];
+        const uint4* src = (const uint4*)(A + (int64_t)ch * TK * KC * 2);
         rah = src[tid];
         ral = src[256 + tid];
         if (bthread) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const uint32_t pr = v.pairs[p + bk8 + q];
-                if (pr != PAIR_PAD) {
-                    rbi[q] = v.fT[(int64_t)(pr & 0xFFFFu) * v.ldc + v.c0 + bcc];
-                    rbj[q] = v.fT[(int64_t)(pr >> 16) * v.ldc + v.c0 + bcc];
-                } else {
-                    rbi[q] = 0.0;
-                    rbj[q] = 0.0;
-                }
-            }
+            const int64_t off = (int64_t)(v.c0 + bcc) * v.ldb + (int)(ce >> 16) + bk8;
+            rbh = *(const uint4*)(Bhi + off);
+            rbl = *(const uint4*)(Blo + off);
         }
+        if (tid < TC) rfi = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + tid];
     };
     auto store_chunk = [&](int buf) {
         *(uint4*)&Ah[buf][arow * BPITCH + ak8] = rah;
         *(uint4*)&Al[buf][arow * BPITCH + ak8] = ral;
         if (bthread) {
-            uint16_t h[8], l[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) split_bf16(rbi[q] * rbj[q], &h[q], &l[q]);
-            uint4 vh, vl;
-            vh.x = h[0] | ((uint32_t)h[1] << 16); vh.y = h[2] | ((uint32_t)h[3] << 16);
-            vh.z = h[4] | ((uint32_t)h[5] << 16); vh.w = h[6] | ((uint32_t)h[7] << 16);
-            vl.x = l[0] | ((uint32_t)l[1] << 16); vl.y = l[2] | ((uint32_t)l[3] << 16);
-            vl.z = l[4] | ((uint32_t)l[5] << 16); vl.w = l[6] | ((uint32_t)l[7] << 16);
-            *(uint4*)&Bh[buf][bcc * BPITCH + bk8] = vh;
-            *(uint4*)&Bl[buf][bcc * BPITCH + bk8] = vl;
+            *(uint4*)&Bh[buf][bcc * BPITCH + bk8] = rbh;
+            *(uint4*)&Bl[buf][bcc * BPITCH + bk8] = rbl;
         }
+        if (tid < TC) Fs[buf][tid] = rfi;
     };
 
     f4 acc[T::MR][T::MC];
-    double acc64[T::MR][T::MC][4];
+    double tot[T::MR][T::MC][4];
 #pragma unroll
     for (int r = 0; r < T::MR; ++r)
 #pragma unroll
         for (int c = 0; c < T::MC; ++c) {
             acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc64[r][c][e] = 0.0;
+            for (int e = 0; e < 4; ++e) tot[r][c][e] = 0.0;
         }
-    if (v.p_begin < v.p_end) {
-        load_chunk(v.p_begin);
+    const int nch = v.c_end - v.c_begin;
+    if (nch > 0) {
+        load_chunk(0);
         store_chunk(0);
     }
     __syncthreads();
-    int buf = 0, it = 0;
+    int buf = 0;
     const int kq = 8 * (lane >> 4);
-    for (int p = v.p_begin; p < v.p_end; p += KC) {
-        const bool more = p + KC < v.p_end;
-        if (more) load_chunk(p + KC);
+    for (int ch = 0; ch < nch; ++ch) {
+        const bool more = ch + 1 < nch;
+        if (more) load_chunk(ch + 1);
         bf16x8 ah[T::MR], al[T::MR], bh[T::MC], bl[T::MC];
 #pragma unroll
         for (int r = 0; r < T::MR; ++r) {
@@ -365,16 +396,19 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
                 acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
                 acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
             }
-        if (++it == FLUSH || !more) {
-            it = 0;
+        const uint32_t ip = ctab[ch] & 0xFFFFu;
+        if (!more || (ctab[ch + 1] & 0xFFFFu) != ip) {
+            // the row ends: tot += f_i * acc in FP64
 #pragma unroll
-            for (int r = 0; r < T::MR; ++r)
+            for (int c = 0; c < T::MC; ++c) {
+                const double fi = Fs[buf][wc * T::WCOLS + c * 16 + (lane & 15)];
 #pragma unroll
-                for (int c = 0; c < T::MC; ++c) {
+                for (int r = 0; r < T::MR; ++r) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) acc64[r][c][e] += (double)acc[r][c][e];
+                    for (int e = 0; e < 4; ++e) tot[r][c][e] += fi * (double)acc[r][c][e];
                     acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
                 }
+            }
         }
         if (more) store_chunk(buf ^ 1);
         __syncthreads();
@@ -384,7 +418,32 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
     for (int r = 0; r < T::MR; ++r)
 #pragma unroll
         for (int c = 0; c < T::MC; ++c)
-            store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, acc64[r][c]);
+            store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, tot[r][c]);
+}
+
+// Pack rows [row0, row0+rows) of the FP64 loss matrix D (N x N, column-major) as a LOSS
+// channel: A[kt][chunk][t][r] = D[row0 + kt*TK + r, state at padded column chunk*KC + t].
+__global__ void pack_loss_kernel(const double* D, int N, const int32_t* unpad, int Np, int row0,
+                                 int rows, int nktiles, double* A) {
+    const int64_t nch = Np / KC;
+    const int64_t total = (int64_t)nktiles * nch * KC * TK;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % TK);
+        const int t = (int)((e / TK) % KC);
+        const int64_t ch = (e / ((int64_t)TK * KC)) % nch;
+        const int kt = (int)(e / ((int64_t)TK * KC * nch));
+        const int row = kt * TK + r;
+        const int jp = (int)(ch * KC + t);
+        const int j = unpad[jp];
+        A[e] = (row < rows && j >= 0) ? D[(int64_t)(row0 + row) + (int64_t)N * j] : 0.0;
+    }
+}
+
+hipError_t launch_pack_loss(const double* D, int N, const int32_t* unpad, int Np, int row0,
+                            int rows, int nktiles, double* A, hipStream_t stream) {
+    pack_loss_kernel<<<4096, 256, 0, stream>>>(D, N, unpad, Np, row0, rows, nktiles, A);
+    return hipGetLastError();
 }
 
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,

@@ -1,6 +1,6 @@
 // kernels_update.hip -- fused per-cell update kernels for gfx950 (HBM-bound).
 //
-// One workgroup per spatial cell.  For every state of the cell it evaluates, in FP64 and with
+// One workgroup per (cell, segment of 256 states).  For every state it evaluates, in FP64 and with
 // the reference's operation order (this file is built with -ffp-contract=off, so no FMA):
 //   ForwardEuler, src/SteppingMethods.jl:31-101
 //     dF  = F f                           (ELL rows, columns ascending = CSC column order)
@@ -10,9 +10,10 @@
 //   plus Cr / Cr_Bin / Cr_Emi / Cr_Flux, isfinite(sum(df)), sum(f) and the per-(cell, species)
 //   number and energy moments of the new f.
 // dE is the cell's emission block times f (ascending column order) and dB the binary term:
-// the sum of the contraction's partial slots plus f_k * (Loss f)_k, times Vol.
+// the sum of the contraction's partial slots (gain and FP64 loss channels), times Vol.
 // Cross-cell reductions go to per-block partials that the last block folds in block order, so
 // every reduction is deterministic.
+#include "bf16_util.h"
 #include "dtx_internal.h"
 
 namespace dtx {
@@ -37,41 +38,49 @@ __device__ inline double cr_min(double m, double r) {
     return m;
 }
 
+// Block reductions: a butterfly over the 64 lanes of each wave (__shfl_xor), then the four wave
+// results combined in wave order.  The pattern is fixed, so every reduction is deterministic.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = v + __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ inline double wave_min(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = cr_min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
 __device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
     const int tid = threadIdx.x;
-    red[tid] = v;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
     __syncthreads();
-    for (int s = UB / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = red[tid] + red[tid + s];
-        __syncthreads();
-    }
     double r = red[0];
+    for (int w = 1; w < UB / 64; ++w) r = r + red[w];
     __syncthreads();
     return r;
 }
 
 __device__ double block_min(double v, double* red) {
+    v = wave_min(v);
     const int tid = threadIdx.x;
-    red[tid] = v;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
     __syncthreads();
-    for (int s = UB / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = cr_min(red[tid], red[tid + s]);
-        __syncthreads();
-    }
     double r = red[0];
+    for (int w = 1; w < UB / 64; ++w) r = cr_min(r, red[w]);
     __syncthreads();
     return r;
 }
 
 __device__ int block_or(int v, int* red) {
     const int tid = threadIdx.x;
-    red[tid] = v;
+    v = __any(v) ? 1 : 0;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
     __syncthreads();
-    for (int s = UB / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] |= red[tid + s];
-        __syncthreads();
-    }
-    int r = red[0];
+    int r = 0;
+    for (int w = 0; w < UB / 64; ++w) r |= red[w];
     __syncthreads();
     return r;
 }
@@ -88,18 +97,36 @@ __device__ inline double ell_row(const int32_t* col, const double* val, int widt
     return acc;
 }
 
-// Binary term of state k of cell c (unscaled by invAp): (gain partials + f_k (Loss f)_k) * Vol.
-__device__ inline double binary_term(const UpdateArgs& a, int c, int k, const double* fc) {
+// Binary term of state k of cell c (unscaled by invAp): the contraction's partial slots (gain
+// channels and the FP64 loss channels), in slot order, times Vol.
+__device__ inline double binary_term(const UpdateArgs& a, int c, int k) {
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
     const int ns = a.nslot[k];
-    for (int s = 0; s < ns; ++s) g = g + a.part[(int64_t)s * a.part_stride + idx];
-    if (a.loss) {
-        double l = 0.0;
-        for (int j = 0; j < a.N; ++j) l = l + a.loss[k + (int64_t)a.N * j] * fc[j];
-        g = g + fc[k] * l;
+    int s = 0;
+    for (; s + 8 <= ns; s += 8) {  // eight loads in flight, summed in slot order
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = a.part[(int64_t)(s + q) * a.part_stride + idx];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) g = g + v[q];
     }
+    for (; s < ns; ++s) g = g + a.part[(int64_t)s * a.part_stride + idx];
     return g * a.vol[c];
+}
+
+// Copies of a new state value for the next contraction: FP64 state-major (padded states), and
+// the FP32 copy or the BF16 hi/lo split when that precision is in use.
+__device__ inline void publish(const UpdateArgs& a, int c, int k, double x) {
+    const int kp = a.pad_index[k];
+    a.fT[(int64_t)kp * a.ldc + c] = x;
+    if (a.fT32) a.fT32[(int64_t)kp * a.ldc + c] = (float)x;
+    if (a.fhi) {
+        uint16_t hi, lo;
+        split_bf16(x, &hi, &lo);
+        a.fhi[(int64_t)c * a.ldb + kp] = hi;
+        a.flo[(int64_t)c * a.ldb + kp] = lo;
+    }
 }
 
 __device__ inline double emission_term(const UpdateArgs& a, int c, int k, const double* fc) {
@@ -109,24 +136,30 @@ __device__ inline double emission_term(const UpdateArgs& a, int c, int k, const 
     return acc;
 }
 
-// Per-species moments of the new state (in LDS) and the per-block partials; the last block to
-// finish folds all partials (block order) into the status record of this step.
-__device__ void finish_block(const UpdateArgs& a, const double* fn, double* red,
+// Per-species moment partials of this block's new states (fn, LDS, indexed k - k0) and the
+// per-block partials; the last block to finish folds all partials (block order) into the
+// moments and the status record of this step.
+__device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int k1, double* red,
                              double sumdf, double sumf, double cr, double crb, double cre,
                              double crf, double crm, double crs) {
-    const int c = blockIdx.x, tid = threadIdx.x;
+    const int c = blockIdx.x, seg = blockIdx.y, nseg = gridDim.y, tid = threadIdx.x;
+    const int64_t b = (int64_t)c * nseg + seg;
+    const int64_t nblocks = (int64_t)gridDim.x * nseg;
     for (int s = 0; s < a.S; ++s) {
-        const int k0 = a.species_off[s], k1 = a.species_off[s + 1];
+        const int lo = max(k0, a.species_off[s]), hi = min(k1, a.species_off[s + 1]);
         double num = 0.0, en = 0.0;
-        for (int k = k0 + tid; k < k1; k += UB) {
-            num = num + fn[k];
-            if (a.eweight) en = en + fn[k] * a.eweight[k];
+        if (lo < hi) {
+            const int k = k0 + tid;
+            if (k >= lo && k < hi) {
+                num = fn[tid];
+                if (a.eweight) en = fn[tid] * a.eweight[k];
+            }
+            num = block_sum(num, red);
+            en = block_sum(en, red);
         }
-        num = block_sum(num, red);
-        en = block_sum(en, red);
         if (tid == 0) {
-            a.moments[((int64_t)c * a.S + s) * 2 + 0] = num;
-            a.moments[((int64_t)c * a.S + s) * 2 + 1] = en;
+            a.mom_part[(b * a.S + s) * 2 + 0] = num;
+            a.mom_part[(b * a.S + s) * 2 + 1] = en;
         }
     }
     sumdf = block_sum(sumdf, red);
@@ -143,7 +176,7 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, double* red,
     }
     __shared__ int is_last;
     if (tid == 0) {
-        double* bp = a.block_part + (int64_t)c * BP_WORDS;
+        double* bp = a.block_part + b * BP_WORDS;
         bp[BP_SUMDF] = sumdf;
         bp[BP_SUMF] = sumf;
         bp[BP_CR] = cr;
@@ -154,15 +187,27 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, double* red,
         bp[BP_CR_SPACE] = crs;
         __threadfence();
         const unsigned prev = atomicAdd(a.counter, 1u);
-        is_last = (prev == gridDim.x - 1);
+        is_last = (prev == (unsigned)nblocks - 1);
     }
     __syncthreads();
     if (!is_last) return;
     __threadfence();
+    // moments of every (cell, species): segments summed in order
+    for (int64_t q = tid; q < (int64_t)a.C * a.S; q += UB) {
+        const int64_t cc = q / a.S, sp = q % a.S;
+        double num = 0.0, en = 0.0;
+        for (int g = 0; g < nseg; ++g) {
+            const volatile double* mp = a.mom_part + (((cc * nseg + g) * a.S) + sp) * 2;
+            num = num + mp[0];
+            en = en + mp[1];
+        }
+        a.moments[q * 2 + 0] = num;
+        a.moments[q * 2 + 1] = en;
+    }
     double v[BP_WORDS];
     for (int q = 0; q < BP_WORDS; ++q) v[q] = (q <= BP_SUMF) ? 0.0 : INFINITY;
-    for (int b = tid; b < (int)gridDim.x; b += UB) {
-        const volatile double* bp = a.block_part + (int64_t)b * BP_WORDS;
+    for (int64_t bb = tid; bb < nblocks; bb += UB) {
+        const volatile double* bp = a.block_part + bb * BP_WORDS;
         v[BP_SUMDF] = v[BP_SUMDF] + bp[BP_SUMDF];
         v[BP_SUMF] = v[BP_SUMF] + bp[BP_SUMF];
         for (int q = BP_CR; q < BP_WORDS; ++q) v[q] = cr_min(v[q], bp[q]);
@@ -191,34 +236,43 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, double* red,
     }
 }
 
+// Load the whole cell into LDS (needed by the emission GEMV and the Domain all-zero test) and
+// return whether any entry is non-zero.
+__device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
+    const int64_t base = (int64_t)c * a.N;
+    int nz = 0;
+    for (int k = threadIdx.x; k < a.N; k += UB) {
+        const double v = a.f[base + k];
+        fc[k] = v;
+        nz |= (v != 0.0);
+    }
+    return block_or(nz, ired);  // also orders the fc writes before use
+}
+
 // ----------------------------------------------------------------------- ForwardEuler ------
 __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
-    double* fc = smem;            // f of this cell
-    double* fn = smem + a.N;      // new f of this cell
+    double* fn = smem;            // new f of this block's states
+    double* fc = smem + UB;       // f of the whole cell (emission / Domain only)
     __shared__ double red[UB];
     __shared__ int ired[UB];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
+    const int k0 = blockIdx.y * UB, k1 = min(N, k0 + UB);
     const int64_t base = (int64_t)c * N;
-
-    double sumf = 0.0;
-    int nz = 0;
-    for (int k = tid; k < N; k += UB) {
-        const double v = a.f[base + k];
-        fc[k] = v;
-        sumf = sumf + v;
-        nz |= (v != 0.0);
-    }
-    const int any_nz = block_or(nz, ired);  // also orders the fc writes before use
+    const bool need_cell = a.emission || (a.binary && a.active);
+    const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
     // update_Big_Bin! evaluates a cell when Domain is nothing, the cell is in Domain, or its f
     // is all zero (src/SteppingMethods.jl:369); otherwise df_Bin keeps its zero initial value.
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
 
-    double sumdf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY, crf = INFINITY;
+    double sumdf = 0.0, sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY,
+           crf = INFINITY;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
-    for (int k = tid; k < N; k += UB) {
+    const int k = k0 + tid;
+    if (k < k1) {
         const int64_t r = base + k;
-        const double fk = fc[k];
+        const double fk = a.f[r];
+        sumf = fk;
         const double dF = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
                                       : 0.0;
         double df = -dF;
@@ -228,11 +282,11 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
             df = df + dE;
         }
         if (a.binary) {
-            dB = active ? binary_term(a, c, k, fc) : 0.0;
+            dB = active ? binary_term(a, c, k) : 0.0;
             df = df + dB;
         }
         df = df * (a.inv_ap[r] * a.s);
-        sumdf = sumdf + df;
+        sumdf = df;
         if (want_cr) {
             cr = cr_min(cr, df / fk);
             if (a.verbose == 3) {
@@ -245,11 +299,11 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
         x = strong_zero(x, x >= THR_1F_10);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
-        fn[k] = x;
-        a.fT[(int64_t)k * a.ldc + c] = x;
+        fn[tid] = x;
+        publish(a, c, k, x);
     }
     __syncthreads();
-    finish_block(a, fn, red, sumdf, sumf, cr, crb, cre, crf, INFINITY, INFINITY);
+    finish_block(a, fn, k0, k1, red, sumdf, sumf, cr, crb, cre, crf, INFINITY, INFINITY);
 }
 
 // ------------------------------------------------------------- ForwardSymplecticEuler ------
@@ -257,19 +311,16 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
 // f_Momentum = f + df_Momentum clamped at 1f-10.
 __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
-    double* fc = smem;
+    double* fc = smem + UB;
     __shared__ int ired[UB];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
+    const int k0 = blockIdx.y * UB, k1 = min(N, k0 + UB);
     const int64_t base = (int64_t)c * N;
-    int nz = 0;
-    for (int k = tid; k < N; k += UB) {
-        const double v = a.f[base + k];
-        fc[k] = v;
-        nz |= (v != 0.0);
-    }
-    const int any_nz = block_or(nz, ired);
+    const bool need_cell = a.emission || (a.binary && a.active);
+    const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
-    for (int k = tid; k < N; k += UB) {
+    const int k = k0 + tid;
+    if (k < k1) {
         const int64_t r = base + k;
         const double dP = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
                                       : 0.0;
@@ -280,14 +331,14 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
             a.demi[r] = dE;
         }
         if (a.binary) {
-            const double dB = active ? binary_term(a, c, k, fc) : 0.0;
+            const double dB = active ? binary_term(a, c, k) : 0.0;
             dm = dm + dB;
             a.dbin[r] = dB;
         }
         dm = dm * (a.inv_ap[r] * a.s);
         a.dpflux[r] = dP;
         a.df_mom[r] = dm;
-        const double x = fc[k] + dm;
+        const double x = a.f[r] + dm;
         a.f_mom[r] = strong_zero(x, x >= THR_1F_10);
     }
 }
@@ -298,14 +349,16 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
     double* fn = smem;
     __shared__ double red[UB];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
+    const int k0 = blockIdx.y * UB, k1 = min(N, k0 + UB);
     const int64_t base = (int64_t)c * N;
     double sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY, crf = INFINITY;
     double crm = INFINITY, crs = INFINITY;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
-    for (int k = tid; k < N; k += UB) {
+    const int k = k0 + tid;
+    if (k < k1) {
         const int64_t r = base + k;
         const double f0 = a.f[r];
-        sumf = sumf + f0;
+        sumf = f0;
         const double fm = a.f_mom[r];
         const double dX = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r,
                                                 a.f_mom)
@@ -321,7 +374,7 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
                 if (a.emission) cre = cr_min(cre, ((a.demi[r] * a.inv_ap[r]) * a.s) / f0);
                 double rp = (((-a.dpflux[r]) * a.inv_ap[r]) * a.s) / f0;
                 if (isnan(rp)) rp = 0.0;  // replace!(df_tmp, NaN => 0.0)
-                if (rp < crf) crf = rp;
+                crf = cr_min(crf, rp);
                 crm = cr_min(crm, dm / f0);
                 crs = cr_min(crs, dS / fm);
             }
@@ -330,11 +383,38 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
         x = strong_zero(x, x >= THR_1F_20) * jl_sign(x);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
-        fn[k] = x;
-        a.fT[(int64_t)k * a.ldc + c] = x;
+        fn[tid] = x;
+        publish(a, c, k, x);
     }
     __syncthreads();
-    finish_block(a, fn, red, 0.0, sumf, cr, crb, cre, crf, crm, crs);
+    finish_block(a, fn, k0, k1, red, 0.0, sumf, cr, crb, cre, crf, crm, crs);
+}
+
+__global__ void publish_state_kernel(UpdateArgs a) {
+    const int64_t total = (int64_t)a.N * a.C;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+        publish(a, (int)(i / a.N), (int)(i % a.N), a.f[i]);
+}
+
+hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
+                                int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
+                                hipStream_t stream) {
+    UpdateArgs a{};
+    a.f = f;
+    a.N = N;
+    a.C = C;
+    a.ldc = ldc;
+    a.ldb = Np;
+    a.pad_index = pad_index;
+    a.fT = fT;
+    a.fT32 = fT32;
+    a.fhi = fhi;
+    a.flo = flo;
+    const int64_t total = (int64_t)N * C;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+    publish_state_kernel<<<blocks, 256, 0, stream>>>(a);
+    return hipGetLastError();
 }
 
 __global__ void fill_zero_kernel(double* p, int64_t count) {
@@ -343,22 +423,24 @@ __global__ void fill_zero_kernel(double* p, int64_t count) {
         p[i] = 0.0;
 }
 
-static size_t update_smem(const UpdateArgs& a, int arrays) {
-    return (size_t)arrays * a.N * sizeof(double);
+static size_t update_smem(const UpdateArgs& a) {
+    return (size_t)(UB + a.N) * sizeof(double);
 }
 
+static dim3 update_grid(const UpdateArgs& a) { return dim3(a.C, (a.N + UB - 1) / UB); }
+
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
-    euler_update_kernel<<<a.C, UB, update_smem(a, 2), stream>>>(a);
+    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
     return hipGetLastError();
 }
 
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream) {
-    symplectic_momentum_kernel<<<a.C, UB, update_smem(a, 1), stream>>>(a);
+    symplectic_momentum_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
     return hipGetLastError();
 }
 
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream) {
-    symplectic_space_kernel<<<a.C, UB, update_smem(a, 1), stream>>>(a);
+    symplectic_space_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
     return hipGetLastError();
 }
 

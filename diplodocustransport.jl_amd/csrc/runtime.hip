@@ -32,7 +32,7 @@ std::string g_create_error;
 struct HostChannel {
     int32_t s_out, s1, s2, kind;
     int32_t No, N1, N2;
-    int64_t npairs, npairs_pad;
+    int64_t npairs, npairs_pad;  // the pair list is not padded: npairs_pad == npairs
     std::vector<double> S;  // [pair][row] FP64
     std::vector<uint32_t> pairs;
 };
@@ -61,11 +61,20 @@ struct dtx_handle {
     bool loss_nonzero = false;
     bool synth_binary = false;      // channels generated on the device
 
-    // device: binary operator and its work table
+    // padded state coordinates of the contraction's B operand (species start on SPECIES_ALIGN)
+    int32_t Np = 0;
+    std::vector<int32_t> sp_pad_off, pad_index, unpad;
+    int32_t* d_pad_index = nullptr;
+    int32_t* d_unpad = nullptr;
+
+    // device: binary operator and its work tables (gain items in the contraction precision,
+    // loss items always FP64)
     std::vector<PackedChannel> chans;
     double* d_loss = nullptr;
+    bool loss_present = false;
     BinWork* d_items = nullptr;
-    int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0;
+    BinWork* d_items_loss = nullptr;
+    int32_t nitems = 0, nitems_loss = 0, tc = 16, ldc = 0, max_slots = 0;
     double* d_part = nullptr;
     int64_t part_stride = 0;
     int32_t* d_nslot = nullptr;
@@ -75,11 +84,15 @@ struct dtx_handle {
     // device: state and vectors
     double* d_f[2] = {nullptr, nullptr};
     int cur = 0;
-    double* d_fT = nullptr;
+    double* d_fT = nullptr;       // FP64, state-major, padded states: [Np][ldc]
+    float* d_fT32 = nullptr;      // unused: the FP32 contraction reads the FP64 copy
+    uint16_t* d_fhi = nullptr;    // BF16 split, cell-major: [ldc][Np]
+    uint16_t* d_flo = nullptr;
     double *d_invap = nullptr, *d_dfinj = nullptr, *d_vol = nullptr, *d_finit = nullptr;
     double* d_eweight = nullptr;
     int32_t *d_state_species = nullptr, *d_species_off = nullptr;
-    double *d_moments = nullptr, *d_block_part = nullptr, *d_status = nullptr;
+    double *d_moments = nullptr, *d_mom_part = nullptr, *d_block_part = nullptr,
+           *d_status = nullptr;
     uint32_t* d_counter = nullptr;
     uint8_t* d_active = nullptr;
     EllMatrix ell[3];
@@ -174,7 +187,7 @@ uint64_t dtx::splitmix_key(uint64_t seed, uint64_t stream) {
 }
 
 // Build the pair enumeration of a channel: TRI (i <= j, i outer) or RECT (i in s1 outer).
-// Entries are cell-local state indices; the table is padded with PAIR_PAD to PAIR_ALIGN.
+// Entries are cell-local state indices (i | j << 16), in row-major pair order.
 static void build_pairs(const dtx_handle* h, HostChannel& ch) {
     ch.pairs.clear();
     const int o1 = h->sp_off[ch.s1], o2 = h->sp_off[ch.s2];
@@ -188,8 +201,27 @@ static void build_pairs(const dtx_handle* h, HostChannel& ch) {
                 ch.pairs.push_back((uint32_t)(o1 + i) | ((uint32_t)(o2 + j) << 16));
     }
     ch.npairs = (int64_t)ch.pairs.size();
-    ch.npairs_pad = (ch.npairs + PAIR_ALIGN - 1) / PAIR_ALIGN * PAIR_ALIGN;
-    ch.pairs.resize((size_t)ch.npairs_pad, PAIR_PAD);
+    ch.npairs_pad = ch.npairs;
+}
+
+// Chunk table of a channel: rows i (padded index) cut into KC-aligned j chunks.  TRI rows start
+// at the KC-aligned block holding i; the A entries of j < i or j past the species are zero.
+static std::vector<uint32_t> chunk_table(const dtx_handle* h, int kind, int s1, int s2) {
+    std::vector<uint32_t> t;
+    const int N1 = h->sp_size[s1], N2 = h->sp_size[s2];
+    const int P1 = h->sp_pad_off[s1], P2 = h->sp_pad_off[s2];
+    if (kind == CH_TRI) {
+        for (int i = 0; i < N1; ++i)
+            for (int jb = i / KC * KC; jb < N1; jb += KC)
+                t.push_back((uint32_t)(P1 + i) | ((uint32_t)(P1 + jb) << 16));
+    } else if (kind == CH_RECT) {
+        for (int i = 0; i < N1; ++i)
+            for (int jb = 0; jb < N2; jb += KC)
+                t.push_back((uint32_t)(P1 + i) | ((uint32_t)(P2 + jb) << 16));
+    } else {
+        for (int jb = 0; jb < h->Np; jb += KC) t.push_back((uint32_t)jb << 16);
+    }
+    return t;
 }
 
 static HostChannel& host_channel(dtx_handle* h, int s_out, int s1, int s2) {
@@ -217,136 +249,195 @@ static void release_binary_device(dtx_handle* h) {
             auto it = std::find(h->allocations.begin(), h->allocations.end(), c.A);
             if (it != h->allocations.end()) h->allocations.erase(it);
         }
-        if (c.pairs) {
-            (void)hipFree(c.pairs);
-            h->device_bytes -= c.npairs_pad * 4;
-            auto it = std::find(h->allocations.begin(), h->allocations.end(), (void*)c.pairs);
+        if (c.chunks) {
+            (void)hipFree(c.chunks);
+            h->device_bytes -= c.nchunks * 4;
+            auto it = std::find(h->allocations.begin(), h->allocations.end(), (void*)c.chunks);
             if (it != h->allocations.end()) h->allocations.erase(it);
         }
     }
     h->chans.clear();
 }
 
-static int64_t packed_elems(int precision, int64_t npairs_pad, int nktiles) {
-    // FP64/FP32: [ktile][pair][TK];  BF16X3: [ktile][chunk][hi TK*32 | lo TK*32]
-    int64_t per_tile = npairs_pad * TK * (precision == DTX_PREC_BF16X3 ? 2 : 1);
-    return per_tile * nktiles;
+static int64_t chunk_elems(int precision) {
+    // FP64/FP32: [t][r] (KC x TK);  BF16X3: hi [r][t] then lo [r][t]
+    return (int64_t)KC * TK * (precision == DTX_PREC_BF16X3 ? 2 : 1);
 }
 
 static int64_t elem_bytes(int precision) {
     return precision == DTX_PREC_FP64 ? 8 : (precision == DTX_PREC_FP32 ? 4 : 2);
 }
 
-// Pack an FP64 [pair][row] channel into the device layout of the handle's precision.
+// Allocate the device arrays of a channel and upload its chunk table.
+static int alloc_channel(dtx_handle* h, PackedChannel& pc, const std::vector<uint32_t>& table) {
+    pc.nchunks = (int64_t)table.size();
+    pc.a_bytes = chunk_elems(pc.precision) * pc.nchunks * pc.nktiles * elem_bytes(pc.precision);
+    hipError_t e = hipMalloc(&pc.A, (size_t)std::max<int64_t>(pc.a_bytes, 1));
+    if (e != hipSuccess) {
+        pc.A = nullptr;
+        return fail(h, DTX_ERR_OOM, "hipMalloc(%lld) for a binary channel failed: %s",
+                    (long long)pc.a_bytes, hipGetErrorString(e));
+    }
+    h->device_bytes += pc.a_bytes;
+    h->allocations.push_back(pc.A);
+    e = hipMalloc((void**)&pc.chunks, (size_t)std::max<int64_t>(pc.nchunks, 1) * 4);
+    if (e != hipSuccess) {
+        pc.chunks = nullptr;
+        return fail(h, DTX_ERR_OOM, "hipMalloc for a chunk table failed: %s",
+                    hipGetErrorString(e));
+    }
+    h->device_bytes += pc.nchunks * 4;
+    h->allocations.push_back(pc.chunks);
+    HIP_CHECK(h, hipMemcpy(pc.chunks, table.data(), (size_t)pc.nchunks * 4,
+                           hipMemcpyHostToDevice));
+    return DTX_OK;
+}
+
+static int64_t tri_index(int i, int j, int n) {
+    return (int64_t)i * n - (int64_t)i * (i - 1) / 2 + (j - i);
+}
+
+// Pack an FP64 pair-ordered channel into the chunked device layout of the handle's precision.
 static int pack_upload_channel(dtx_handle* h, const HostChannel& hc) {
     PackedChannel pc;
     pc.s_out = hc.s_out;
     pc.s1 = hc.s1;
     pc.s2 = hc.s2;
     pc.kind = hc.kind;
+    pc.precision = h->precision;
     pc.No = hc.No;
     pc.N1 = hc.N1;
     pc.N2 = hc.N2;
     pc.npairs = hc.npairs;
-    pc.npairs_pad = hc.npairs_pad;
     pc.nktiles = (hc.No + TK - 1) / TK;
-    const int64_t elems = packed_elems(h->precision, pc.npairs_pad, pc.nktiles);
-    const int64_t eb = elem_bytes(h->precision);
-    pc.a_bytes = elems * eb;
-    std::vector<uint8_t> buf((size_t)pc.a_bytes, 0);
-    const int64_t P = pc.npairs_pad;
-    for (int kt = 0; kt < pc.nktiles; ++kt) {
-        for (int64_t p = 0; p < P; ++p) {
-            for (int r = 0; r < TK; ++r) {
-                const int row = kt * TK + r;
-                const double v = (row < hc.No) ? hc.S[(size_t)p * hc.No + row] : 0.0;
-                if (h->precision == DTX_PREC_FP64) {
-                    ((double*)buf.data())[(kt * P + p) * TK + r] = v;
-                } else if (h->precision == DTX_PREC_FP32) {
-                    ((float*)buf.data())[(kt * P + p) * TK + r] = (float)v;
-                } else {
-                    uint16_t hi, lo;
-                    split_bf16(v, &hi, &lo);
-                    const int64_t chunk = p / 32, pp = p % 32;
-                    uint16_t* base = (uint16_t*)buf.data() + kt * P * TK * 2 + chunk * TK * 64;
-                    base[r * 32 + pp] = hi;
-                    base[TK * 32 + r * 32 + pp] = lo;
+    const std::vector<uint32_t> table = chunk_table(h, hc.kind, hc.s1, hc.s2);
+    const int P1 = h->sp_pad_off[hc.s1], P2 = h->sp_pad_off[hc.s2];
+    const int64_t NCH = (int64_t)table.size();
+    const int64_t CE = chunk_elems(h->precision);
+    std::vector<uint8_t> buf((size_t)(CE * NCH * pc.nktiles * elem_bytes(h->precision)), 0);
+    for (int kt = 0; kt < pc.nktiles; ++kt)
+        for (int64_t ch = 0; ch < NCH; ++ch) {
+            const int i = (int)(table[ch] & 0xFFFFu) - P1;
+            const int jb = (int)(table[ch] >> 16) - (hc.kind == CH_TRI ? P1 : P2);
+            const int64_t base = ((int64_t)kt * NCH + ch) * CE;
+            for (int t = 0; t < KC; ++t) {
+                const int j = jb + t;
+                int64_t p = -1;
+                if (hc.kind == CH_TRI) {
+                    if (j >= i && j < hc.N1) p = tri_index(i, j, hc.N1);
+                } else if (j < hc.N2) {
+                    p = (int64_t)i * hc.N2 + j;
+                }
+                for (int r = 0; r < TK; ++r) {
+                    const int row = kt * TK + r;
+                    const double v = (p >= 0 && row < hc.No) ? hc.S[(size_t)p * hc.No + row] : 0.0;
+                    if (h->precision == DTX_PREC_FP64) {
+                        ((double*)buf.data())[base + t * TK + r] = v;
+                    } else if (h->precision == DTX_PREC_FP32) {
+                        ((float*)buf.data())[base + t * TK + r] = (float)v;
+                    } else {
+                        uint16_t hi, lo;
+                        split_bf16(v, &hi, &lo);
+                        ((uint16_t*)buf.data())[base + r * KC + t] = hi;
+                        ((uint16_t*)buf.data())[base + TK * KC + r * KC + t] = lo;
+                    }
                 }
             }
         }
-    }
-    hipError_t e = hipMalloc(&pc.A, (size_t)pc.a_bytes);
-    if (e != hipSuccess)
-        return fail(h, DTX_ERR_OOM, "hipMalloc(%lld) for a gain channel failed: %s",
-                    (long long)pc.a_bytes, hipGetErrorString(e));
-    h->device_bytes += pc.a_bytes;
-    h->allocations.push_back(pc.A);
-    e = hipMalloc((void**)&pc.pairs, (size_t)P * 4);
-    if (e != hipSuccess) {
-        h->chans.push_back(pc);
-        return fail(h, DTX_ERR_OOM, "hipMalloc for a pair table failed: %s",
-                    hipGetErrorString(e));
-    }
-    h->device_bytes += P * 4;
-    h->allocations.push_back(pc.pairs);
+    int rc = alloc_channel(h, pc, table);
     h->chans.push_back(pc);
+    if (rc) return rc;
     HIP_CHECK(h, hipMemcpy(pc.A, buf.data(), (size_t)pc.a_bytes, hipMemcpyHostToDevice));
-    HIP_CHECK(h, hipMemcpy(pc.pairs, hc.pairs.data(), (size_t)P * 4, hipMemcpyHostToDevice));
     return DTX_OK;
 }
 
-// Build the grouped work table for the handle's channels and cell count.
+// LOSS channels: one per species, rows of the species against all (padded) states, FP64,
+// packed on the device from the loss matrix d_loss.
+static int pack_loss_channels(dtx_handle* h) {
+    if (!h->d_loss) return DTX_OK;
+    for (int s = 0; s < h->S; ++s) {
+        PackedChannel pc;
+        pc.s_out = s;
+        pc.s1 = s;
+        pc.s2 = s;
+        pc.kind = CH_LOSS;
+        pc.precision = DTX_PREC_FP64;
+        pc.No = h->sp_size[s];
+        pc.N1 = h->sp_size[s];
+        pc.N2 = h->N;
+        pc.npairs = (int64_t)pc.No * h->N;
+        pc.nktiles = (pc.No + TK - 1) / TK;
+        int rc = alloc_channel(h, pc, chunk_table(h, CH_LOSS, s, s));
+        h->chans.push_back(pc);
+        if (rc) return rc;
+        HIP_CHECK(h, launch_pack_loss(h->d_loss, h->N, h->d_unpad, h->Np, h->sp_off[s], pc.No,
+                                      pc.nktiles, (double*)pc.A, h->stream));
+    }
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return DTX_OK;
+}
+
+// Build the grouped work tables for the handle's channels and cell count.
 static int build_work(dtx_handle* h) {
     dfree(h, &h->d_items, h->nitems);
+    dfree(h, &h->d_items_loss, h->nitems_loss);
     dfree(h, &h->d_part, h->part_stride * h->max_slots);
     dfree(h, &h->d_nslot, h->N);
-    h->nitems = 0;
+    h->nitems = h->nitems_loss = 0;
     h->max_slots = 0;
     const int C = h->C;
-    h->tc = (C <= 16) ? 16 : 64;
     const int ctiles = (C + h->tc - 1) / h->tc;
-    h->ldc = ctiles * h->tc;
     double total = 0;
     h->pairs_total = 0;
     h->bin_flops = 0;
     h->gain_bytes = 0;
     for (auto& c : h->chans) {
-        total += (double)c.nktiles * ctiles * c.npairs_pad;
-        h->pairs_total += c.npairs;
-        h->bin_flops += 2.0 * c.No * (double)c.npairs * C;
+        total += (double)c.nktiles * ctiles * c.nchunks;
+        if (c.kind != CH_LOSS) {
+            h->pairs_total += c.npairs;
+            h->bin_flops += 2.0 * c.No * (double)c.npairs * C;
+        } else {
+            h->bin_flops += 2.0 * c.No * (double)h->N * C;
+        }
         h->gain_bytes += (double)c.a_bytes;
     }
     const int target_items = 1024;
-    int64_t chunk = (int64_t)std::ceil(total / target_items);
-    chunk = std::max<int64_t>(512, (chunk + PAIR_ALIGN - 1) / PAIR_ALIGN * PAIR_ALIGN);
+    int64_t range = (int64_t)std::ceil(total / target_items);
+    range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
     std::vector<int> next_slot(h->S, 0);
-    std::vector<BinWork> items;
-    const int64_t cellN = (int64_t)C * h->N;
-    h->part_stride = cellN;
-    // slots are assigned first; the partial pointer is patched once the buffer exists
-    struct Pending { BinWork w; int slot; };
+    h->part_stride = (int64_t)C * h->N;
+    struct Pending { BinWork w; int slot; bool loss; };
     std::vector<Pending> pend;
+    const bool bf16 = h->precision == DTX_PREC_BF16X3;
     for (auto& c : h->chans) {
-        for (int64_t p0 = 0; p0 < c.npairs_pad; p0 += chunk) {
-            const int64_t p1 = std::min<int64_t>(p0 + chunk, c.npairs_pad);
+        const bool loss = c.kind == CH_LOSS;
+        const int prec = loss ? DTX_PREC_FP64 : h->precision;
+        for (int64_t c0 = 0; c0 < c.nchunks; c0 += range) {
+            const int64_t c1 = std::min<int64_t>(c0 + range, c.nchunks);
             const int slot = next_slot[c.s_out]++;
             for (int kt = 0; kt < c.nktiles; ++kt)
                 for (int ct = 0; ct < ctiles; ++ct) {
                     BinWork w{};
                     w.A = c.A;
-                    w.pairs = c.pairs;
+                    w.chunks = c.chunks;
                     w.fT = h->d_fT;
-                    w.a_ktile_stride = packed_elems(h->precision, c.npairs_pad, 1);
+                    if (prec == DTX_PREC_FP64) w.B = h->d_fT;
+                    else w.B = h->d_fhi;
+                    w.b_lo = bf16 ? (int64_t)(h->d_flo - h->d_fhi) : 0;
+                    w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
                     w.ktile = kt;
                     w.ctile = ct;
-                    w.p_begin = (int32_t)p0;
-                    w.p_end = (int32_t)p1;
+                    w.c_begin = (int32_t)c0;
+                    w.c_end = (int32_t)c1;
                     w.row_base = h->sp_off[c.s_out];
+                    w.row_pad_base = h->sp_pad_off[c.s_out];
                     w.rows = c.No;
                     w.cells = C;
                     w.ldc = h->ldc;
+                    w.ldb = h->Np;
                     w.N = h->N;
-                    pend.push_back({w, slot});
+                    w.kind = c.kind;
+                    pend.push_back({w, slot, loss && h->precision != DTX_PREC_FP64});
                 }
         }
     }
@@ -355,15 +446,21 @@ static int build_work(dtx_handle* h) {
         int rc = dalloc(h, &h->d_part, h->part_stride * h->max_slots);
         if (rc) return rc;
     }
+    std::vector<BinWork> items, items_loss;
     for (auto& pd : pend) {
         pd.w.part = h->d_part + (int64_t)pd.slot * h->part_stride;
-        items.push_back(pd.w);
+        (pd.loss ? items_loss : items).push_back(pd.w);
     }
     h->nitems = (int32_t)items.size();
+    h->nitems_loss = (int32_t)items_loss.size();
     if (h->nitems) {
         int rc = dalloc(h, &h->d_items, h->nitems);
+        if (!rc) rc = upload(h, h->d_items, items.data(), h->nitems);
         if (rc) return rc;
-        rc = upload(h, h->d_items, items.data(), h->nitems);
+    }
+    if (h->nitems_loss) {
+        int rc = dalloc(h, &h->d_items_loss, h->nitems_loss);
+        if (!rc) rc = upload(h, h->d_items_loss, items_loss.data(), h->nitems_loss);
         if (rc) return rc;
     }
     std::vector<int32_t> nslot(h->N, 0);
@@ -398,26 +495,20 @@ static int finalize_binary(dtx_handle* h) {
         h->loss_host.clear();
         h->loss_host.shrink_to_fit();
     }
-    int rc = build_work(h);
+    int rc = pack_loss_channels(h);
+    if (rc) return rc;
+    h->loss_present = h->d_loss != nullptr;
+    dfree(h, &h->d_loss, (int64_t)h->N * h->N);
+    rc = build_work(h);
     if (rc) return rc;
     h->binary_dirty = false;
     return DTX_OK;
 }
 
-// fT[k * ldc + c] = f[c * N + k]
-__global__ void transpose_state_kernel(const double* f, double* fT, int N, int C, int ldc) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)N * C) return;
-    int c = (int)(i / N), k = (int)(i % N);
-    fT[(int64_t)k * ldc + c] = f[i];
-}
-
+// Publish the current f into the contraction's B-operand copies (padded, per precision).
 static int refresh_fT(dtx_handle* h) {
-    int64_t total = (int64_t)h->N * h->C;
-    int blocks = (int)((total + 255) / 256);
-    transpose_state_kernel<<<blocks, 256, 0, h->stream>>>(h->d_f[h->cur], h->d_fT, h->N, h->C,
-                                                          h->ldc);
-    HIP_CHECK(h, hipGetLastError());
+    HIP_CHECK(h, launch_publish_state(h->d_f[h->cur], h->N, h->C, h->ldc, h->d_pad_index, h->Np,
+                                      h->d_fT, h->d_fT32, h->d_fhi, h->d_flo, h->stream));
     return DTX_OK;
 }
 
@@ -514,13 +605,17 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.f = h->d_f[h->cur];
     a.f_out = h->d_f[h->cur ^ 1];
     a.fT = h->d_fT;
+    a.fT32 = h->d_fT32;
+    a.fhi = h->d_fhi;
+    a.flo = h->d_flo;
+    a.pad_index = h->d_pad_index;
+    a.ldb = h->Np;
     a.inv_ap = h->d_invap;
     a.df_inj = h->d_dfinj;
     a.vol = h->d_vol;
     a.part = h->d_part;
     a.nslot = h->d_nslot;
     a.part_stride = h->part_stride;
-    a.loss = h->d_loss;
     a.emi = h->d_emi;
     a.active = h->d_active;
     const EllMatrix& E = h->ell[which_ell];
@@ -531,6 +626,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.state_species = h->d_state_species;
     a.species_off = h->d_species_off;
     a.moments = h->d_moments;
+    a.mom_part = h->d_mom_part;
     a.block_part = h->d_block_part;
     a.status = h->d_status;
     a.counter = h->d_counter;
@@ -544,7 +640,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.C = h->C;
     a.S = h->S;
     a.ldc = h->ldc;
-    a.binary = h->binary && (h->nitems > 0 || h->d_loss != nullptr);
+    a.binary = h->binary && (h->nitems + h->nitems_loss > 0);
     a.emission = h->emission;
     a.verbose = verbose;
     a.step_index = (int32_t)(h->step_counter % 0x7FFFFFFF);
@@ -575,12 +671,14 @@ static int timed_end(dtx_handle* h, TimedEvent* ev) {
 // Enqueue one functor call with an already clipped dt.
 static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save) {
     const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
-    const bool bin = h->binary && h->nitems > 0;
+    const bool bin = h->binary && (h->nitems + h->nitems_loss) > 0;
     TimedEvent ev{};
     if (bin) {
         int rc = timed_begin(h, 0, &ev);
         if (rc) return rc;
         HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
+        HIP_CHECK(h, launch_binary(DTX_PREC_FP64, h->tc, h->d_items_loss, h->nitems_loss,
+                                   h->stream));
         rc = timed_end(h, &ev);
         if (rc) return rc;
     }
@@ -700,6 +798,20 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     h->method = cfg->method;
     h->tc = (h->C <= 16) ? 16 : 64;
     h->ldc = (h->C + h->tc - 1) / h->tc * h->tc;
+    h->sp_pad_off.assign(h->S + 1, 0);
+    for (int s = 0; s < h->S; ++s)
+        h->sp_pad_off[s + 1] = h->sp_pad_off[s] +
+                               (h->sp_size[s] + SPECIES_ALIGN - 1) / SPECIES_ALIGN * SPECIES_ALIGN;
+    h->Np = h->sp_pad_off[h->S];
+    if (h->Np > 65535)
+        return fail(nullptr, DTX_ERR_UNSUPPORTED, "padded state count exceeds 65535");
+    h->pad_index.assign((size_t)N, 0);
+    h->unpad.assign((size_t)h->Np, -1);
+    for (int s = 0; s < h->S; ++s)
+        for (int k = 0; k < h->sp_size[s]; ++k) {
+            h->pad_index[(size_t)(h->sp_off[s] + k)] = h->sp_pad_off[s] + k;
+            h->unpad[(size_t)(h->sp_pad_off[s] + k)] = h->sp_off[s] + k;
+        }
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess)
         return fail(nullptr, DTX_ERR_HIP, "hipSetDevice(%d): %s", cfg->device,
@@ -712,7 +824,13 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     const int64_t n = h->n;
     rc |= dalloc(hp, &h->d_f[0], n);
     rc |= dalloc(hp, &h->d_f[1], n);
-    rc |= dalloc(hp, &h->d_fT, (int64_t)h->ldc * h->N);
+    rc |= dalloc(hp, &h->d_fT, (int64_t)h->ldc * h->Np);
+    if (h->precision == DTX_PREC_BF16X3) {
+        rc |= dalloc(hp, &h->d_fhi, (int64_t)h->ldc * h->Np);
+        rc |= dalloc(hp, &h->d_flo, (int64_t)h->ldc * h->Np);
+    }
+    rc |= dalloc(hp, &h->d_pad_index, N);
+    rc |= dalloc(hp, &h->d_unpad, h->Np);
     rc |= dalloc(hp, &h->d_invap, n);
     rc |= dalloc(hp, &h->d_dfinj, n);
     rc |= dalloc(hp, &h->d_vol, C);
@@ -720,7 +838,9 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_state_species, N);
     rc |= dalloc(hp, &h->d_species_off, h->S + 1);
     rc |= dalloc(hp, &h->d_moments, C * h->S * 2);
-    rc |= dalloc(hp, &h->d_block_part, C * BP_WORDS);
+    const int64_t nseg = (N + 255) / 256;  // update kernels: one block per 256 states
+    rc |= dalloc(hp, &h->d_block_part, C * nseg * BP_WORDS);
+    rc |= dalloc(hp, &h->d_mom_part, C * nseg * h->S * 2);
     rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
     rc |= dalloc(hp, &h->d_counter, 1);
     if (h->method == DTX_METHOD_FORWARD_SYMPLECTIC_EULER) {
@@ -739,10 +859,18 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     for (int s = 0; s < h->S; ++s)
         for (int k = 0; k < h->sp_size[s]; ++k) ss[(size_t)(h->sp_off[s] + k)] = s;
     rc = upload(hp, h->d_state_species, ss.data(), N);
+    if (!rc) rc = upload(hp, h->d_pad_index, h->pad_index.data(), N);
+    if (!rc) rc = upload(hp, h->d_unpad, h->unpad.data(), h->Np);
     if (!rc) rc = upload(hp, h->d_species_off, h->sp_off.data(), h->S + 1);
     if (!rc) {
         e = hipMemsetAsync(h->d_counter, 0, sizeof(uint32_t), h->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * N * 8, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * h->Np * 8, h->stream);
+        if (e == hipSuccess && h->d_fT32)
+            e = hipMemsetAsync(h->d_fT32, 0, (size_t)h->ldc * h->Np * 4, h->stream);
+        if (e == hipSuccess && h->d_fhi)
+            e = hipMemsetAsync(h->d_fhi, 0, (size_t)h->ldc * h->Np * 2, h->stream);
+        if (e == hipSuccess && h->d_flo)
+            e = hipMemsetAsync(h->d_flo, 0, (size_t)h->ldc * h->Np * 2, h->stream);
         if (e == hipSuccess) e = hipMemsetAsync(h->d_status, 0,
                                                 (size_t)STATUS_RING * STATUS_WORDS * 8,
                                                 h->stream);
@@ -987,6 +1115,7 @@ int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
     if (rc) return rc;
     HIP_CHECK(h, launch_fill_zero(h->d_loss, (int64_t)h->N * h->N, h->stream));
     uint64_t stream = 0;
+    // interactions a + b -> a + b for every species pair a <= b (one gain block per output)
     for (int a = 0; a < h->S; ++a)
         for (int b = a; b < h->S; ++b) {
             const int outs[2] = {a, b};
@@ -994,57 +1123,30 @@ int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
             for (int q = 0; q < nout; ++q) {
                 const int so = outs[q];
                 const uint64_t key = splitmix_key(seed, stream++);
-                HostChannel hc;
-                hc.s_out = so;
-                hc.s1 = a;
-                hc.s2 = b;
-                hc.kind = (a == b) ? CH_TRI : CH_RECT;
-                hc.No = h->sp_size[so];
-                hc.N1 = h->sp_size[a];
-                hc.N2 = h->sp_size[b];
-                build_pairs(h, hc);
                 PackedChannel pc;
                 pc.s_out = so;
                 pc.s1 = a;
                 pc.s2 = b;
-                pc.kind = hc.kind;
-                pc.No = hc.No;
-                pc.N1 = hc.N1;
-                pc.N2 = hc.N2;
-                pc.npairs = hc.npairs;
-                pc.npairs_pad = hc.npairs_pad;
-                pc.nktiles = (hc.No + TK - 1) / TK;
-                pc.a_bytes = packed_elems(h->precision, pc.npairs_pad, pc.nktiles) *
-                             elem_bytes(h->precision);
-                hipError_t e = hipMalloc(&pc.A, (size_t)pc.a_bytes);
-                if (e != hipSuccess)
-                    return fail(h, DTX_ERR_OOM, "hipMalloc(%lld) for a synthetic channel: %s",
-                                (long long)pc.a_bytes, hipGetErrorString(e));
-                h->device_bytes += pc.a_bytes;
-                h->allocations.push_back(pc.A);
-                e = hipMalloc((void**)&pc.pairs, (size_t)pc.npairs_pad * 4);
-                if (e != hipSuccess) {
-                    h->chans.push_back(pc);
-                    return fail(h, DTX_ERR_OOM, "hipMalloc for a pair table: %s",
-                                hipGetErrorString(e));
-                }
-                h->device_bytes += pc.npairs_pad * 4;
-                h->allocations.push_back(pc.pairs);
+                pc.kind = (a == b) ? CH_TRI : CH_RECT;
+                pc.precision = h->precision;
+                pc.No = h->sp_size[so];
+                pc.N1 = h->sp_size[a];
+                pc.N2 = h->sp_size[b];
+                pc.npairs = (a == b) ? (int64_t)pc.N1 * (pc.N1 + 1) / 2 : (int64_t)pc.N1 * pc.N2;
+                pc.nktiles = (pc.No + TK - 1) / TK;
+                rc = alloc_channel(h, pc, chunk_table(h, pc.kind, a, b));
                 h->chans.push_back(pc);
-                HIP_CHECK(h, hipMemcpy(pc.pairs, hc.pairs.data(), (size_t)pc.npairs_pad * 4,
-                                       hipMemcpyHostToDevice));
-                HIP_CHECK(h, launch_synth_gain_packed(h->precision, pc.A, pc.pairs,
-                                                      pc.npairs_pad, pc.No, pc.nktiles,
-                                                      pc.kind, pc.N1, pc.N2, key, scale,
-                                                      h->stream));
-                // loss of the species whose particle leaves: out a -> loss on a (i), out b ->
-                // loss on b (j); a == b -> one loss over (i, j)
-                const int oo = h->sp_off[so];
+                if (rc) return rc;
+                HIP_CHECK(h, launch_synth_gain_packed(h->precision, pc.A, pc.chunks, pc.nchunks,
+                                                      pc.No, pc.nktiles, pc.kind, pc.N1, pc.N2,
+                                                      h->sp_pad_off[a], h->sp_pad_off[b], key,
+                                                      scale, h->stream));
+                // the particle of the output species leaves: out a -> loss on a (rows i),
+                // out b -> loss on b (rows j); a == b -> one loss over (i, j)
                 const int loss_on_first = (a == b) || (so == a);
-                HIP_CHECK(h, launch_synth_loss(h->d_loss, h->N, h->sp_off[a], hc.N1,
-                                               h->sp_off[b], hc.N2, hc.No, key, scale,
+                HIP_CHECK(h, launch_synth_loss(h->d_loss, h->N, h->sp_off[a], pc.N1,
+                                               h->sp_off[b], pc.N2, pc.No, key, scale,
                                                loss_on_first, h->stream));
-                (void)oo;
             }
         }
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
@@ -1227,14 +1329,14 @@ int dtx_work(const dtx_handle* hc, dtx_work_info* out) {
     out->bin_launch_items = h->nitems;
     out->gain_bytes = h->gain_bytes;
     if (h->binary) {
-        out->bin_flops = h->bin_flops + (h->d_loss ? 2.0 * N * N * C : 0.0);
-        // packed gain once + pair tables + f reads (fT) + partial writes
+        out->bin_flops = h->bin_flops;  // gain contraction + FP64 loss channels
+        // packed operator once + chunk tables + f reads (fT) + partial writes
         double pair_bytes = 0;
-        for (auto& c : h->chans) pair_bytes += c.npairs_pad * 4.0;
+        for (auto& c : h->chans) pair_bytes += c.nchunks * 4.0;
         out->bin_bytes = h->gain_bytes + pair_bytes + C * N * 8.0 +
                          (double)h->max_slots * C * N * 8.0;
     }
-    // update: f, f_out, fT, invAp, df_inj, ELL (col+val), partial reads, loss (L2), moments
+    // update: f, f_out, fT, invAp, df_inj, ELL (col+val), partial reads, moments
     double ell_bytes = 0;
     for (int w = 0; w < 3; ++w) ell_bytes += (double)h->ell[w].width * h->ell[w].n * 12.0;
     out->update_bytes = h->n * 8.0 * 5 + ell_bytes + (double)h->max_slots * C * N * 8.0;

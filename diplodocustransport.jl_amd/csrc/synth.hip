@@ -27,43 +27,48 @@ __device__ inline double d_lognormal(uint64_t key, uint64_t idx) {
 
 struct SynthArgs {
     void* A;
-    const uint32_t* pairs;
-    int64_t npairs_pad;
-    int No, nktiles, kind, N1, N2, o1, o2, precision;
+    const uint32_t* chunks;
+    int64_t nchunks;
+    int No, nktiles, kind, N1, N2, pad1, pad2, precision;
     uint64_t key;
     double scale;
 };
 
-// One thread per (packed row r of k-tile kt, pair p).
+// One thread per packed element (k-tile, chunk, j offset t, row r) of the chunked layout.
 __global__ void synth_gain_kernel(SynthArgs s) {
-    const int64_t total = (int64_t)s.nktiles * s.npairs_pad * TK;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % TK);
-        const int64_t p = (t / TK) % s.npairs_pad;
-        const int kt = (int)(t / (TK * s.npairs_pad));
+    const int64_t per_tile = s.nchunks * KC * TK;
+    const int64_t total = (int64_t)s.nktiles * per_tile;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int kt = (int)(e / per_tile);
+        const int64_t rem = e % per_tile;
+        const int64_t ch = rem / (KC * TK);
+        const int t = (int)((rem / TK) % KC);
+        const int r = (int)(rem % TK);
         const int k = kt * TK + r;
+        const uint32_t c = s.chunks[ch];
+        const int i = (int)(c & 0xFFFFu) - s.pad1;
+        const int j = (int)(c >> 16) - (s.kind == CH_TRI ? s.pad1 : s.pad2) + t;
+        const bool valid = (s.kind == CH_TRI) ? (j >= i && j < s.N1) : (j < s.N2);
         double v = 0.0;
-        const uint32_t pr = s.pairs[p];
-        if (k < s.No && pr != PAIR_PAD) {
-            const int i = (int)(pr & 0xFFFFu) - s.o1, j = (int)(pr >> 16) - s.o2;
+        if (k < s.No && valid) {
             v = s.scale * d_lognormal(s.key, (uint64_t)k + (uint64_t)s.No *
                                                  ((uint64_t)i + (uint64_t)s.N1 * j));
             if (s.kind == CH_TRI && i != j)
                 v += s.scale * d_lognormal(s.key, (uint64_t)k + (uint64_t)s.No *
                                                       ((uint64_t)j + (uint64_t)s.N1 * i));
         }
-        const int64_t P = s.npairs_pad;
+        const int64_t base = ((int64_t)kt * s.nchunks + ch) * KC * TK;
         if (s.precision == DTX_PREC_FP64) {
-            ((double*)s.A)[((int64_t)kt * P + p) * TK + r] = v;
+            ((double*)s.A)[base + t * TK + r] = v;
         } else if (s.precision == DTX_PREC_FP32) {
-            ((float*)s.A)[((int64_t)kt * P + p) * TK + r] = (float)v;
+            ((float*)s.A)[base + t * TK + r] = (float)v;
         } else {
             uint16_t hi, lo;
             split_bf16(v, &hi, &lo);
-            uint16_t* base = (uint16_t*)s.A + (int64_t)kt * P * TK * 2 + (p / 32) * TK * 64;
-            base[r * 32 + (p % 32)] = hi;
-            base[TK * 32 + r * 32 + (p % 32)] = lo;
+            uint16_t* b = (uint16_t*)s.A + 2 * base;
+            b[r * KC + t] = hi;
+            b[TK * KC + r * KC + t] = lo;
         }
     }
 }
@@ -84,25 +89,21 @@ __global__ void synth_loss_kernel(double* D, int N, int o1, int N1, int o2, int 
     }
 }
 
-hipError_t launch_synth_gain_packed(int precision, void* A, const uint32_t* pairs,
-                                    int64_t npairs_pad, int No, int nktiles, int kind, int N1,
-                                    int N2, uint64_t key, double scale, hipStream_t stream) {
-    // The pair table holds cell-local state indices; recover the species offsets from its
-    // first entry (pair 0 is (o1, o1) for TRI and (o1, o2) for RECT).
-    uint32_t first = 0;
-    hipError_t e = hipMemcpy(&first, pairs, 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return e;
+hipError_t launch_synth_gain_packed(int precision, void* A, const uint32_t* chunks,
+                                    int64_t nchunks, int No, int nktiles, int kind, int N1,
+                                    int N2, int pad1, int pad2, uint64_t key, double scale,
+                                    hipStream_t stream) {
     SynthArgs s;
     s.A = A;
-    s.pairs = pairs;
-    s.npairs_pad = npairs_pad;
+    s.chunks = chunks;
+    s.nchunks = nchunks;
     s.No = No;
     s.nktiles = nktiles;
     s.kind = kind;
     s.N1 = N1;
     s.N2 = N2;
-    s.o1 = (int)(first & 0xFFFFu);
-    s.o2 = (int)(first >> 16);
+    s.pad1 = pad1;
+    s.pad2 = pad2;
     s.precision = precision;
     s.key = key;
     s.scale = scale;

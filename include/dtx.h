@@ -190,7 +190,7 @@ int dtx_recent_stats(dtx_handle* h, dtx_step_stats* out, int32_t max_steps, int3
 int64_t dtx_device_bytes(const dtx_handle* h);
 /* Algorithmic work of one step: binary FLOPs (math, counted once) and HBM bytes per part. */
 typedef struct {
-    double bin_flops;      /* 2 * No * pairs * cells summed over channels (+ loss GEMV)   */
+    double bin_flops;      /* contraction kernels: 2 * No * pairs * cells + FP64 loss GEMM */
     double bin_bytes;      /* packed gain bytes + loss matrix + f reads + partial traffic */
     double update_bytes;   /* fused update kernel: f, partials, invAp, df_inj, ELL, out   */
     double gain_bytes;     /* packed gain operator bytes (streamed once per step)         */
