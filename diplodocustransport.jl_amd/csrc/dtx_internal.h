@@ -98,7 +98,7 @@ struct UpdateArgs {
     double* mom_part;       // per-block moment partials [cell][segment][S][2]
     double* block_part;     // per-block partials [nblocks][8]
     double* status;         // status ring [ring][16]
-    uint32_t* counter;      // last-block counter
+    uint32_t* counter;      // unused (kept zero)
     // symplectic intermediates
     double* f_mom;          // f after the momentum update
     double* df_mom;         // df_Momentum (scaled)
@@ -114,6 +114,7 @@ struct UpdateArgs {
     int32_t save;
 };
 
+constexpr int UPDATE_STATES = 64;  // states per update workgroup (four threads each)
 constexpr int STATUS_RING = 4096;
 constexpr int STATUS_WORDS = 16;  // doubles per status record
 // status record layout

@@ -1,6 +1,7 @@
 // kernels_update.hip -- fused per-cell update kernels for gfx950 (HBM-bound).
 //
-// One workgroup per (cell, segment of 256 states).  For every state it evaluates, in FP64 and with
+// One workgroup per (cell, segment of UPDATE_STATES = 64 states), four threads per state: the
+// four split the state's partial-slot loads, and the first of them evaluates, in FP64 and with
 // the reference's operation order (this file is built with -ffp-contract=off, so no FMA):
 //   ForwardEuler, src/SteppingMethods.jl:31-101
 //     dF  = F f                           (ELL rows, columns ascending = CSC column order)
@@ -11,8 +12,8 @@
 //   number and energy moments of the new f.
 // dE is the cell's emission block times f (ascending column order) and dB the binary term:
 // the sum of the contraction's partial slots (gain and FP64 loss channels), times Vol.
-// Cross-cell reductions go to per-block partials that the last block folds in block order, so
-// every reduction is deterministic.
+// Cross-cell reductions go to per-block partials that a one-workgroup kernel folds in block
+// order, so every reduction is deterministic.
 #include "bf16_util.h"
 #include "dtx_internal.h"
 
@@ -98,21 +99,27 @@ __device__ inline double ell_row(const int32_t* col, const double* val, int widt
 }
 
 // Binary term of state k of cell c (unscaled by invAp): the contraction's partial slots (gain
-// channels and the FP64 loss channels), in slot order, times Vol.
-__device__ inline double binary_term(const UpdateArgs& a, int c, int k) {
+// channels and the FP64 loss channels) times Vol.  The four threads q = 0..3 of the state sum
+// slots q, q+4, ...; the four sums are combined as (s0 + s1) + (s2 + s3) by a butterfly, so every
+// lane holds the same, order-fixed result.
+__device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, bool valid) {
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
-    const int ns = a.nslot[k];
-    int s = 0;
-    for (; s + 8 <= ns; s += 8) {  // eight loads in flight, summed in slot order
-        double v[8];
+    if (valid) {
+        const int ns = a.nslot[k];
+        int s = q;
+        for (; s + 12 < ns; s += 16) {  // four loads in flight per thread
+            double v[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = a.part[(int64_t)(s + q) * a.part_stride + idx];
+            for (int u = 0; u < 4; ++u) v[u] = a.part[(int64_t)(s + 4 * u) * a.part_stride + idx];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) g = g + v[q];
+            for (int u = 0; u < 4; ++u) g = g + v[u];
+        }
+        for (; s < ns; s += 4) g = g + a.part[(int64_t)s * a.part_stride + idx];
     }
-    for (; s < ns; ++s) g = g + a.part[(int64_t)s * a.part_stride + idx];
-    return g * a.vol[c];
+    g = g + __shfl_xor(g, 1, 64);
+    g = g + __shfl_xor(g, 2, 64);
+    return g * (valid ? a.vol[c] : 0.0);
 }
 
 // Copies of a new state value for the next contraction: FP64 state-major (padded states), and
@@ -136,23 +143,21 @@ __device__ inline double emission_term(const UpdateArgs& a, int c, int k, const 
     return acc;
 }
 
-// Per-species moment partials of this block's new states (fn, LDS, indexed k - k0) and the
-// per-block partials; the last block to finish folds all partials (block order) into the
-// moments and the status record of this step.
+// Per-species moment partials of this block's new states (fn, LDS, indexed by local state) and
+// the per-block partials (sums and Cr minima).  finalize_step_kernel folds them in block order.
 __device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int k1, double* red,
                              double sumdf, double sumf, double cr, double crb, double cre,
                              double crf, double crm, double crs) {
     const int c = blockIdx.x, seg = blockIdx.y, nseg = gridDim.y, tid = threadIdx.x;
     const int64_t b = (int64_t)c * nseg + seg;
-    const int64_t nblocks = (int64_t)gridDim.x * nseg;
     for (int s = 0; s < a.S; ++s) {
         const int lo = max(k0, a.species_off[s]), hi = min(k1, a.species_off[s + 1]);
         double num = 0.0, en = 0.0;
         if (lo < hi) {
-            const int k = k0 + tid;
-            if (k >= lo && k < hi) {
-                num = fn[tid];
-                if (a.eweight) en = fn[tid] * a.eweight[k];
+            const int k = k0 + (tid >> 2);
+            if ((tid & 3) == 0 && k >= lo && k < hi) {
+                num = fn[tid >> 2];
+                if (a.eweight) en = fn[tid >> 2] * a.eweight[k];
             }
             num = block_sum(num, red);
             en = block_sum(en, red);
@@ -174,7 +179,6 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int 
             crs = block_min(crs, red);
         }
     }
-    __shared__ int is_last;
     if (tid == 0) {
         double* bp = a.block_part + b * BP_WORDS;
         bp[BP_SUMDF] = sumdf;
@@ -185,19 +189,20 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int 
         bp[BP_CR_FLUX] = crf;
         bp[BP_CR_MOM] = crm;
         bp[BP_CR_SPACE] = crs;
-        __threadfence();
-        const unsigned prev = atomicAdd(a.counter, 1u);
-        is_last = (prev == (unsigned)nblocks - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    // moments of every (cell, species): segments summed in order
+}
+
+// One workgroup: folds the per-block partials of the step (block order) into the moments of
+// every (cell, species) and the status record of the step.
+__global__ __launch_bounds__(UB) void finalize_step_kernel(UpdateArgs a, int nseg) {
+    __shared__ double red[UB / 64];
+    const int tid = threadIdx.x;
+    const int64_t nblocks = (int64_t)a.C * nseg;
     for (int64_t q = tid; q < (int64_t)a.C * a.S; q += UB) {
         const int64_t cc = q / a.S, sp = q % a.S;
         double num = 0.0, en = 0.0;
         for (int g = 0; g < nseg; ++g) {
-            const volatile double* mp = a.mom_part + (((cc * nseg + g) * a.S) + sp) * 2;
+            const double* mp = a.mom_part + (((cc * nseg + g) * a.S) + sp) * 2;
             num = num + mp[0];
             en = en + mp[1];
         }
@@ -207,7 +212,7 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int 
     double v[BP_WORDS];
     for (int q = 0; q < BP_WORDS; ++q) v[q] = (q <= BP_SUMF) ? 0.0 : INFINITY;
     for (int64_t bb = tid; bb < nblocks; bb += UB) {
-        const volatile double* bp = a.block_part + bb * BP_WORDS;
+        const double* bp = a.block_part + bb * BP_WORDS;
         v[BP_SUMDF] = v[BP_SUMDF] + bp[BP_SUMDF];
         v[BP_SUMF] = v[BP_SUMF] + bp[BP_SUMF];
         for (int q = BP_CR; q < BP_WORDS; ++q) v[q] = cr_min(v[q], bp[q]);
@@ -232,7 +237,6 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int 
         st[ST_TSTART] = a.t_start;
         st[ST_SAVE] = (double)a.save;
         st[ST_SUMDF] = v[BP_SUMDF];
-        *a.counter = 0u;  // ready for the next launch
     }
 }
 
@@ -252,12 +256,12 @@ __device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
 // ----------------------------------------------------------------------- ForwardEuler ------
 __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
-    double* fn = smem;            // new f of this block's states
-    double* fc = smem + UB;       // f of the whole cell (emission / Domain only)
-    __shared__ double red[UB];
-    __shared__ int ired[UB];
+    double* fn = smem;                    // new f of this block's states
+    double* fc = smem + UPDATE_STATES;    // f of the whole cell (emission / Domain only)
+    __shared__ double red[UB / 64];
+    __shared__ int ired[UB / 64];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
-    const int k0 = blockIdx.y * UB, k1 = min(N, k0 + UB);
+    const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
     const bool need_cell = a.emission || (a.binary && a.active);
     const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
@@ -268,23 +272,22 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
     double sumdf = 0.0, sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY,
            crf = INFINITY;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
-    const int k = k0 + tid;
-    if (k < k1) {
+    const int k = k0 + (tid >> 2), q = tid & 3;
+    const bool mine = k < k1;
+    const double dB = a.binary ? binary_term(a, c, k, q, mine && active) : 0.0;
+    if (mine && q == 0) {
         const int64_t r = base + k;
         const double fk = a.f[r];
         sumf = fk;
         const double dF = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
                                       : 0.0;
         double df = -dF;
-        double dE = 0.0, dB = 0.0;
+        double dE = 0.0;
         if (a.emission) {
             dE = emission_term(a, c, k, fc);
             df = df + dE;
         }
-        if (a.binary) {
-            dB = active ? binary_term(a, c, k) : 0.0;
-            df = df + dB;
-        }
+        if (a.binary) df = df + dB;
         df = df * (a.inv_ap[r] * a.s);
         sumdf = df;
         if (want_cr) {
@@ -299,7 +302,7 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
         x = strong_zero(x, x >= THR_1F_10);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
-        fn[tid] = x;
+        fn[tid >> 2] = x;
         publish(a, c, k, x);
     }
     __syncthreads();
@@ -311,16 +314,18 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
 // f_Momentum = f + df_Momentum clamped at 1f-10.
 __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
-    double* fc = smem + UB;
-    __shared__ int ired[UB];
+    double* fc = smem + UPDATE_STATES;
+    __shared__ int ired[UB / 64];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
-    const int k0 = blockIdx.y * UB, k1 = min(N, k0 + UB);
+    const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
     const bool need_cell = a.emission || (a.binary && a.active);
     const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
-    const int k = k0 + tid;
-    if (k < k1) {
+    const int k = k0 + (tid >> 2), q = tid & 3;
+    const bool mine = k < k1;
+    const double dB = a.binary ? binary_term(a, c, k, q, mine && active) : 0.0;
+    if (mine && q == 0) {
         const int64_t r = base + k;
         const double dP = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
                                       : 0.0;
@@ -331,7 +336,6 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
             a.demi[r] = dE;
         }
         if (a.binary) {
-            const double dB = active ? binary_term(a, c, k) : 0.0;
             dm = dm + dB;
             a.dbin[r] = dB;
         }
@@ -347,15 +351,15 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
 __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
     double* fn = smem;
-    __shared__ double red[UB];
+    __shared__ double red[UB / 64];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
-    const int k0 = blockIdx.y * UB, k1 = min(N, k0 + UB);
+    const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
     double sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY, crf = INFINITY;
     double crm = INFINITY, crs = INFINITY;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
-    const int k = k0 + tid;
-    if (k < k1) {
+    const int k = k0 + (tid >> 2);
+    if (k < k1 && (tid & 3) == 0) {
         const int64_t r = base + k;
         const double f0 = a.f[r];
         sumf = f0;
@@ -383,7 +387,7 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
         x = strong_zero(x, x >= THR_1F_20) * jl_sign(x);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
-        fn[tid] = x;
+        fn[tid >> 2] = x;
         publish(a, c, k, x);
     }
     __syncthreads();
@@ -424,14 +428,22 @@ __global__ void fill_zero_kernel(double* p, int64_t count) {
 }
 
 static size_t update_smem(const UpdateArgs& a) {
-    return (size_t)(UB + a.N) * sizeof(double);
+    return (size_t)(UPDATE_STATES + a.N) * sizeof(double);
 }
 
-static dim3 update_grid(const UpdateArgs& a) { return dim3(a.C, (a.N + UB - 1) / UB); }
+static dim3 update_grid(const UpdateArgs& a) {
+    return dim3(a.C, (a.N + UPDATE_STATES - 1) / UPDATE_STATES);
+}
+
+static hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream) {
+    finalize_step_kernel<<<1, UB, 0, stream>>>(a, (int)update_grid(a).y);
+    return hipGetLastError();
+}
 
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
     euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize(a, stream);
 }
 
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream) {
@@ -441,7 +453,8 @@ hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream) {
 
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream) {
     symplectic_space_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize(a, stream);
 }
 
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream) {

@@ -838,7 +838,7 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_state_species, N);
     rc |= dalloc(hp, &h->d_species_off, h->S + 1);
     rc |= dalloc(hp, &h->d_moments, C * h->S * 2);
-    const int64_t nseg = (N + 255) / 256;  // update kernels: one block per 256 states
+    const int64_t nseg = (N + UPDATE_STATES - 1) / UPDATE_STATES;  // update workgroups per cell
     rc |= dalloc(hp, &h->d_block_part, C * nseg * BP_WORDS);
     rc |= dalloc(hp, &h->d_mom_part, C * nseg * h->S * 2);
     rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
