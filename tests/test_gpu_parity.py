@@ -134,3 +134,16 @@ def test_medium_transport_parity_fp64():
     m, out, fo, to, _ = _solve_both(w, n_save=5)
     for i in range(len(out.f)):
         assert rel_linf(out.f[i], fo[i]) < TOL64, i
+
+
+@pytest.mark.parametrize("path", __import__("fixture_util").fixtures(), ids=lambda p: p.stem)
+def test_golden_fixtures_fp64(path):
+    """The HIP path against the committed golden outputs (oracle + brute force agreed on them)."""
+    from fixture_util import load
+    w, z = load(path)
+    method = "euler" if int(z["method"][0]) == 0 else "symplectic"
+    m = workloads.make_struct(w, precision="fp64", method=method)
+    out = Solve(m, w.dt, z["t_save"], Verbose=1)
+    np.testing.assert_array_equal(out.t, z["t_out"])
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], z["f_out"][i]) < TOL64, (path.stem, i)
