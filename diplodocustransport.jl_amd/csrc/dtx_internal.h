@@ -134,6 +134,8 @@ hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, in
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream);
+hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* block_start,
+                                    int count, int total_blocks, int max_N, hipStream_t stream);
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream);
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
                                 int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
@@ -149,5 +151,14 @@ hipError_t launch_synth_loss(double* loss, int N, int off1, int N1, int off2, in
                              hipStream_t stream);
 
 uint64_t splitmix_key(uint64_t seed, uint64_t stream);
+
+// grouped stepping of several handles (runtime.hip)
+struct Group;
+Group* group_create(int device);
+void group_destroy(Group* g);
+const char* group_error(const Group* g);
+int group_set(Group* g, dtx_handle* const* hs, int count);
+int group_step(Group* g, const double* dt, int verbose);
+int group_sync(Group* g);
 
 }  // namespace dtx

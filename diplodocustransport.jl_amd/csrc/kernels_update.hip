@@ -145,10 +145,10 @@ __device__ inline double emission_term(const UpdateArgs& a, int c, int k, const 
 
 // Per-species moment partials of this block's new states (fn, LDS, indexed by local state) and
 // the per-block partials (sums and Cr minima).  finalize_step_kernel folds them in block order.
-__device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int k1, double* red,
-                             double sumdf, double sumf, double cr, double crb, double cre,
-                             double crf, double crm, double crs) {
-    const int c = blockIdx.x, seg = blockIdx.y, nseg = gridDim.y, tid = threadIdx.x;
+__device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, const double* fn,
+                             int k0, int k1, double* red, double sumdf, double sumf, double cr,
+                             double crb, double cre, double crf, double crm, double crs) {
+    const int tid = threadIdx.x;
     const int64_t b = (int64_t)c * nseg + seg;
     for (int s = 0; s < a.S; ++s) {
         const int lo = max(k0, a.species_off[s]), hi = min(k1, a.species_off[s + 1]);
@@ -194,8 +194,7 @@ __device__ void finish_block(const UpdateArgs& a, const double* fn, int k0, int 
 
 // One workgroup: folds the per-block partials of the step (block order) into the moments of
 // every (cell, species) and the status record of the step.
-__global__ __launch_bounds__(UB) void finalize_step_kernel(UpdateArgs a, int nseg) {
-    __shared__ double red[UB / 64];
+__device__ void finalize_step_body(const UpdateArgs& a, int nseg, double* red) {
     const int tid = threadIdx.x;
     const int64_t nblocks = (int64_t)a.C * nseg;
     for (int64_t q = tid; q < (int64_t)a.C * a.S; q += UB) {
@@ -254,14 +253,12 @@ __device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
 }
 
 // ----------------------------------------------------------------------- ForwardEuler ------
-__global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
-    extern __shared__ double smem[];
+__device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg, double* smem,
+                                  double* red, int* ired) {
     double* fn = smem;                    // new f of this block's states
     double* fc = smem + UPDATE_STATES;    // f of the whole cell (emission / Domain only)
-    __shared__ double red[UB / 64];
-    __shared__ int ired[UB / 64];
-    const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
-    const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
+    const int tid = threadIdx.x, N = a.N;
+    const int k0 = seg * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
     const bool need_cell = a.emission || (a.binary && a.active);
     const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
@@ -306,7 +303,48 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
         publish(a, c, k, x);
     }
     __syncthreads();
-    finish_block(a, fn, k0, k1, red, sumdf, sumf, cr, crb, cre, crf, INFINITY, INFINITY);
+    finish_block(a, c, seg, nseg, fn, k0, k1, red, sumdf, sumf, cr, crb, cre, crf, INFINITY,
+                 INFINITY);
+}
+
+__global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
+    extern __shared__ double smem[];
+    __shared__ double red[UB / 64];
+    __shared__ int ired[UB / 64];
+    euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
+}
+
+__global__ __launch_bounds__(UB) void finalize_step_kernel(UpdateArgs a, int nseg) {
+    __shared__ double red[UB / 64];
+    finalize_step_body(a, nseg, red);
+}
+
+// Grouped forms for several simulations in one launch (serving): block b of the update grid
+// belongs to the simulation g with block_start[g] <= b < block_start[g+1]; the finalize grid has
+// one block per simulation.
+__global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs* table,
+                                                                const int32_t* block_start,
+                                                                int count) {
+    extern __shared__ double smem[];
+    __shared__ double red[UB / 64];
+    __shared__ int ired[UB / 64];
+    const int b = blockIdx.x;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {  // last g with block_start[g] <= b
+        const int mid = (lo + hi + 1) >> 1;
+        if (block_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const UpdateArgs a = table[lo];
+    const int nseg = (a.N + UPDATE_STATES - 1) / UPDATE_STATES;
+    const int local = b - block_start[lo];
+    euler_update_body(a, local / nseg, local % nseg, nseg, smem, red, ired);
+}
+
+__global__ __launch_bounds__(UB) void finalize_group_kernel(const UpdateArgs* table) {
+    __shared__ double red[UB / 64];
+    const UpdateArgs a = table[blockIdx.x];
+    finalize_step_body(a, (a.N + UPDATE_STATES - 1) / UPDATE_STATES, red);
 }
 
 // ------------------------------------------------------------- ForwardSymplecticEuler ------
@@ -391,7 +429,8 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
         publish(a, c, k, x);
     }
     __syncthreads();
-    finish_block(a, fn, k0, k1, red, 0.0, sumf, cr, crb, cre, crf, crm, crs);
+    finish_block(a, c, blockIdx.y, gridDim.y, fn, k0, k1, red, 0.0, sumf, cr, crb, cre, crf, crm,
+                 crs);
 }
 
 __global__ void publish_state_kernel(UpdateArgs a) {
@@ -455,6 +494,17 @@ hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream) {
     symplectic_space_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize(a, stream);
+}
+
+hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* block_start,
+                                    int count, int total_blocks, int max_N, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    euler_update_group_kernel<<<total_blocks, UB, (size_t)(UPDATE_STATES + max_N) * 8, stream>>>(
+        table, block_start, count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    finalize_group_kernel<<<count, UB, 0, stream>>>(table);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream) {

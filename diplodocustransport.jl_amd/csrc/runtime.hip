@@ -74,6 +74,7 @@ struct dtx_handle {
     bool loss_present = false;
     BinWork* d_items = nullptr;
     BinWork* d_items_loss = nullptr;
+    std::vector<BinWork> items_host, items_loss_host;  // host copies (grouped launches)
     int32_t nitems = 0, nitems_loss = 0, tc = 16, ldc = 0, max_slots = 0;
     double* d_part = nullptr;
     int64_t part_stride = 0;
@@ -453,6 +454,8 @@ static int build_work(dtx_handle* h) {
     }
     h->nitems = (int32_t)items.size();
     h->nitems_loss = (int32_t)items_loss.size();
+    h->items_host = items;
+    h->items_loss_host = items_loss;
     if (h->nitems) {
         int rc = dalloc(h, &h->d_items, h->nitems);
         if (!rc) rc = upload(h, h->d_items, items.data(), h->nitems);
@@ -749,6 +752,203 @@ static int set_ell(dtx_handle* h, EllMatrix& E, int64_t n, const int64_t* colptr
     if (rc) return rc;
     return upload(h, E.val, val.data(), width * n);
 }
+
+// ------------------------------------------------------------------- grouped stepping -------
+// Several simulations advanced together: one contraction launch per (kernel precision, cell-tile
+// width) over the members' concatenated work tables, one update launch over a table of the
+// members' update arguments.  Each member's work items and reduction order are the ones it uses
+// alone, so its results do not depend on the group.
+
+namespace dtx {
+
+constexpr int GROUP_RING = 8;
+
+struct Group {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<dtx_handle*> members;
+    BinWork* d_items[4] = {nullptr, nullptr, nullptr, nullptr};  // gain16 gain64 loss16 loss64
+    int n_items[4] = {0, 0, 0, 0};
+    int precision = 0;
+    int32_t* d_block_start = nullptr;
+    int total_blocks = 0, max_N = 0;
+    UpdateArgs* d_table = nullptr;
+    int table_cap = 0;
+    UpdateArgs* h_stage[GROUP_RING] = {};
+    hipEvent_t ev[GROUP_RING] = {};
+    bool ev_used[GROUP_RING] = {};
+    int ring_pos = 0;
+    std::string err;
+};
+
+static void group_free_tables(Group* g) {
+    for (int q = 0; q < 4; ++q) {
+        if (g->d_items[q]) (void)hipFree(g->d_items[q]);
+        g->d_items[q] = nullptr;
+        g->n_items[q] = 0;
+    }
+    if (g->d_block_start) (void)hipFree(g->d_block_start);
+    g->d_block_start = nullptr;
+}
+
+Group* group_create(int device) {
+    Group* g = new Group();
+    g->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return nullptr;
+    }
+    for (int q = 0; q < GROUP_RING; ++q) (void)hipEventCreateWithFlags(&g->ev[q], hipEventDisableTiming);
+    return g;
+}
+
+void group_destroy(Group* g) {
+    if (!g) return;
+    (void)hipStreamSynchronize(g->stream);
+    group_free_tables(g);
+    if (g->d_table) (void)hipFree(g->d_table);
+    for (int q = 0; q < GROUP_RING; ++q) {
+        if (g->h_stage[q]) (void)hipHostFree(g->h_stage[q]);
+        (void)hipEventDestroy(g->ev[q]);
+    }
+    (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+const char* group_error(const Group* g) { return g->err.c_str(); }
+
+int group_sync(Group* g) {
+    hipError_t e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) {
+        g->err = std::string("group stream: ") + hipGetErrorString(e);
+        return DTX_ERR_HIP;
+    }
+    return DTX_OK;
+}
+
+// (Re)build the combined work tables for a member set.  Members must be idle.
+int group_set(Group* g, dtx_handle* const* hs, int count) {
+    int rc = group_sync(g);
+    if (rc) return rc;
+    group_free_tables(g);
+    g->members.assign(hs, hs + count);
+    std::vector<BinWork> lists[4];
+    std::vector<int32_t> starts;
+    g->total_blocks = 0;
+    g->max_N = 0;
+    for (int m = 0; m < count; ++m) {
+        dtx_handle* h = hs[m];
+        if (h->method != DTX_METHOD_FORWARD_EULER) {
+            g->err = "grouped stepping supports ForwardEuler members only";
+            return DTX_ERR_UNSUPPORTED;
+        }
+        if (m == 0) g->precision = h->precision;
+        if (h->precision != g->precision) {
+            g->err = "grouped members must share one contraction precision";
+            return DTX_ERR_UNSUPPORTED;
+        }
+        rc = ensure_ready(h);
+        if (rc) {
+            g->err = h->err;
+            return rc;
+        }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+            g->err = "member stream failed";
+            return DTX_ERR_HIP;
+        }
+        const int wide = h->tc == 64 ? 1 : 0;
+        if (h->binary) {
+            lists[wide].insert(lists[wide].end(), h->items_host.begin(), h->items_host.end());
+            lists[2 + wide].insert(lists[2 + wide].end(), h->items_loss_host.begin(),
+                                   h->items_loss_host.end());
+        }
+        starts.push_back(g->total_blocks);
+        g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
+        g->max_N = std::max(g->max_N, h->N);
+    }
+    for (int q = 0; q < 4; ++q) {
+        g->n_items[q] = (int)lists[q].size();
+        if (!g->n_items[q]) continue;
+        if (hipMalloc((void**)&g->d_items[q], lists[q].size() * sizeof(BinWork)) != hipSuccess ||
+            hipMemcpy(g->d_items[q], lists[q].data(), lists[q].size() * sizeof(BinWork),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            g->err = "work table upload failed";
+            return DTX_ERR_OOM;
+        }
+    }
+    if (count > 0) {
+        if (hipMalloc((void**)&g->d_block_start, starts.size() * 4) != hipSuccess ||
+            hipMemcpy(g->d_block_start, starts.data(), starts.size() * 4,
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            g->err = "block table upload failed";
+            return DTX_ERR_OOM;
+        }
+    }
+    if (count > g->table_cap) {
+        if (g->d_table) (void)hipFree(g->d_table);
+        for (int q = 0; q < GROUP_RING; ++q) {
+            if (g->h_stage[q]) (void)hipHostFree(g->h_stage[q]);
+            g->h_stage[q] = nullptr;
+            g->ev_used[q] = false;
+        }
+        g->table_cap = std::max(count, 2 * g->table_cap);
+        if (hipMalloc((void**)&g->d_table, (size_t)g->table_cap * sizeof(UpdateArgs)) !=
+            hipSuccess) {
+            g->err = "update table allocation failed";
+            return DTX_ERR_OOM;
+        }
+        for (int q = 0; q < GROUP_RING; ++q)
+            if (hipHostMalloc((void**)&g->h_stage[q], (size_t)g->table_cap * sizeof(UpdateArgs),
+                              hipHostMallocDefault) != hipSuccess) {
+                g->err = "pinned staging allocation failed";
+                return DTX_ERR_OOM;
+            }
+    }
+    return DTX_OK;
+}
+
+// Enqueue one ForwardEuler step (dt[m], no save clipping) of every member.
+int group_step(Group* g, const double* dt, int verbose) {
+    const int count = (int)g->members.size();
+    if (count == 0) return DTX_OK;
+    const int slot = g->ring_pos++ % GROUP_RING;
+    if (g->ev_used[slot] && hipEventSynchronize(g->ev[slot]) != hipSuccess) {
+        g->err = "staging event failed";
+        return DTX_ERR_HIP;
+    }
+    for (int m = 0; m < count; ++m) {
+        dtx_handle* h = g->members[m];
+        g->h_stage[slot][m] = make_args(h, dt[m] / h->cfg.dt0, verbose, 0.0, dt[m], 0, DTX_FLUX_F);
+    }
+    hipError_t e = hipMemcpyAsync(g->d_table, g->h_stage[slot], (size_t)count * sizeof(UpdateArgs),
+                                  hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) e = hipEventRecord(g->ev[slot], g->stream);
+    g->ev_used[slot] = true;
+    const int kprec[4] = {g->precision, g->precision, DTX_PREC_FP64, DTX_PREC_FP64};
+    const int ktc[4] = {16, 64, 16, 64};
+    for (int q = 0; q < 4 && e == hipSuccess; ++q)
+        if (!(g->precision == DTX_PREC_FP64 && q >= 2))
+            e = launch_binary(kprec[q], ktc[q], g->d_items[q], g->n_items[q], g->stream);
+    if (e == hipSuccess)
+        e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
+                                      g->max_N, g->stream);
+    if (e != hipSuccess) {
+        g->err = std::string("group step: ") + hipGetErrorString(e);
+        return DTX_ERR_HIP;
+    }
+    for (int m = 0; m < count; ++m) {
+        dtx_handle* h = g->members[m];
+        if (h->step_verbose.size() != (size_t)STATUS_RING) h->step_verbose.assign(STATUS_RING, 0);
+        h->step_verbose[(size_t)(h->step_counter % STATUS_RING)] = (int8_t)verbose;
+        h->cur ^= 1;
+        h->step_counter++;
+        h->reported = h->step_counter;  // grouped steps are not echoed
+    }
+    return DTX_OK;
+}
+
+}  // namespace dtx
 
 // ------------------------------------------------------------------------ C ABI ----------
 
