@@ -96,6 +96,36 @@ def run_cpu_baseline(w, sample_steps):
                        f"{O.num_threads()} threads of {cpu_model()}")
 
 
+def run_serving(count, serial_sample):
+    """Serving workload: `count` seeded requests (p in {16,32,64}, u in {4,8}, x in {1,32,128},
+    1-2 species, 200-2000 steps).  Capacity = solves/s with every request queued at t = 0; then
+    Poisson arrivals at 80% of that capacity give the time-to-solution percentiles.  The serial
+    plain-FP64 rate runs the first `serial_sample` requests one at a time."""
+    from dtx import serving
+    specs = serving.request_mix(count=count, rate_per_s=1e9)
+    log(f"[bench] serving capacity: {count} requests queued at t=0 (FP32 grouped kernels)")
+    res, wall, _ = serving.run(specs, precision="fp32", realtime=False)
+    capacity = count / wall
+    rate = 0.8 * capacity
+    specs = serving.request_mix(count=count, rate_per_s=rate)
+    log(f"[bench] serving: Poisson arrivals at {rate:.2f}/s")
+    res, wall, _ = serving.run(specs, precision="fp32", realtime=True)
+    lat = serving.latency_stats(res)
+    serial = None
+    if serial_sample:
+        log(f"[bench] serial plain FP64: first {serial_sample} requests one at a time")
+        t0 = time.perf_counter()
+        for sp in specs[:serial_sample]:
+            serving.run([sp], precision="fp64", realtime=False)
+        serial = serial_sample / (time.perf_counter() - t0)
+    return dict(requests=count, precision="fp32", hbm_budget_gb=200,
+                capacity_solves_per_s=round(capacity, 3), arrival_rate_per_s=round(rate, 3),
+                solves_per_s=round(count / wall, 3), p50_time_to_solution_s=round(lat["p50_s"], 3),
+                p99_time_to_solution_s=round(lat["p99_s"], 3),
+                serial_fp64_solves_per_s=round(serial, 3) if serial else None,
+                serial_fp64_sample=serial_sample)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +134,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=256)
+    ap.add_argument("--serving-serial-sample", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -143,6 +176,9 @@ def main():
                 unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["fp64"], 4), traffic=None,
                 kernel="bin_f64_kernel<64>",
                 flops_per_launch=base["bin_flops"], kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
+    serving_line = None
+    if not args.no_serving:
+        serving_line = run_serving(args.serving_requests, args.serving_serial_sample)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle steps")
@@ -169,6 +205,7 @@ def main():
         "precision_modes": prec_report,
         "update_kernel_us": round(base["update_ms"] * 1e3, 2),
         "roofline": roof,
+        "serving": serving_line,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
