@@ -137,7 +137,20 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=256)
     ap.add_argument("--serving-serial-sample", type=int, default=16)
+    ap.add_argument("--copy-kernel-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (5 times), the "
+                         "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_kernel_mib:
+        import torch
+        n = args.copy_kernel_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the path shards by cells across GPUs only by design, see the design notes)",

@@ -1,0 +1,258 @@
+# DiplodocusTransportHIP.jl -- HIP (AMD MI355X) backend for DiplodocusTransport.jl's explicit steps.
+#
+# `include` this file inside `module DiplodocusTransport`, after src/Backends.jl and
+# src/StructsAndDictionaries.jl.  It adds:
+#   * HIPBackend <: BackendType                      (beside CPUBackend/CUDABackend, src/Backends.jl:3-4)
+#   * DtxHandle, the owner of one libdtx.so simulation (device state and operators)
+#   * dtx_attach!(method, PhaseSpace, Initial, Injection, BinM, EmiM, FluxM, backend)
+#       the constructor branch `elseif Backend isa HIPBackend` of ForwardEulerStruct /
+#       ForwardSymplecticEulerStruct (src/SteppingStructs.jl:57-75, 175-193)
+#   * dtx_step!(method, t_start, t_stop, dt, Verbose) -> (dt, save)
+#       the whole functor body (src/SteppingMethods.jl:12-107, 120-243) as one ccall
+#   * DtxState <: AbstractVector{Float64}, the value of `method.f` on this backend, so that
+#       Solve's `@. method.f = method.f_init` and `copyto!(f, method.f)` (src/Solver.jl:22,40)
+#       work unchanged.
+#
+# Every numerical operation runs in libdtx.so (C ABI: include/dtx.h).  Arrays are passed in the
+# reference's own layouts: M_Bin (N^2 x N, column-major), SparseMatrixCSC{Float64,Int64}
+# (1-based), state vectors in GlobalIndices_To_StateIndex order.
+
+const libdtx = get(ENV, "DTX_LIBRARY", normpath(joinpath(@__DIR__, "..", "libdtx.so")))
+
+const DTX_MAX_SPECIES = 16
+const DTX_PRECISIONS = Dict(:FP64 => Int32(0), :FP32 => Int32(1), :BF16x3 => Int32(2))
+
+"""
+    HIPBackend(; device=0, precision=:FP64)
+
+MI355X backend.  `precision` selects the arithmetic of the binary gain contraction (`:FP64`,
+`:FP32` MFMA, or `:BF16x3` split-bf16 MFMA); f, the Euler update, losses and moments stay Float64.
+"""
+struct HIPBackend <: BackendType
+    device::Int32
+    precision::Int32
+end
+HIPBackend(; device::Integer=0, precision::Symbol=:FP64) =
+    HIPBackend(Int32(device), get(DTX_PRECISIONS, precision) do
+        error("HIPBackend precision must be :FP64, :FP32 or :BF16x3")
+    end)
+
+function Base.zeros(::HIPBackend, T::Type{<:AbstractFloat}, dims::Int...)
+    return zeros(T, dims...)   # host scratch only: device buffers belong to the DtxHandle
+end
+
+# Mirrors of the C structs (same field order and types => same layout).
+struct DtxConfig
+    num_species::Int32
+    px_num::NTuple{DTX_MAX_SPECIES,Int32}
+    py_num::NTuple{DTX_MAX_SPECIES,Int32}
+    pz_num::NTuple{DTX_MAX_SPECIES,Int32}
+    x_num::Int32
+    y_num::Int32
+    z_num::Int32
+    precision::Int32
+    method::Int32
+    device::Int32
+    dt0::Float64
+end
+
+struct DtxStepStats
+    finite::Int32
+    cr_evaluated::Int32
+    save::Int32
+    step_index::Int32
+    cr::Float64
+    cr_bin::Float64
+    cr_emi::Float64
+    cr_flux::Float64
+    cr_momentum::Float64
+    cr_space::Float64
+    dt::Float64
+    t_start::Float64
+    sum_f::Float64
+end
+
+mutable struct DtxHandle
+    ptr::Ptr{Cvoid}
+    n::Int
+    num_species::Int
+    n_space::Int
+end
+
+function dtx_error(ptr::Ptr{Cvoid})
+    msg = ccall((:dtx_last_error, libdtx), Cstring, (Ptr{Cvoid},), ptr)
+    return unsafe_string(msg)
+end
+
+dtx_check(h::DtxHandle, rc::Cint) = rc == 0 || error("libdtx: " * dtx_error(h.ptr))
+
+function dtx_close!(h::DtxHandle)
+    if h.ptr != C_NULL
+        ccall((:dtx_destroy, libdtx), Cvoid, (Ptr{Cvoid},), h.ptr)
+        h.ptr = C_NULL
+    end
+    return nothing
+end
+
+"""
+    DtxState(handle)
+
+`method.f` on the HIP backend: the state lives in HBM.  Reads copy the whole state to the host
+(dtx_get_f); `copyto!(state, v)` and broadcast assignment upload it (dtx_set_f).
+"""
+struct DtxState <: AbstractVector{Float64}
+    h::DtxHandle
+end
+Base.size(s::DtxState) = (s.h.n,)
+Base.IndexStyle(::Type{DtxState}) = IndexLinear()
+function Base.copyto!(dest::AbstractVector{Float64}, src::DtxState)
+    length(dest) == src.h.n || throw(DimensionMismatch("state length"))
+    buf = dest isa Vector{Float64} ? dest : Vector{Float64}(undef, src.h.n)
+    dtx_check(src.h, ccall((:dtx_get_f, libdtx), Cint, (Ptr{Cvoid}, Ptr{Float64}), src.h.ptr, buf))
+    buf === dest || copyto!(dest, buf)
+    return dest
+end
+Base.getindex(s::DtxState, i::Int) = copyto!(Vector{Float64}(undef, s.h.n), s)[i]
+Base.collect(s::DtxState) = copyto!(Vector{Float64}(undef, s.h.n), s)
+function Base.copyto!(dest::DtxState, src::AbstractVector)
+    v = convert(Vector{Float64}, src)
+    length(v) == dest.h.n || throw(DimensionMismatch("state length"))
+    dtx_check(dest.h, ccall((:dtx_set_f, libdtx), Cint, (Ptr{Cvoid}, Ptr{Float64}), dest.h.ptr, v))
+    return dest
+end
+# `@. method.f = method.f_init` (src/Solver.jl:22) lowers to materialize!(dest, bc)
+Base.copyto!(dest::DtxState, bc::Base.Broadcast.Broadcasted) =
+    copyto!(dest, Base.materialize(bc))
+
+"""
+    dtx_energy_weights(PhaseSpace) -> Vector{Float64}
+
+Per-state T^00 weight of one cell, written as StressEnergyTensor writes it
+(src/DistributionMoments.jl:145 massive, :162 massless) and expanded over (py, pz) in the state
+order of src/GlobalToStateIndices.jl:7-25.
+"""
+function dtx_energy_weights(PhaseSpace)
+    Momentum = PhaseSpace.Momentum
+    Grids = PhaseSpace.Grids
+    w = Float64[]
+    for s in eachindex(PhaseSpace.name_list)
+        pr = Grids.pxr_list[s]
+        m = Grids.mass_list[s]
+        px = Momentum.px_num_list[s]
+        wp = Vector{Float64}(undef, px)
+        for p in 1:px
+            wp[p] = m != 0e0 ?
+                (pr[p]sqrt(m^2 + pr[p]^2) - pr[p+1]sqrt(m^2 + pr[p+1]^2) +
+                 m^2*(asinh(pr[p]/m) - asinh(pr[p+1]/m))) / (2(pr[p] - pr[p+1])) :
+                (pr[p] + pr[p+1]) / 2
+        end
+        append!(w, repeat(wp, Momentum.py_num_list[s] * Momentum.pz_num_list[s]))
+    end
+    return w
+end
+
+"""
+    dtx_attach!(method, PhaseSpace, Initial, Injection, BinM, EmiM, FluxM, backend; symplectic)
+
+Creates the libdtx simulation for `method` and uploads its operators.  Called from the
+`elseif Backend isa HIPBackend` branch of the stepping-struct constructors; afterwards
+`method.f` is a DtxState and `method.HIP` holds the handle.
+"""
+function dtx_attach!(method, PhaseSpace, Initial::Vector{Float64}, Injection::Vector{Float64},
+                     BinM, EmiM, FluxM, backend::HIPBackend; symplectic::Bool=false)
+    Momentum = PhaseSpace.Momentum
+    Space = PhaseSpace.Space
+    S = length(PhaseSpace.name_list)
+    S <= DTX_MAX_SPECIES || error("HIPBackend supports at most $DTX_MAX_SPECIES species")
+    pad(v) = ntuple(i -> i <= S ? Int32(v[i]) : Int32(0), DTX_MAX_SPECIES)
+    cfg = DtxConfig(Int32(S), pad(Momentum.px_num_list), pad(Momentum.py_num_list),
+                    pad(Momentum.pz_num_list), Int32(Space.x_num), Int32(Space.y_num),
+                    Int32(Space.z_num), backend.precision, Int32(symplectic ? 1 : 0),
+                    backend.device, Float64(PhaseSpace.Time.dt0))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:dtx_create, libdtx), Cint, (Ref{DtxConfig}, Ref{Ptr{Cvoid}}), cfg, out)
+    rc == 0 || error("libdtx: " * dtx_error(C_NULL))
+    n = length(Initial)
+    h = DtxHandle(out[], n, S, Space.x_num * Space.y_num * Space.z_num)
+    finalizer(dtx_close!, h)
+
+    N = sum(Momentum.px_num_list .* Momentum.py_num_list .* Momentum.pz_num_list)
+    set_csc(which, A) = begin
+        F = SparseMatrixCSC{Float64,Int64}(A)
+        dtx_check(h, ccall((:dtx_set_flux_csc, libdtx), Cint,
+                           (Ptr{Cvoid}, Int32, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                           h.ptr, Int32(which), Int64(n), F.colptr, F.rowval, F.nzval))
+    end
+    if symplectic
+        set_csc(1, FluxM.X_Flux)     # DTX_FLUX_X
+        set_csc(2, FluxM.P_Flux)     # DTX_FLUX_P
+    else
+        set_csc(0, FluxM.X_Flux + FluxM.P_Flux)   # F_Flux (src/SteppingStructs.jl:60)
+    end
+    if !isempty(BinM.Binary_list)
+        M = Matrix{Float64}(BinM.M_Bin)
+        size(M) == (N^2, N) || error("M_Bin must be (N^2 x N) with N = $N")
+        dtx_check(h, ccall((:dtx_set_binary_dense, libdtx), Cint,
+                           (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(N), M))
+        dom = BinM.Domain
+        if dom === nothing
+            dtx_check(h, ccall((:dtx_set_binary_domain, libdtx), Cint,
+                               (Ptr{Cvoid}, Int64, Ptr{Int64}), h.ptr, Int64(-1), C_NULL))
+        else
+            d = Vector{Int64}(dom)
+            dtx_check(h, ccall((:dtx_set_binary_domain, libdtx), Cint,
+                               (Ptr{Cvoid}, Int64, Ptr{Int64}), h.ptr, Int64(length(d)), d))
+        end
+    end
+    if !isempty(EmiM.Emission_list)
+        E = Matrix{Float64}(EmiM.M_Emi)
+        dtx_check(h, ccall((:dtx_set_emission_dense, libdtx), Cint,
+                           (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(n), E))
+    end
+    dtx_check(h, ccall((:dtx_set_vectors, libdtx), Cint,
+                       (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                       h.ptr, Vector{Float64}(FluxM.Ap_Flux), Vector{Float64}(FluxM.Vol),
+                       Initial, Injection))
+    dtx_check(h, ccall((:dtx_set_moment_weights, libdtx), Cint, (Ptr{Cvoid}, Ptr{Float64}),
+                       h.ptr, dtx_energy_weights(PhaseSpace)))
+    method.HIP = h
+    method.f = DtxState(h)
+    return method
+end
+
+"""
+    dtx_step!(method, t_start, t_stop, dt, Verbose) -> (dt, save)
+
+The functor on the HIP backend: the dt clip, dt_scale and the whole update run in libdtx; the
+CFL / non-finite diagnostics are printed with the reference's wording when the device reports
+them.  With the `save` result true the call also waits for the step to finish.
+"""
+function dtx_step!(method, t_start, t_stop, dt, Verbose::Int64)
+    h = method.HIP
+    dt_out = Ref{Float64}(0.0)
+    save = Ref{Int32}(0)
+    dtx_check(h, ccall((:dtx_step, libdtx), Cint,
+                       (Ptr{Cvoid}, Float64, Float64, Float64, Int32, Ref{Float64}, Ref{Int32},
+                        Ptr{DtxStepStats}),
+                       h.ptr, Float64(t_start), Float64(t_stop), Float64(dt), Int32(Verbose),
+                       dt_out, save, C_NULL))
+    if save[] != 0
+        dtx_check(h, ccall((:dtx_synchronize, libdtx), Cint, (Ptr{Cvoid},), h.ptr))
+    end
+    return dt_out[], save[] != 0
+end
+
+"""
+    dtx_moments(method) -> (number, energy)
+
+Number N^0 and energy T^00 of the current state per (cell, species), each a
+(num_species x n_space) matrix, from the FP64 sums the step kernel keeps.
+"""
+function dtx_moments(method)
+    h = method.HIP
+    num = Matrix{Float64}(undef, h.num_species, h.n_space)
+    en = Matrix{Float64}(undef, h.num_species, h.n_space)
+    dtx_check(h, ccall((:dtx_moments, libdtx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                       h.ptr, num, en))
+    return num, en
+end
