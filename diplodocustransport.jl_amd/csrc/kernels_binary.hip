@@ -12,7 +12,8 @@
 //
 // Every workgroup takes one work item (channel, 64-row tile, cell tile, chunk range) from a work
 // table and writes its FP64 partial sum to its own slot, so the reduction order is fixed by the
-// item list alone: a simulation gives bit-identical results alone or grouped with others.
+// item list alone: a simulation gives bit-identical results alone or grouped with others.  LOSS
+// items share the launch with the gain items of any precision (they take the FP64 body).
 //
 // Wave tiling: 4 waves = WK (rows) x WC (cells); each wave owns MR x MC 16x16 MFMA tiles.
 // MFMA operand layout (16x16xK): lane l supplies A[l%16][kgroup(l/16)], B[kgroup(l/16)][l%16].
@@ -46,8 +47,7 @@ struct ItemView {
     int c_begin, c_end, krow0, row_base, row_pad_base, rows, cells, ldc, ldb, N, c0, kind;
 };
 
-__device__ inline ItemView load_item(const BinWork* items, int tc, int elem_bytes) {
-    const BinWork& w = items[blockIdx.x];
+__device__ inline ItemView load_item(const BinWork& w, int tc, int elem_bytes) {
     ItemView v;
     v.A = (const char*)w.A + (int64_t)w.ktile * w.a_ktile_stride * elem_bytes;
     v.chunks = w.chunks;
@@ -112,17 +112,25 @@ __device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
 // hold the whole item's contribution and no epilogue runs inside the loop.  Each 32-j chunk is
 // staged through LDS in two halves of 16 j, double-buffered (32 KB of LDS).
 template <int TC>
-__global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict__ items) {
+struct SmemF64 {
+    static constexpr int HK = KC / 2;   // j per LDS stage
+    double As[2][HK * TK];
+    double Bs[2][HK * TC];
+    uint32_t ctab[MAX_RANGE];
+};
+
+template <int TC>
+__device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     using T = Tiling<TC>;
-    constexpr int HK = KC / 2;          // j per LDS stage
+    constexpr int HK = SmemF64<TC>::HK;
     constexpr int AP = HK * TK / 256;   // 4 A elements per thread
     constexpr int BP = HK * TC / 256;   // B elements per thread (consecutive cells)
     constexpr int BT = TC / BP;         // threads per B row
-    __shared__ double As[2][HK * TK];
-    __shared__ double Bs[2][HK * TC];
-    __shared__ uint32_t ctab[MAX_RANGE];
+    auto& As = sm.As;
+    auto& Bs = sm.Bs;
+    uint32_t* ctab = sm.ctab;
 
-    const ItemView v = load_item(items, TC, 8);
+    const ItemView v = load_item(w, TC, 8);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
     const double* A = (const double*)v.A + (int64_t)v.c_begin * KC * TK;
@@ -200,22 +208,35 @@ __global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict_
         }
 }
 
+template <int TC>
+__global__ __launch_bounds__(256) void bin_f64_kernel(const BinWork* __restrict__ items) {
+    __shared__ SmemF64<TC> sm;
+    f64_item<TC>(items[blockIdx.x], sm);
+}
+
 // ------------------------------------------------------------------------------- FP32 -----
 // As FP64, with B'[t][c] = (float)(f_i^c * f_j^c) formed in FP64 and rounded once, FP32 MFMA
 // accumulation, and the FP32 accumulators folded into FP64 every FLUSH_CHUNKS chunks.
 constexpr int FLUSH_CHUNKS = 16;
 
 template <int TC>
-__global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict__ items) {
+struct SmemF32 {
+    float As[2][KC * TK];
+    float Bs[2][KC * TC];
+    uint32_t ctab[MAX_RANGE];
+};
+
+template <int TC>
+__device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
     using T = Tiling<TC>;
     constexpr int AP = KC * TK / 256;   // 8
     constexpr int BP = KC * TC / 256;
     constexpr int BT = TC / BP;
-    __shared__ float As[2][KC * TK];
-    __shared__ float Bs[2][KC * TC];
-    __shared__ uint32_t ctab[MAX_RANGE];
+    auto& As = sm.As;
+    auto& Bs = sm.Bs;
+    uint32_t* ctab = sm.ctab;
 
-    const ItemView v = load_item(items, TC, 4);
+    const ItemView v = load_item(w, TC, 4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
     const float* A = (const float*)v.A + (int64_t)v.c_begin * KC * TK;
@@ -302,6 +323,18 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
             store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, tot[r][c]);
 }
 
+// LOSS items (FP64 in every mode) run the FP64 body inside the same launch.
+template <int TC>
+__global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict__ items) {
+    __shared__ union {
+        SmemF64<TC> d;
+        SmemF32<TC> s;
+    } sm;
+    const BinWork& w = items[blockIdx.x];
+    if (w.kind == CH_LOSS) f64_item<TC>(w, sm.d);
+    else f32_item<TC>(w, sm.s);
+}
+
 // ------------------------------------------------------------------------ split BF16 x3 -----
 // B is the cell-major hi/lo split of f (no per-element work while staging); the row multiplier
 // f_i is staged in LDS beside each chunk and folded into FP64 when the row ends.  Packed A per
@@ -310,16 +343,26 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
 constexpr int BPITCH = 40;
 
 template <int TC>
-__global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restrict__ items) {
-    using T = Tiling<TC>;
-    __shared__ __attribute__((aligned(16))) uint16_t Ah[2][TK * BPITCH];
-    __shared__ __attribute__((aligned(16))) uint16_t Al[2][TK * BPITCH];
-    __shared__ __attribute__((aligned(16))) uint16_t Bh[2][TC * BPITCH];
-    __shared__ __attribute__((aligned(16))) uint16_t Bl[2][TC * BPITCH];
-    __shared__ double Fs[2][TC];
-    __shared__ uint32_t ctab[MAX_RANGE];
+struct SmemBF16 {
+    __attribute__((aligned(16))) uint16_t Ah[2][TK * BPITCH];
+    __attribute__((aligned(16))) uint16_t Al[2][TK * BPITCH];
+    __attribute__((aligned(16))) uint16_t Bh[2][TC * BPITCH];
+    __attribute__((aligned(16))) uint16_t Bl[2][TC * BPITCH];
+    double Fs[2][TC];
+    uint32_t ctab[MAX_RANGE];
+};
 
-    const ItemView v = load_item(items, TC, 2);
+template <int TC>
+__device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) {
+    using T = Tiling<TC>;
+    auto& Ah = sm.Ah;
+    auto& Al = sm.Al;
+    auto& Bh = sm.Bh;
+    auto& Bl = sm.Bl;
+    auto& Fs = sm.Fs;
+    uint32_t* ctab = sm.ctab;
+
+    const ItemView v = load_item(w, TC, 2);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
     const uint16_t* A = (const uint16_t*)v.A + (int64_t)v.c_begin * TK * KC * 2;
@@ -419,6 +462,17 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
 #pragma unroll
         for (int c = 0; c < T::MC; ++c)
             store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, tot[r][c]);
+}
+
+template <int TC>
+__global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restrict__ items) {
+    __shared__ union {
+        SmemF64<TC> d;
+        SmemBF16<TC> b;
+    } sm;
+    const BinWork& w = items[blockIdx.x];
+    if (w.kind == CH_LOSS) f64_item<TC>(w, sm.d);
+    else bf16x3_item<TC>(w, sm.b);
 }
 
 // Pack rows [row0, row0+rows) of the FP64 loss matrix D (N x N, column-major) as a LOSS

@@ -39,8 +39,7 @@ __device__ inline double cr_min(double m, double r) {
     return m;
 }
 
-// Block reductions: a butterfly over the 64 lanes of each wave (__shfl_xor), then the four wave
-// results combined in wave order.  The pattern is fixed, so every reduction is deterministic.
+// Wave reductions: a butterfly over the 64 lanes of each wave (__shfl_xor).
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) v = v + __shfl_xor(v, m, 64);
@@ -51,28 +50,6 @@ __device__ inline double wave_min(double v) {
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) v = cr_min(v, __shfl_xor(v, m, 64));
     return v;
-}
-
-__device__ double block_sum(double v, double* red) {
-    v = wave_sum(v);
-    const int tid = threadIdx.x;
-    if ((tid & 63) == 0) red[tid >> 6] = v;
-    __syncthreads();
-    double r = red[0];
-    for (int w = 1; w < UB / 64; ++w) r = r + red[w];
-    __syncthreads();
-    return r;
-}
-
-__device__ double block_min(double v, double* red) {
-    v = wave_min(v);
-    const int tid = threadIdx.x;
-    if ((tid & 63) == 0) red[tid >> 6] = v;
-    __syncthreads();
-    double r = red[0];
-    for (int w = 1; w < UB / 64; ++w) r = cr_min(r, red[w]);
-    __syncthreads();
-    return r;
 }
 
 __device__ int block_or(int v, int* red) {
@@ -86,14 +63,28 @@ __device__ int block_or(int v, int* red) {
     return r;
 }
 
-// SpMV row: sum over ELL slots in ascending column order, separate multiply and add.
+// SpMV row: sum over ELL slots in ascending column order, separate multiply and add.  A row's
+// column and value loads are all issued before the first gather, and the gathers before the
+// first add, so a row costs two memory round trips whatever its width.  Padding slots (column
+// -1) sit after a row's entries and are skipped.
 __device__ inline double ell_row(const int32_t* col, const double* val, int width, int64_t n,
                                  int64_t r, const double* x) {
+    constexpr int EB = 8;
     double acc = 0.0;
-    for (int s = 0; s < width; ++s) {
-        const int32_t c = col[(int64_t)s * n + r];
-        if (c < 0) break;
-        acc = acc + val[(int64_t)s * n + r] * x[c];
+    for (int s0 = 0; s0 < width; s0 += EB) {
+        int32_t c[EB];
+        double v[EB], xv[EB];
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
+            const bool in = s0 + u < width;
+            c[u] = in ? col[(int64_t)(s0 + u) * n + r] : -1;
+            v[u] = in ? val[(int64_t)(s0 + u) * n + r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < EB; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < EB; ++u)
+            if (c[u] >= 0) acc = acc + v[u] * xv[u];
     }
     return acc;
 }
@@ -103,19 +94,22 @@ __device__ inline double ell_row(const int32_t* col, const double* val, int widt
 // slots q, q+4, ...; the four sums are combined as (s0 + s1) + (s2 + s3) by a butterfly, so every
 // lane holds the same, order-fixed result.
 __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, bool valid) {
+    constexpr int SB = 8;  // slot loads in flight per thread
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
     if (valid) {
         const int ns = a.nslot[k];
-        int s = q;
-        for (; s + 12 < ns; s += 16) {  // four loads in flight per thread
-            double v[4];
+        for (int s0 = q; s0 < ns; s0 += 4 * SB) {
+            double v[SB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = a.part[(int64_t)(s + 4 * u) * a.part_stride + idx];
+            for (int u = 0; u < SB; ++u) {
+                const int s = s0 + 4 * u;
+                v[u] = s < ns ? a.part[(int64_t)s * a.part_stride + idx] : 0.0;
+            }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) g = g + v[u];
+            for (int u = 0; u < SB; ++u)
+                if (s0 + 4 * u < ns) g = g + v[u];
         }
-        for (; s < ns; s += 4) g = g + a.part[(int64_t)s * a.part_stride + idx];
     }
     g = g + __shfl_xor(g, 1, 64);
     g = g + __shfl_xor(g, 2, 64);
@@ -143,99 +137,142 @@ __device__ inline double emission_term(const UpdateArgs& a, int c, int k, const 
     return acc;
 }
 
-// Per-species moment partials of this block's new states (fn, LDS, indexed by local state) and
-// the per-block partials (sums and Cr minima).  finalize_step_kernel folds them in block order.
-__device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, const double* fn,
-                             int k0, int k1, double* red, double sumdf, double sumf, double cr,
+// Values reduced together by one workgroup: per-species number and energy sums, then the block
+// partial words (sums first, minima after).
+constexpr int RED_VALUES = 2 * DTX_MAX_SPECIES + BP_WORDS;
+
+// Fused workgroup reductions.  Each value is first reduced over its wave by a butterfly
+// (wave_put), and after one barrier thread t combines the four wave results of value t in wave
+// order (wave_combine).  The pattern is fixed, so the reductions are deterministic.
+__device__ inline void wave_put(double x, int i, bool is_sum, double* red) {
+    const double w = is_sum ? wave_sum(x) : wave_min(x);
+    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * RED_VALUES + i] = w;
+}
+
+__device__ inline double wave_combine(int t, bool is_sum, const double* red) {
+    double r = red[t];
+    for (int w = 1; w < UB / 64; ++w)
+        r = is_sum ? r + red[w * RED_VALUES + t] : cr_min(r, red[w * RED_VALUES + t]);
+    return r;
+}
+
+// Per-species moment partials of this block's new states and the per-block partials (sums and
+// Cr minima), in one fused reduction.  Thread (tid & 3) == 0 of each state holds its values;
+// finalize_step_kernel folds the partials in block order.
+__device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, bool mine, int k,
+                             double fnew, double* red, double sumdf, double sumf, double cr,
                              double crb, double cre, double crf, double crm, double crs) {
     const int tid = threadIdx.x;
     const int64_t b = (int64_t)c * nseg + seg;
-    for (int s = 0; s < a.S; ++s) {
-        const int lo = max(k0, a.species_off[s]), hi = min(k1, a.species_off[s + 1]);
-        double num = 0.0, en = 0.0;
-        if (lo < hi) {
-            const int k = k0 + (tid >> 2);
-            if ((tid & 3) == 0 && k >= lo && k < hi) {
-                num = fn[tid >> 2];
-                if (a.eweight) en = fn[tid >> 2] * a.eweight[k];
-            }
-            num = block_sum(num, red);
-            en = block_sum(en, red);
-        }
-        if (tid == 0) {
-            a.mom_part[(b * a.S + s) * 2 + 0] = num;
-            a.mom_part[(b * a.S + s) * 2 + 1] = en;
-        }
+    const int S = a.S;
+    const int sp = mine ? a.state_species[k] : -1;
+    const double ew = (mine && a.eweight) ? a.eweight[k] : 0.0;
+    for (int s = 0; s < S; ++s) {
+        wave_put(s == sp ? fnew : 0.0, 2 * s, true, red);
+        wave_put((s == sp && a.eweight) ? fnew * ew : 0.0, 2 * s + 1, true, red);
     }
-    sumdf = block_sum(sumdf, red);
-    sumf = block_sum(sumf, red);
-    if (a.verbose >= 1 && a.verbose <= 3) {
-        cr = block_min(cr, red);
+    const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
+    const int w0 = 2 * S;
+    wave_put(sumdf, w0 + BP_SUMDF, true, red);
+    wave_put(sumf, w0 + BP_SUMF, true, red);
+    if (want_cr) {
+        wave_put(cr, w0 + BP_CR, false, red);
         if (a.verbose == 3) {
-            crb = block_min(crb, red);
-            cre = block_min(cre, red);
-            crf = block_min(crf, red);
-            crm = block_min(crm, red);
-            crs = block_min(crs, red);
+            wave_put(crb, w0 + BP_CR_BIN, false, red);
+            wave_put(cre, w0 + BP_CR_EMI, false, red);
+            wave_put(crf, w0 + BP_CR_FLUX, false, red);
+            wave_put(crm, w0 + BP_CR_MOM, false, red);
+            wave_put(crs, w0 + BP_CR_SPACE, false, red);
         }
     }
-    if (tid == 0) {
-        double* bp = a.block_part + b * BP_WORDS;
-        bp[BP_SUMDF] = sumdf;
-        bp[BP_SUMF] = sumf;
-        bp[BP_CR] = cr;
-        bp[BP_CR_BIN] = crb;
-        bp[BP_CR_EMI] = cre;
-        bp[BP_CR_FLUX] = crf;
-        bp[BP_CR_MOM] = crm;
-        bp[BP_CR_SPACE] = crs;
+    __syncthreads();
+    if (tid < w0) {
+        a.mom_part[(b * S + (tid >> 1)) * 2 + (tid & 1)] = wave_combine(tid, true, red);
+    } else if (tid < w0 + BP_WORDS) {
+        const int w = tid - w0;
+        const bool reduced = w <= BP_SUMF || (want_cr && (w == BP_CR || a.verbose == 3));
+        a.block_part[b * BP_WORDS + w] =
+            reduced ? wave_combine(tid, w <= BP_SUMF, red) : INFINITY;
     }
 }
 
 // One workgroup: folds the per-block partials of the step (block order) into the moments of
-// every (cell, species) and the status record of the step.
+// every (cell, species) and the status record of the step.  Loads are issued in batches ahead
+// of the ordered adds.
 __device__ void finalize_step_body(const UpdateArgs& a, int nseg, double* red) {
+    constexpr int LB = 8;
     const int tid = threadIdx.x;
     const int64_t nblocks = (int64_t)a.C * nseg;
     for (int64_t q = tid; q < (int64_t)a.C * a.S; q += UB) {
         const int64_t cc = q / a.S, sp = q % a.S;
         double num = 0.0, en = 0.0;
-        for (int g = 0; g < nseg; ++g) {
-            const double* mp = a.mom_part + (((cc * nseg + g) * a.S) + sp) * 2;
-            num = num + mp[0];
-            en = en + mp[1];
+        for (int g0 = 0; g0 < nseg; g0 += LB) {
+            double pn[LB], pe[LB];
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int g = g0 + u;
+                const double* mp = a.mom_part + (((cc * nseg + g) * a.S) + sp) * 2;
+                pn[u] = g < nseg ? mp[0] : 0.0;
+                pe[u] = g < nseg ? mp[1] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < LB; ++u)
+                if (g0 + u < nseg) {
+                    num = num + pn[u];
+                    en = en + pe[u];
+                }
         }
         a.moments[q * 2 + 0] = num;
         a.moments[q * 2 + 1] = en;
     }
     double v[BP_WORDS];
+#pragma unroll
     for (int q = 0; q < BP_WORDS; ++q) v[q] = (q <= BP_SUMF) ? 0.0 : INFINITY;
-    for (int64_t bb = tid; bb < nblocks; bb += UB) {
-        const double* bp = a.block_part + bb * BP_WORDS;
-        v[BP_SUMDF] = v[BP_SUMDF] + bp[BP_SUMDF];
-        v[BP_SUMF] = v[BP_SUMF] + bp[BP_SUMF];
-        for (int q = BP_CR; q < BP_WORDS; ++q) v[q] = cr_min(v[q], bp[q]);
+    constexpr int BB = 4;  // blocks per batch (BB * BP_WORDS loads in flight)
+    for (int64_t b0 = tid; b0 < nblocks; b0 += (int64_t)UB * BB) {
+        double p[BB][BP_WORDS];
+#pragma unroll
+        for (int u = 0; u < BB; ++u) {
+            const int64_t bb = b0 + (int64_t)u * UB;
+#pragma unroll
+            for (int q = 0; q < BP_WORDS; ++q)
+                p[u][q] = bb < nblocks ? a.block_part[bb * BP_WORDS + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < BB; ++u) {
+            if (b0 + (int64_t)u * UB >= nblocks) break;
+            v[BP_SUMDF] = v[BP_SUMDF] + p[u][BP_SUMDF];
+            v[BP_SUMF] = v[BP_SUMF] + p[u][BP_SUMF];
+#pragma unroll
+            for (int q = BP_CR; q < BP_WORDS; ++q) v[q] = cr_min(v[q], p[u][q]);
+        }
     }
-    v[BP_SUMDF] = block_sum(v[BP_SUMDF], red);
-    v[BP_SUMF] = block_sum(v[BP_SUMF], red);
-    for (int q = BP_CR; q < BP_WORDS; ++q) v[q] = block_min(v[q], red);
+#pragma unroll
+    for (int q = 0; q < BP_WORDS; ++q) wave_put(v[q], q, q <= BP_SUMF, red);
+    __syncthreads();
+    const double r = tid < BP_WORDS ? wave_combine(tid, tid <= BP_SUMF, red) : 0.0;
+    __syncthreads();
+    if (tid < BP_WORDS) red[tid] = r;
+    __syncthreads();
     if (tid == 0) {
+        double w[BP_WORDS];
+        for (int q = 0; q < BP_WORDS; ++q) w[q] = red[q];
         double* st = a.status + (int64_t)(a.step_index % a.ring) * STATUS_WORDS;
-        const bool cr_eval = (a.verbose >= 1 && a.verbose <= 3) && (v[BP_SUMF] != 0.0);
+        const bool cr_eval = (a.verbose >= 1 && a.verbose <= 3) && (w[BP_SUMF] != 0.0);
         st[ST_STEP] = (double)a.step_index;
-        st[ST_FINITE] = isfinite(v[BP_SUMDF]) ? 1.0 : 0.0;
+        st[ST_FINITE] = isfinite(w[BP_SUMDF]) ? 1.0 : 0.0;
         st[ST_CR_EVAL] = cr_eval ? 1.0 : 0.0;
-        st[ST_CR] = cr_eval ? -v[BP_CR] : 0.0;
-        st[ST_CR_BIN] = (cr_eval && a.verbose == 3 && a.binary) ? -v[BP_CR_BIN] : 0.0;
-        st[ST_CR_EMI] = (cr_eval && a.verbose == 3 && a.emission) ? -v[BP_CR_EMI] : 0.0;
-        st[ST_CR_FLUX] = (cr_eval && a.verbose == 3) ? -v[BP_CR_FLUX] : 0.0;
-        st[ST_CR_MOM] = (cr_eval && a.verbose == 3) ? -v[BP_CR_MOM] : 0.0;
-        st[ST_CR_SPACE] = (cr_eval && a.verbose == 3) ? -v[BP_CR_SPACE] : 0.0;
-        st[ST_SUMF] = v[BP_SUMF];
+        st[ST_CR] = cr_eval ? -w[BP_CR] : 0.0;
+        st[ST_CR_BIN] = (cr_eval && a.verbose == 3 && a.binary) ? -w[BP_CR_BIN] : 0.0;
+        st[ST_CR_EMI] = (cr_eval && a.verbose == 3 && a.emission) ? -w[BP_CR_EMI] : 0.0;
+        st[ST_CR_FLUX] = (cr_eval && a.verbose == 3) ? -w[BP_CR_FLUX] : 0.0;
+        st[ST_CR_MOM] = (cr_eval && a.verbose == 3) ? -w[BP_CR_MOM] : 0.0;
+        st[ST_CR_SPACE] = (cr_eval && a.verbose == 3) ? -w[BP_CR_SPACE] : 0.0;
+        st[ST_SUMF] = w[BP_SUMF];
         st[ST_DT] = a.dt;
         st[ST_TSTART] = a.t_start;
         st[ST_SAVE] = (double)a.save;
-        st[ST_SUMDF] = v[BP_SUMDF];
+        st[ST_SUMDF] = w[BP_SUMDF];
     }
 }
 
@@ -255,7 +292,6 @@ __device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
 // ----------------------------------------------------------------------- ForwardEuler ------
 __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg, double* smem,
                                   double* red, int* ired) {
-    double* fn = smem;                    // new f of this block's states
     double* fc = smem + UPDATE_STATES;    // f of the whole cell (emission / Domain only)
     const int tid = threadIdx.x, N = a.N;
     const int k0 = seg * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
@@ -267,7 +303,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
 
     double sumdf = 0.0, sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY,
-           crf = INFINITY;
+           crf = INFINITY, fnew = 0.0;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const int k = k0 + (tid >> 2), q = tid & 3;
     const bool mine = k < k1;
@@ -299,23 +335,22 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
         x = strong_zero(x, x >= THR_1F_10);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
-        fn[tid >> 2] = x;
+        fnew = x;
         publish(a, c, k, x);
     }
-    __syncthreads();
-    finish_block(a, c, seg, nseg, fn, k0, k1, red, sumdf, sumf, cr, crb, cre, crf, INFINITY,
-                 INFINITY);
+    finish_block(a, c, seg, nseg, mine && q == 0, k, fnew, red, sumdf, sumf, cr, crb, cre, crf,
+                 INFINITY, INFINITY);
 }
 
 __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
-    __shared__ double red[UB / 64];
+    __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
     euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
 }
 
 __global__ __launch_bounds__(UB) void finalize_step_kernel(UpdateArgs a, int nseg) {
-    __shared__ double red[UB / 64];
+    __shared__ double red[(UB / 64) * RED_VALUES];
     finalize_step_body(a, nseg, red);
 }
 
@@ -326,7 +361,7 @@ __global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs
                                                                 const int32_t* block_start,
                                                                 int count) {
     extern __shared__ double smem[];
-    __shared__ double red[UB / 64];
+    __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
     const int b = blockIdx.x;
     int lo = 0, hi = count - 1;
@@ -342,7 +377,7 @@ __global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs
 }
 
 __global__ __launch_bounds__(UB) void finalize_group_kernel(const UpdateArgs* table) {
-    __shared__ double red[UB / 64];
+    __shared__ double red[(UB / 64) * RED_VALUES];
     const UpdateArgs a = table[blockIdx.x];
     finalize_step_body(a, (a.N + UPDATE_STATES - 1) / UPDATE_STATES, red);
 }
@@ -388,13 +423,12 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
 // Space half and final update (src/SteppingMethods.jl:162-237).  The ELL fields hold X_Flux.
 __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
-    double* fn = smem;
-    __shared__ double red[UB / 64];
+    __shared__ double red[(UB / 64) * RED_VALUES];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
     const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
     double sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY, crf = INFINITY;
-    double crm = INFINITY, crs = INFINITY;
+    double crm = INFINITY, crs = INFINITY, fnew = 0.0;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const int k = k0 + (tid >> 2);
     if (k < k1 && (tid & 3) == 0) {
@@ -425,11 +459,11 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
         x = strong_zero(x, x >= THR_1F_20) * jl_sign(x);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
-        fn[tid >> 2] = x;
+        fnew = x;
         publish(a, c, k, x);
     }
-    __syncthreads();
-    finish_block(a, c, blockIdx.y, gridDim.y, fn, k0, k1, red, 0.0, sumf, cr, crb, cre, crf, crm,
+    finish_block(a, c, blockIdx.y, gridDim.y, k < k1 && (tid & 3) == 0, k, fnew, red, 0.0, sumf,
+                 cr, crb, cre, crf, crm,
                  crs);
 }
 

@@ -72,10 +72,9 @@ struct dtx_handle {
     std::vector<PackedChannel> chans;
     double* d_loss = nullptr;
     bool loss_present = false;
-    BinWork* d_items = nullptr;
-    BinWork* d_items_loss = nullptr;
-    std::vector<BinWork> items_host, items_loss_host;  // host copies (grouped launches)
-    int32_t nitems = 0, nitems_loss = 0, tc = 16, ldc = 0, max_slots = 0;
+    BinWork* d_items = nullptr;     // loss and gain items of one launch
+    std::vector<BinWork> items_host;  // host copy (grouped launches)
+    int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0;
     double* d_part = nullptr;
     int64_t part_stride = 0;
     int32_t* d_nslot = nullptr;
@@ -381,10 +380,9 @@ static int pack_loss_channels(dtx_handle* h) {
 // Build the grouped work tables for the handle's channels and cell count.
 static int build_work(dtx_handle* h) {
     dfree(h, &h->d_items, h->nitems);
-    dfree(h, &h->d_items_loss, h->nitems_loss);
     dfree(h, &h->d_part, h->part_stride * h->max_slots);
     dfree(h, &h->d_nslot, h->N);
-    h->nitems = h->nitems_loss = 0;
+    h->nitems = 0;
     h->max_slots = 0;
     const int C = h->C;
     const int ctiles = (C + h->tc - 1) / h->tc;
@@ -402,32 +400,40 @@ static int build_work(dtx_handle* h) {
         }
         h->gain_bytes += (double)c.a_bytes;
     }
-    const int target_items = 1024;
+    // About four resident workgroups per CU for the 64-cell tiles, two for the 128-cell tiles
+    // (their LDS and registers allow two).
+    const int target_items = h->tc >= 128 ? 512 : 1024;
     int64_t range = (int64_t)std::ceil(total / target_items);
     range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
+    // LOSS channels are a small FP64 GEMV next to the gain contraction: short ranges keep their
+    // items from forming the tail of the launch.
+    const int64_t loss_range = std::min<int64_t>(range, 4);
     std::vector<int> next_slot(h->S, 0);
     h->part_stride = (int64_t)C * h->N;
-    struct Pending { BinWork w; int slot; bool loss; };
-    std::vector<Pending> pend;
+    // A unit is (channel, chunk range, k tile); it has one item per cell tile.  Gain slots of a
+    // species come first, then its loss slots, and the update kernel adds them in slot order.
+    struct Unit { BinWork w; int slot; bool loss; };
+    std::vector<Unit> gain_units, loss_units;
     const bool bf16 = h->precision == DTX_PREC_BF16X3;
-    for (auto& c : h->chans) {
-        const bool loss = c.kind == CH_LOSS;
-        const int prec = loss ? DTX_PREC_FP64 : h->precision;
-        for (int64_t c0 = 0; c0 < c.nchunks; c0 += range) {
-            const int64_t c1 = std::min<int64_t>(c0 + range, c.nchunks);
-            const int slot = next_slot[c.s_out]++;
-            for (int kt = 0; kt < c.nktiles; ++kt)
-                for (int ct = 0; ct < ctiles; ++ct) {
+    for (int pass = 0; pass < 2; ++pass)
+        for (auto& c : h->chans) {
+            const bool loss = c.kind == CH_LOSS;
+            if (loss != (pass == 1)) continue;
+            const int prec = loss ? DTX_PREC_FP64 : h->precision;
+            const int64_t rg = loss ? loss_range : range;
+            for (int64_t c0 = 0; c0 < c.nchunks; c0 += rg) {
+                const int64_t c1 = std::min<int64_t>(c0 + rg, c.nchunks);
+                const int slot = next_slot[c.s_out]++;
+                for (int kt = 0; kt < c.nktiles; ++kt) {
                     BinWork w{};
                     w.A = c.A;
                     w.chunks = c.chunks;
                     w.fT = h->d_fT;
                     if (prec == DTX_PREC_FP64) w.B = h->d_fT;
                     else w.B = h->d_fhi;
-                    w.b_lo = bf16 ? (int64_t)(h->d_flo - h->d_fhi) : 0;
+                    w.b_lo = (bf16 && !loss) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
                     w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
                     w.ktile = kt;
-                    w.ctile = ct;
                     w.c_begin = (int32_t)c0;
                     w.c_end = (int32_t)c1;
                     w.row_base = h->sp_off[c.s_out];
@@ -438,32 +444,32 @@ static int build_work(dtx_handle* h) {
                     w.ldb = h->Np;
                     w.N = h->N;
                     w.kind = c.kind;
-                    pend.push_back({w, slot, loss && h->precision != DTX_PREC_FP64});
+                    (loss ? loss_units : gain_units).push_back({w, slot, loss});
                 }
+            }
         }
-    }
     for (int s = 0; s < h->S; ++s) h->max_slots = std::max(h->max_slots, next_slot[s]);
     if (h->max_slots > 0) {
         int rc = dalloc(h, &h->d_part, h->part_stride * h->max_slots);
         if (rc) return rc;
     }
-    std::vector<BinWork> items, items_loss;
-    for (auto& pd : pend) {
-        pd.w.part = h->d_part + (int64_t)pd.slot * h->part_stride;
-        (pd.loss ? items_loss : items).push_back(pd.w);
-    }
+    // Launch order: gain units, then the short loss units; the cell tiles of a unit (which read
+    // the same operator block) are adjacent.
+    std::vector<BinWork> items;
+    items.reserve((gain_units.size() + loss_units.size()) * ctiles);
+    for (const auto* list : {&gain_units, &loss_units})
+        for (const Unit& u : *list)
+            for (int ct = 0; ct < ctiles; ++ct) {
+                BinWork w = u.w;
+                w.ctile = ct;
+                w.part = h->d_part + (int64_t)u.slot * h->part_stride;
+                items.push_back(w);
+            }
     h->nitems = (int32_t)items.size();
-    h->nitems_loss = (int32_t)items_loss.size();
     h->items_host = items;
-    h->items_loss_host = items_loss;
     if (h->nitems) {
         int rc = dalloc(h, &h->d_items, h->nitems);
         if (!rc) rc = upload(h, h->d_items, items.data(), h->nitems);
-        if (rc) return rc;
-    }
-    if (h->nitems_loss) {
-        int rc = dalloc(h, &h->d_items_loss, h->nitems_loss);
-        if (!rc) rc = upload(h, h->d_items_loss, items_loss.data(), h->nitems_loss);
         if (rc) return rc;
     }
     std::vector<int32_t> nslot(h->N, 0);
@@ -643,7 +649,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.C = h->C;
     a.S = h->S;
     a.ldc = h->ldc;
-    a.binary = h->binary && (h->nitems + h->nitems_loss > 0);
+    a.binary = h->binary && h->nitems > 0;
     a.emission = h->emission;
     a.verbose = verbose;
     a.step_index = (int32_t)(h->step_counter % 0x7FFFFFFF);
@@ -674,14 +680,12 @@ static int timed_end(dtx_handle* h, TimedEvent* ev) {
 // Enqueue one functor call with an already clipped dt.
 static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save) {
     const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
-    const bool bin = h->binary && (h->nitems + h->nitems_loss) > 0;
+    const bool bin = h->binary && h->nitems > 0;
     TimedEvent ev{};
     if (bin) {
         int rc = timed_begin(h, 0, &ev);
         if (rc) return rc;
         HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
-        HIP_CHECK(h, launch_binary(DTX_PREC_FP64, h->tc, h->d_items_loss, h->nitems_loss,
-                                   h->stream));
         rc = timed_end(h, &ev);
         if (rc) return rc;
     }
@@ -767,8 +771,8 @@ struct Group {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<dtx_handle*> members;
-    BinWork* d_items[4] = {nullptr, nullptr, nullptr, nullptr};  // gain16 gain64 loss16 loss64
-    int n_items[4] = {0, 0, 0, 0};
+    BinWork* d_items[3] = {nullptr, nullptr, nullptr};  // members with cell tiles of 16/64/128
+    int n_items[3] = {0, 0, 0};
     int precision = 0;
     int32_t* d_block_start = nullptr;
     int total_blocks = 0, max_N = 0;
@@ -782,7 +786,7 @@ struct Group {
 };
 
 static void group_free_tables(Group* g) {
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 3; ++q) {
         if (g->d_items[q]) (void)hipFree(g->d_items[q]);
         g->d_items[q] = nullptr;
         g->n_items[q] = 0;
@@ -833,7 +837,7 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
     if (rc) return rc;
     group_free_tables(g);
     g->members.assign(hs, hs + count);
-    std::vector<BinWork> lists[4];
+    std::vector<BinWork> lists[3];  // cell tiles of 16, 64 and 128
     std::vector<int32_t> starts;
     g->total_blocks = 0;
     g->max_N = 0;
@@ -857,17 +861,14 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
             g->err = "member stream failed";
             return DTX_ERR_HIP;
         }
-        const int wide = h->tc == 64 ? 1 : 0;
-        if (h->binary) {
+        const int wide = h->tc == 16 ? 0 : (h->tc == 64 ? 1 : 2);
+        if (h->binary)
             lists[wide].insert(lists[wide].end(), h->items_host.begin(), h->items_host.end());
-            lists[2 + wide].insert(lists[2 + wide].end(), h->items_loss_host.begin(),
-                                   h->items_loss_host.end());
-        }
         starts.push_back(g->total_blocks);
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
         g->max_N = std::max(g->max_N, h->N);
     }
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 3; ++q) {
         g->n_items[q] = (int)lists[q].size();
         if (!g->n_items[q]) continue;
         if (hipMalloc((void**)&g->d_items[q], lists[q].size() * sizeof(BinWork)) != hipSuccess ||
@@ -925,11 +926,9 @@ int group_step(Group* g, const double* dt, int verbose) {
                                   hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess) e = hipEventRecord(g->ev[slot], g->stream);
     g->ev_used[slot] = true;
-    const int kprec[4] = {g->precision, g->precision, DTX_PREC_FP64, DTX_PREC_FP64};
-    const int ktc[4] = {16, 64, 16, 64};
-    for (int q = 0; q < 4 && e == hipSuccess; ++q)
-        if (!(g->precision == DTX_PREC_FP64 && q >= 2))
-            e = launch_binary(kprec[q], ktc[q], g->d_items[q], g->n_items[q], g->stream);
+    const int ktc[3] = {16, 64, 128};
+    for (int q = 0; q < 3 && e == hipSuccess; ++q)
+        e = launch_binary(g->precision, ktc[q], g->d_items[q], g->n_items[q], g->stream);
     if (e == hipSuccess)
         e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
                                       g->max_N, g->stream);
