@@ -26,6 +26,24 @@ sys.path.insert(0, str(ROOT / "diplodocustransport.jl_amd"))
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 PEAK_HBM_GBS = 8000.0
 
+DESCRIPTIONS = {
+    "transport_1d": "2 species x 256 states (32 p x 8 u x 1 phi), 128 periodic x cells, 65536 "
+                    "unknowns, upwind x advection + synchrotron cooling, per-cell binary "
+                    "collisions, plain FP64",
+    "precision_box": "homogeneous box, 3 species x 1024 states (64 p x 8 u x 2 phi), 9 gain "
+                     "channels of up to 1024^3 generated on the device, plain FP64",
+    "transport_2d": "2 species x 512 states (32 p x 8 u x 2 phi), 32 x 32 periodic x-y cells, "
+                    "1048576 unknowns, upwind x/y advection + synchrotron cooling, per-cell "
+                    "binary collisions with a shared operator, plain FP64",
+    "homogeneous_box": "homogeneous box, 2 species x 256 states (32 p x 8 u x 1 phi), "
+                       "monolithic 512^2 x 512 FP64 binary tensor, plain FP64",
+}
+
+# HBM bytes per launch of the FP64 contraction from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (separate runs, each scaled by the same counter over a 1 GiB device copy):
+# profiles/transport_1d_hbm_traffic.txt.
+TRAFFIC = {("transport_1d", "bin_f64_kernel<64>"): None}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -41,9 +59,10 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_gpu(w, precision, steps, warmup, verbose=1):
+def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True):
     from dtx import workloads
-    m = workloads.make_struct(w, precision=precision, use_monolithic=False)
+    m = workloads.make_struct(w, precision=precision, use_monolithic=False,
+                              moment_correction=moment_correction)
     work = m.work()
     m.reset()
     t, big = 0.0, 1e300
@@ -129,11 +148,18 @@ def run_serving(count, serial_sample):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--workload", default="transport_1d",
+                    choices=["transport_1d", "precision_box", "transport_2d", "homogeneous_box"],
+                    help="transport_1d is the headline; the others are the precision, largest-"
+                         "solve and homogeneous-box configurations, measured the same way")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 500 for transport_1d, else the workload's own)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
+    ap.add_argument("--no-uncorrected", action="store_true",
+                    help="skip the FP32 / BF16x3 runs without the FP64 moment-rate correction")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=256)
     ap.add_argument("--serving-serial-sample", type=int, default=16)
@@ -159,41 +185,64 @@ def main():
     import numpy as np
     from dtx import workloads
     t0 = time.perf_counter()
-    w = workloads.transport_1d()
-    log(f"workload built in {time.perf_counter() - t0:.1f}s: N={w.ps.n_momentum} "
+    w = getattr(workloads, args.workload)()
+    steps = args.steps if args.steps is not None else (500 if args.workload == "transport_1d"
+                                                        else w.steps)
+    log(f"workload {w.name} built in {time.perf_counter() - t0:.1f}s: N={w.ps.n_momentum} "
         f"C={w.ps.n_space} n={w.ps.n} dt={w.dt:.4e}")
     res = {}
+    runs = []
     for prec in args.precisions.split(","):
-        log(f"[bench] {prec}: {args.warmup} warmup + {args.steps} timed steps")
-        res[prec] = run_gpu(w, prec, args.steps, args.warmup)
-        r = res[prec]
-        log(f"[bench] {prec}: {r['steps_per_s']:.1f} steps/s, binary kernel {r['bin_ms']*1e3:.1f}"
-            f" us, update kernel {r['update_ms']*1e3:.1f} us, {r['items']} work items")
+        runs.append((prec, True))
+        if prec != "fp64" and not args.no_uncorrected:
+            runs.append((prec, False))
+    for prec, corr in runs:
+        key = prec if corr or prec == "fp64" else prec + "_uncorrected"
+        log(f"[bench] {key}: {args.warmup} warmup + {steps} timed steps")
+        res[key] = run_gpu(w, prec, steps, args.warmup, moment_correction=corr)
+        r = res[key]
+        log(f"[bench] {key}: {r['steps_per_s']:.1f} steps/s, binary kernel "
+            f"{r['bin_ms']*1e3:.1f} us, update kernel {r['update_ms']*1e3:.1f} us, "
+            f"{r['items']} work items")
     base = res["fp64"]
     fp = base["f"]
     prec_report = {}
-    for prec, r in res.items():
-        if prec == "fp64":
+    for key, r in res.items():
+        if key == "fp64":
             continue
         mask = np.abs(fp) > 1e-8 * np.max(np.abs(fp))
-        prec_report[prec] = dict(
+        prec_report[key] = dict(
             steps_per_s=round(r["steps_per_s"], 2),
             speedup_vs_fp64=round(r["steps_per_s"] / base["steps_per_s"], 3),
             rel_linf_f=float(np.max(np.abs(r["f"][mask] - fp[mask]) / np.abs(fp[mask]))),
             number_drift=abs(r["number"] - base["number"]) / abs(base["number"]),
             energy_drift=abs(r["energy"] - base["energy"]) / abs(base["energy"]),
             binary_kernel_us=round(r["bin_ms"] * 1e3, 2),
-            binary_tflops=round(r["bin_flops"] / (r["bin_ms"] * 1e-3) / 1e12, 2))
-    achieved = base["bin_flops"] / (base["bin_ms"] * 1e-3) / 1e12
-    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["fp64"],
-                unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["fp64"], 4), traffic=None,
-                kernel="bin_f64_kernel<64>",
-                flops_per_launch=base["bin_flops"], kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
+            binary_tflops=round(r["bin_flops"] / (r["bin_ms"] * 1e-3) / 1e12, 2),
+            binary_hbm_gbs=round(r["bin_bytes"] / (r["bin_ms"] * 1e-3) / 1e9, 1))
+    # The contraction is MFMA-bound when its arithmetic intensity exceeds the FP64 ridge
+    # (78.6 TFLOP/s / 8 TB/s), else HBM-bound (single-cell boxes stream the whole operator).
+    ai = base["bin_flops"] / base["bin_bytes"]
+    kernel = f"bin_f64_kernel<{16 if w.ps.n_space <= 16 else 64}>"
+    if ai >= PEAK_TFLOPS["fp64"] * 1e12 / (PEAK_HBM_GBS * 1e9):
+        achieved = base["bin_flops"] / (base["bin_ms"] * 1e-3) / 1e12
+        roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["fp64"],
+                    unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["fp64"], 4),
+                    traffic=TRAFFIC.get((w.name, kernel)), kernel=kernel,
+                    flops_per_launch=base["bin_flops"], bytes_per_launch=base["bin_bytes"],
+                    kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
+    else:
+        achieved = base["bin_bytes"] / (base["bin_ms"] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM_GBS, 4),
+                    traffic=TRAFFIC.get((w.name, kernel)), kernel=kernel,
+                    flops_per_launch=base["bin_flops"], bytes_per_launch=base["bin_bytes"],
+                    kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
     serving_line = None
-    if not args.no_serving:
+    if not args.no_serving and args.workload == "transport_1d":
         serving_line = run_serving(args.serving_requests, args.serving_serial_sample)
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload in ("transport_1d", "homogeneous_box"):
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle steps")
         cpu = run_cpu_baseline(w, args.cpu_steps)
     line = {
@@ -201,7 +250,7 @@ def main():
         "value": round(base["steps_per_s"], 3),
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": round(base["ms_per_step"], 5),
         "higher_is_better": True,
@@ -209,12 +258,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded lognormal gain tensors, Maxwell-Juttner f0)",
-        "config": {"workload": "transport_1d: 2 species x 256 states (32 p x 8 u x 1 phi), "
-                               "128 periodic x cells, 65536 unknowns, upwind x advection + "
-                               "synchrotron cooling, per-cell binary collisions, plain FP64",
+        "config": {"workload": f"{w.name}: {DESCRIPTIONS[w.name]}",
                    "solve_steps": w.steps,
                    "solves_per_s": round(base["steps_per_s"] / w.steps, 4),
-                   "kernels_per_step": 2},
+                   "kernels_per_step": 3},
         "precision_modes": prec_report,
         "update_kernel_us": round(base["update_ms"] * 1e3, 2),
         "roofline": roof,

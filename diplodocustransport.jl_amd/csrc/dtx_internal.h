@@ -57,6 +57,10 @@ struct PackedChannel {
     int64_t npairs;         // real (i, j) pairs
     int64_t nchunks;        // chunks of KC
     int32_t nktiles;        // ceil(No / TK)
+    int32_t moment = -1;    // moment-rate channel: 0 number, 1 energy (-1: operator channel)
+    bool synth = false;     // generated on the device from (synth_key, synth_scale)
+    uint64_t synth_key = 0;
+    double synth_scale = 0.0;
     void* A = nullptr;      // device: packed A
     uint32_t* chunks = nullptr;  // device chunk table
     int64_t a_bytes = 0;
@@ -126,7 +130,26 @@ enum StatusSlot {
 enum BlockSlot { BP_SUMDF = 0, BP_SUMF, BP_CR, BP_CR_BIN, BP_CR_EMI, BP_CR_FLUX, BP_CR_MOM,
                  BP_CR_SPACE, BP_WORDS };
 
+// FP64 moment-rate correction of a reduced-precision gain term, one workgroup per (cell, output
+// species s).  dBg = sum of the species' gain slots, dBl = sum of its loss slots; R_N and R_E are
+// the exact FP64 number and energy rates of the gain term (sums of the moment-rate slots listed
+// for (s, moment) in mom_off / mom_list).  The corrected binary sum dbsum = dBg (1 + a + b E) +
+// dBl, with (a, b) chosen so that sum dBg' = R_N and sum E dBg' = R_E.
+struct CorrectArgs {
+    const double* part;        // partial slots [slot][cell][N]
+    int64_t part_stride;
+    double* dbsum;             // output [cell][N]
+    const int32_t* species_off;  // (S+1)
+    const int32_t* gslots;     // gain slots per species (S)
+    const int32_t* nslots;     // gain + loss slots per species (S)
+    const double* eweight;     // per-state energy weight (N) or null (number only)
+    const int32_t* mom_off;    // (2S+1) offsets into mom_list for (s, moment)
+    const int32_t* mom_list;   // triples (slot, row begin, row end)
+    int32_t N, C, S;
+};
+
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------
+hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream);
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
 hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,
@@ -149,6 +172,12 @@ hipError_t launch_synth_gain_packed(int precision, void* A, const uint32_t* chun
 hipError_t launch_synth_loss(double* loss, int N, int off1, int N1, int off2, int N2,
                              int No, uint64_t key, double scale, int kind_same,
                              hipStream_t stream);
+// Moment-rate matrices W_N[i,j] = sum_k G'[k,i,j], W_E[i,j] = sum_k E_k G'[k,i,j] of a
+// synthetic gain channel (G' its symmetric pair form), packed as LOSS rows i over the j chunks
+// of species 2.  ew (No weights of the output species) or AE may be null.
+hipError_t launch_synth_moment_packed(double* AN, double* AE, int64_t nchunks, int nktiles,
+                                      int kind, int No, int N1, int N2, const double* ew,
+                                      uint64_t key, double scale, hipStream_t stream);
 
 uint64_t splitmix_key(uint64_t seed, uint64_t stream);
 

@@ -467,6 +467,96 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
                  crs);
 }
 
+// --------------------------------------------------------- FP64 moment-rate correction -----
+// One workgroup per (cell c, output species s).  The reduced-precision gain contraction carries
+// a relative error of its arithmetic in every term; the exact FP64 number and energy rates of
+// the gain term, R_N = sum_ij W_N[i,j] f_i f_j and R_E likewise, come from the moment-rate
+// slots.  The gain sums of the species are rescaled per state by (1 + alpha + beta E_k) with
+// (alpha, beta) such that their number and energy sums equal R_N and R_E (number only when no
+// energy weights are set), and the loss sums are added unchanged.
+__global__ __launch_bounds__(UB) void gain_correct_kernel(CorrectArgs a) {
+    extern __shared__ double gl[];  // gain sums [ns], then loss sums [ns]
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    __shared__ double coef[2];
+    constexpr int LB = 8;
+    const int c = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const int k0 = a.species_off[s], ns = a.species_off[s + 1] - k0;
+    double* g = gl;
+    double* l = gl + ns;
+    const int gs = a.gslots[s], nsl = a.nslots[s];
+    const int64_t base = (int64_t)c * a.N;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int q = tid; q < ns; q += UB) {
+        const int64_t idx = base + k0 + q;
+        double gv = 0.0, lv = 0.0;
+        for (int b0 = 0; b0 < nsl; b0 += LB) {
+            double v[LB];
+#pragma unroll
+            for (int u = 0; u < LB; ++u)
+                v[u] = b0 + u < nsl ? a.part[(int64_t)(b0 + u) * a.part_stride + idx] : 0.0;
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                if (b0 + u < gs) gv = gv + v[u];
+                else if (b0 + u < nsl) lv = lv + v[u];
+            }
+        }
+        g[q] = gv;
+        l[q] = lv;
+        const double e = a.eweight ? a.eweight[k0 + q] : 0.0;
+        s0 = s0 + gv;
+        s1 = s1 + e * gv;
+        s2 = s2 + (e * e) * gv;
+    }
+    double rate[2] = {0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+        for (int t = a.mom_off[2 * s + m]; t < a.mom_off[2 * s + m + 1]; ++t) {
+            const int slot = a.mom_list[3 * t];
+            const int r0 = a.mom_list[3 * t + 1], r1 = a.mom_list[3 * t + 2];
+            const double* p = a.part + (int64_t)slot * a.part_stride + base;
+            for (int r = r0 + tid; r < r1; r += UB) rate[m] = rate[m] + p[r];
+        }
+    wave_put(s0, 0, true, red);
+    wave_put(s1, 1, true, red);
+    wave_put(s2, 2, true, red);
+    wave_put(rate[0], 3, true, red);
+    wave_put(rate[1], 4, true, red);
+    __syncthreads();
+    const double r = tid < 5 ? wave_combine(tid, true, red) : 0.0;
+    __syncthreads();
+    if (tid < 5) red[tid] = r;
+    __syncthreads();
+    if (tid == 0) {
+        const double S0 = red[0], S1 = red[1], S2 = red[2], RN = red[3], RE = red[4];
+        double alpha = 0.0, beta = 0.0;
+        if (S0 != 0.0) {
+            const double dn = RN - S0, de = RE - S1;
+            const double det = S0 * S2 - S1 * S1;
+            if (a.eweight && fabs(det) > 1e-12 * fabs(S0 * S2)) {
+                alpha = (dn * S2 - de * S1) / det;
+                beta = (de * S0 - dn * S1) / det;
+            } else {
+                alpha = dn / S0;
+            }
+            if (!isfinite(alpha) || !isfinite(beta)) alpha = beta = 0.0;
+        }
+        coef[0] = alpha;
+        coef[1] = beta;
+    }
+    __syncthreads();
+    const double alpha = coef[0], beta = coef[1];
+    for (int q = tid; q < ns; q += UB) {
+        const double e = a.eweight ? a.eweight[k0 + q] : 0.0;
+        a.dbsum[base + k0 + q] = g[q] * (1.0 + (alpha + beta * e)) + l[q];
+    }
+}
+
+hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream) {
+    gain_correct_kernel<<<dim3(a.C, a.S), UB, (size_t)2 * max_species_states * sizeof(double),
+                          stream>>>(a);
+    return hipGetLastError();
+}
+
 __global__ void publish_state_kernel(UpdateArgs a) {
     const int64_t total = (int64_t)a.N * a.C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -70,11 +71,22 @@ struct dtx_handle {
     // device: binary operator and its work tables (gain items in the contraction precision,
     // loss items always FP64)
     std::vector<PackedChannel> chans;
+    // FP64 moment-rate correction of a reduced-precision gain term: per gain channel the
+    // matrices W_N / W_E (LOSS layout, rows of species 1 over the states of species 2), their
+    // slots, and the corrected binary sum the update kernels read
+    int32_t moment_correction = 1;  // requested (applies to FP32 / BF16x3)
+    bool correct_active = false;
+    std::vector<PackedChannel> mchans;
+    std::vector<double> eweight_host;
+    double* d_dbsum = nullptr;
+    int32_t *d_ones = nullptr, *d_gslots = nullptr, *d_nslots_s = nullptr, *d_mom_off = nullptr,
+            *d_mom_list = nullptr;
+    int32_t n_mom_list = 0, max_ns = 0;
     double* d_loss = nullptr;
     bool loss_present = false;
     BinWork* d_items = nullptr;     // loss and gain items of one launch
     std::vector<BinWork> items_host;  // host copy (grouped launches)
-    int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0;
+    int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0, part_slots = 0;
     double* d_part = nullptr;
     int64_t part_stride = 0;
     int32_t* d_nslot = nullptr;
@@ -241,8 +253,8 @@ static HostChannel& host_channel(dtx_handle* h, int s_out, int s1, int s2) {
     return h->hch.emplace(key, std::move(ch)).first->second;
 }
 
-static void release_binary_device(dtx_handle* h) {
-    for (auto& c : h->chans) {
+static void release_channels(dtx_handle* h, std::vector<PackedChannel>& list) {
+    for (auto& c : list) {
         if (c.A) {
             (void)hipFree(c.A);
             h->device_bytes -= c.a_bytes;
@@ -256,7 +268,12 @@ static void release_binary_device(dtx_handle* h) {
             if (it != h->allocations.end()) h->allocations.erase(it);
         }
     }
-    h->chans.clear();
+    list.clear();
+}
+
+static void release_binary_device(dtx_handle* h) {
+    release_channels(h, h->chans);
+    release_channels(h, h->mchans);
 }
 
 static int64_t chunk_elems(int precision) {
@@ -351,6 +368,82 @@ static int pack_upload_channel(dtx_handle* h, const HostChannel& hc) {
     return DTX_OK;
 }
 
+// Chunk table of a moment-rate channel: the j chunks of species s2 (LOSS layout, no row part).
+static std::vector<uint32_t> moment_chunk_table(const dtx_handle* h, int s2) {
+    std::vector<uint32_t> t;
+    for (int jb = 0; jb < h->sp_size[s2]; jb += KC)
+        t.push_back((uint32_t)(h->sp_pad_off[s2] + jb) << 16);
+    return t;
+}
+
+static PackedChannel moment_channel(const dtx_handle* h, int s_out, int s1, int s2, int kind,
+                                    int moment) {
+    PackedChannel mc;
+    mc.s_out = s_out;
+    mc.s1 = s1;
+    mc.s2 = s2;
+    mc.kind = CH_LOSS;  // rows i of s1, times f_i at the end
+    mc.precision = DTX_PREC_FP64;
+    mc.No = h->sp_size[s1];
+    mc.N1 = h->sp_size[s1];
+    mc.N2 = h->sp_size[s2];
+    mc.npairs = kind == CH_TRI ? (int64_t)mc.N1 * (mc.N1 + 1) / 2 : (int64_t)mc.N1 * mc.N2;
+    mc.nktiles = (mc.N1 + TK - 1) / TK;
+    mc.moment = moment;
+    return mc;
+}
+
+// Moment-rate matrices of a host channel: W_N[i,j] = sum_k S[k,(i,j)] and
+// W_E[i,j] = sum_k E_k S[k,(i,j)] in FP64 (k ascending), from the channel's FP64 pair form.
+static int build_moment_channels_host(dtx_handle* h, const HostChannel& hc) {
+    const bool energy = !h->eweight_host.empty();
+    const std::vector<uint32_t> table = moment_chunk_table(h, hc.s2);
+    const int64_t nch = (int64_t)table.size();
+    const double* ew = energy ? h->eweight_host.data() + h->sp_off[hc.s_out] : nullptr;
+    for (int m = 0; m < (energy ? 2 : 1); ++m) {
+        PackedChannel mc = moment_channel(h, hc.s_out, hc.s1, hc.s2, hc.kind, m);
+        std::vector<double> buf((size_t)(mc.nktiles * nch * KC * TK), 0.0);
+        for (int i = 0; i < hc.N1; ++i)
+            for (int j = (hc.kind == CH_TRI ? i : 0); j < hc.N2; ++j) {
+                const int64_t p = hc.kind == CH_TRI ? tri_index(i, j, hc.N1)
+                                                    : (int64_t)i * hc.N2 + j;
+                const double* S = hc.S.data() + (size_t)p * hc.No;
+                double w = 0.0;
+                for (int k = 0; k < hc.No; ++k) w = w + (m ? ew[k] * S[k] : S[k]);
+                const int kt = i / TK, r = i % TK, ch = j / KC, t = j % KC;
+                buf[(size_t)(((kt * nch + ch) * KC + t) * TK + r)] = w;
+            }
+        int rc = alloc_channel(h, mc, table);
+        h->mchans.push_back(mc);
+        if (rc) return rc;
+        HIP_CHECK(h, hipMemcpy(mc.A, buf.data(), (size_t)mc.a_bytes, hipMemcpyHostToDevice));
+    }
+    return DTX_OK;
+}
+
+// The same matrices for a channel generated on the device, from its generator key.
+static int build_moment_channels_synth(dtx_handle* h, const PackedChannel& pc) {
+    const bool energy = h->d_eweight != nullptr;
+    const std::vector<uint32_t> table = moment_chunk_table(h, pc.s2);
+    PackedChannel mn = moment_channel(h, pc.s_out, pc.s1, pc.s2, pc.kind, 0);
+    int rc = alloc_channel(h, mn, table);
+    h->mchans.push_back(mn);
+    if (rc) return rc;
+    double* AE = nullptr;
+    if (energy) {
+        PackedChannel me = moment_channel(h, pc.s_out, pc.s1, pc.s2, pc.kind, 1);
+        rc = alloc_channel(h, me, table);
+        h->mchans.push_back(me);
+        if (rc) return rc;
+        AE = (double*)me.A;
+    }
+    HIP_CHECK(h, launch_synth_moment_packed((double*)mn.A, AE, mn.nchunks, mn.nktiles, pc.kind,
+                                            pc.No, pc.N1, pc.N2,
+                                            energy ? h->d_eweight + h->sp_off[pc.s_out] : nullptr,
+                                            pc.synth_key, pc.synth_scale, h->stream));
+    return DTX_OK;
+}
+
 // LOSS channels: one per species, rows of the species against all (padded) states, FP64,
 // packed on the device from the loss matrix d_loss.
 static int pack_loss_channels(dtx_handle* h) {
@@ -380,10 +473,17 @@ static int pack_loss_channels(dtx_handle* h) {
 // Build the grouped work tables for the handle's channels and cell count.
 static int build_work(dtx_handle* h) {
     dfree(h, &h->d_items, h->nitems);
-    dfree(h, &h->d_part, h->part_stride * h->max_slots);
+    dfree(h, &h->d_part, h->part_stride * h->part_slots);
     dfree(h, &h->d_nslot, h->N);
+    dfree(h, &h->d_dbsum, h->n);
+    dfree(h, &h->d_ones, h->N);
+    dfree(h, &h->d_gslots, h->S);
+    dfree(h, &h->d_nslots_s, h->S);
+    dfree(h, &h->d_mom_off, 2 * h->S + 1);
+    dfree(h, &h->d_mom_list, 3 * h->n_mom_list);
     h->nitems = 0;
-    h->max_slots = 0;
+    h->max_slots = h->part_slots = 0;
+    h->correct_active = false;
     const int C = h->C;
     const int ctiles = (C + h->tc - 1) / h->tc;
     double total = 0;
@@ -400,64 +500,82 @@ static int build_work(dtx_handle* h) {
         }
         h->gain_bytes += (double)c.a_bytes;
     }
-    // About four resident workgroups per CU for the 64-cell tiles, two for the 128-cell tiles
-    // (their LDS and registers allow two).
-    const int target_items = h->tc >= 128 ? 512 : 1024;
+    const int target_items = 1024;  // about four resident workgroups per CU
     int64_t range = (int64_t)std::ceil(total / target_items);
     range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
-    // LOSS channels are a small FP64 GEMV next to the gain contraction: short ranges keep their
-    // items from forming the tail of the launch.
+    // LOSS-layout channels (losses, moment rates) are small FP64 GEMVs next to the gain
+    // contraction: short ranges keep their items from forming the tail of the launch.
     const int64_t loss_range = std::min<int64_t>(range, 4);
-    std::vector<int> next_slot(h->S, 0);
+    std::vector<int> next_slot(h->S, 0), gain_slots(h->S, 0);
     h->part_stride = (int64_t)C * h->N;
     // A unit is (channel, chunk range, k tile); it has one item per cell tile.  Gain slots of a
     // species come first, then its loss slots, and the update kernel adds them in slot order.
-    struct Unit { BinWork w; int slot; bool loss; };
-    std::vector<Unit> gain_units, loss_units;
+    // Moment-rate slots follow all of them.
+    struct Unit { BinWork w; int slot; };
+    std::vector<Unit> gain_units, loss_units, moment_units;
     const bool bf16 = h->precision == DTX_PREC_BF16X3;
-    for (int pass = 0; pass < 2; ++pass)
-        for (auto& c : h->chans) {
-            const bool loss = c.kind == CH_LOSS;
-            if (loss != (pass == 1)) continue;
-            const int prec = loss ? DTX_PREC_FP64 : h->precision;
-            const int64_t rg = loss ? loss_range : range;
-            for (int64_t c0 = 0; c0 < c.nchunks; c0 += rg) {
-                const int64_t c1 = std::min<int64_t>(c0 + rg, c.nchunks);
-                const int slot = next_slot[c.s_out]++;
-                for (int kt = 0; kt < c.nktiles; ++kt) {
-                    BinWork w{};
-                    w.A = c.A;
-                    w.chunks = c.chunks;
-                    w.fT = h->d_fT;
-                    if (prec == DTX_PREC_FP64) w.B = h->d_fT;
-                    else w.B = h->d_fhi;
-                    w.b_lo = (bf16 && !loss) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
-                    w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
-                    w.ktile = kt;
-                    w.c_begin = (int32_t)c0;
-                    w.c_end = (int32_t)c1;
-                    w.row_base = h->sp_off[c.s_out];
-                    w.row_pad_base = h->sp_pad_off[c.s_out];
-                    w.rows = c.No;
-                    w.cells = C;
-                    w.ldc = h->ldc;
-                    w.ldb = h->Np;
-                    w.N = h->N;
-                    w.kind = c.kind;
-                    (loss ? loss_units : gain_units).push_back({w, slot, loss});
-                }
+    auto add_units = [&](const PackedChannel& c, int64_t rg, std::vector<Unit>& out,
+                         const std::function<int()>& next) {
+        const bool loss = c.kind == CH_LOSS;
+        const int prec = loss ? DTX_PREC_FP64 : h->precision;
+        const int s_row = c.moment >= 0 ? c.s1 : c.s_out;  // owner of the output rows
+        for (int64_t c0 = 0; c0 < c.nchunks; c0 += rg) {
+            const int64_t c1 = std::min<int64_t>(c0 + rg, c.nchunks);
+            const int slot = next();
+            for (int kt = 0; kt < c.nktiles; ++kt) {
+                BinWork w{};
+                w.A = c.A;
+                w.chunks = c.chunks;
+                w.fT = h->d_fT;
+                if (prec == DTX_PREC_FP64) w.B = h->d_fT;
+                else w.B = h->d_fhi;
+                w.b_lo = (bf16 && !loss) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
+                w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
+                w.ktile = kt;
+                w.c_begin = (int32_t)c0;
+                w.c_end = (int32_t)c1;
+                w.row_base = h->sp_off[s_row];
+                w.row_pad_base = h->sp_pad_off[s_row];
+                w.rows = c.No;
+                w.cells = C;
+                w.ldc = h->ldc;
+                w.ldb = h->Np;
+                w.N = h->N;
+                w.kind = c.kind;
+                out.push_back({w, slot});
             }
         }
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        for (auto& c : h->chans)
+            if ((c.kind == CH_LOSS) == (pass == 1))
+                add_units(c, pass ? loss_range : range, pass ? loss_units : gain_units,
+                          [&]() { return next_slot[c.s_out]++; });
+        if (pass == 0) gain_slots = next_slot;  // gain slots of species s: [0, gain_slots[s])
+    }
     for (int s = 0; s < h->S; ++s) h->max_slots = std::max(h->max_slots, next_slot[s]);
-    if (h->max_slots > 0) {
-        int rc = dalloc(h, &h->d_part, h->part_stride * h->max_slots);
+    // moment-rate slots: one per (moment channel, chunk range), listed per (species, moment)
+    int n_moment = 0;
+    std::vector<std::vector<int32_t>> mom(2 * h->S);
+    for (auto& c : h->mchans)
+        add_units(c, loss_range, moment_units, [&]() {
+            const int slot = h->max_slots + n_moment++;
+            auto& l = mom[2 * c.s_out + c.moment];
+            l.push_back(slot);
+            l.push_back(h->sp_off[c.s1]);
+            l.push_back(h->sp_off[c.s1] + c.N1);
+            return slot;
+        });
+    h->part_slots = h->max_slots + n_moment;
+    if (h->part_slots > 0) {
+        int rc = dalloc(h, &h->d_part, h->part_stride * h->part_slots);
         if (rc) return rc;
     }
-    // Launch order: gain units, then the short loss units; the cell tiles of a unit (which read
-    // the same operator block) are adjacent.
+    // Launch order: gain units, then the short LOSS-layout units; the cell tiles of a unit
+    // (which read the same operator block) are adjacent.
     std::vector<BinWork> items;
-    items.reserve((gain_units.size() + loss_units.size()) * ctiles);
-    for (const auto* list : {&gain_units, &loss_units})
+    items.reserve((gain_units.size() + loss_units.size() + moment_units.size()) * ctiles);
+    for (const auto* list : {&gain_units, &loss_units, &moment_units})
         for (const Unit& u : *list)
             for (int ct = 0; ct < ctiles; ++ct) {
                 BinWork w = u.w;
@@ -476,12 +594,39 @@ static int build_work(dtx_handle* h) {
     for (int s = 0; s < h->S; ++s)
         for (int k = 0; k < h->sp_size[s]; ++k) nslot[h->sp_off[s] + k] = next_slot[s];
     int rc = dalloc(h, &h->d_nslot, h->N);
+    if (!rc) rc = upload(h, h->d_nslot, nslot.data(), h->N);
     if (rc) return rc;
-    return upload(h, h->d_nslot, nslot.data(), h->N);
+    if (h->mchans.empty()) return DTX_OK;
+    // correction tables
+    std::vector<int32_t> ones(h->N, 1), moff(2 * h->S + 1, 0), mlist;
+    for (int q = 0; q < 2 * h->S; ++q) {
+        moff[q] = (int32_t)(mlist.size() / 3);
+        mlist.insert(mlist.end(), mom[q].begin(), mom[q].end());
+    }
+    moff[2 * h->S] = (int32_t)(mlist.size() / 3);
+    h->n_mom_list = (int32_t)(mlist.size() / 3);
+    h->max_ns = 0;
+    for (int s = 0; s < h->S; ++s) h->max_ns = std::max(h->max_ns, h->sp_size[s]);
+    std::vector<int32_t> nsl(next_slot.begin(), next_slot.end());
+    rc = dalloc(h, &h->d_dbsum, h->n);
+    if (!rc) rc = dalloc(h, &h->d_ones, h->N);
+    if (!rc) rc = dalloc(h, &h->d_gslots, h->S);
+    if (!rc) rc = dalloc(h, &h->d_nslots_s, h->S);
+    if (!rc) rc = dalloc(h, &h->d_mom_off, 2 * h->S + 1);
+    if (!rc) rc = dalloc(h, &h->d_mom_list, 3 * h->n_mom_list);
+    if (!rc) rc = upload(h, h->d_ones, ones.data(), h->N);
+    if (!rc) rc = upload(h, h->d_gslots, gain_slots.data(), h->S);
+    if (!rc) rc = upload(h, h->d_nslots_s, nsl.data(), h->S);
+    if (!rc) rc = upload(h, h->d_mom_off, moff.data(), 2 * h->S + 1);
+    if (!rc) rc = upload(h, h->d_mom_list, mlist.data(), 3 * h->n_mom_list);
+    if (rc) return rc;
+    h->correct_active = true;
+    return DTX_OK;
 }
 
 static int finalize_binary(dtx_handle* h) {
     if (!h->binary_dirty) return DTX_OK;
+    const bool want_correction = h->moment_correction && h->precision != DTX_PREC_FP64;
     if (!h->synth_binary) {
         release_binary_device(h);
         for (auto& kv : h->hch) {
@@ -491,6 +636,7 @@ static int finalize_binary(dtx_handle* h) {
                 if (v != 0.0) { any = true; break; }
             if (!any) continue;
             int rc = pack_upload_channel(h, hc);
+            if (!rc && want_correction) rc = build_moment_channels_host(h, hc);
             if (rc) return rc;
         }
         h->hch.clear();
@@ -503,6 +649,15 @@ static int finalize_binary(dtx_handle* h) {
         }
         h->loss_host.clear();
         h->loss_host.shrink_to_fit();
+    }
+    if (h->synth_binary) {
+        release_channels(h, h->mchans);
+        if (want_correction)
+            for (const PackedChannel& pc : h->chans)
+                if (pc.synth) {
+                    int rc = build_moment_channels_synth(h, pc);
+                    if (rc) return rc;
+                }
     }
     int rc = pack_loss_channels(h);
     if (rc) return rc;
@@ -622,8 +777,13 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.inv_ap = h->d_invap;
     a.df_inj = h->d_dfinj;
     a.vol = h->d_vol;
-    a.part = h->d_part;
-    a.nslot = h->d_nslot;
+    if (h->correct_active) {  // one corrected binary sum per state
+        a.part = h->d_dbsum;
+        a.nslot = h->d_ones;
+    } else {
+        a.part = h->d_part;
+        a.nslot = h->d_nslot;
+    }
     a.part_stride = h->part_stride;
     a.emi = h->d_emi;
     a.active = h->d_active;
@@ -686,6 +846,22 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
         int rc = timed_begin(h, 0, &ev);
         if (rc) return rc;
         HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
+        if (h->correct_active) {
+            CorrectArgs ca{};
+            ca.part = h->d_part;
+            ca.part_stride = h->part_stride;
+            ca.dbsum = h->d_dbsum;
+            ca.species_off = h->d_species_off;
+            ca.gslots = h->d_gslots;
+            ca.nslots = h->d_nslots_s;
+            ca.eweight = h->d_eweight;
+            ca.mom_off = h->d_mom_off;
+            ca.mom_list = h->d_mom_list;
+            ca.N = h->N;
+            ca.C = h->C;
+            ca.S = h->S;
+            HIP_CHECK(h, launch_gain_correct(ca, h->max_ns, h->stream));
+        }
         rc = timed_end(h, &ev);
         if (rc) return rc;
     }
@@ -850,6 +1026,10 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
         if (m == 0) g->precision = h->precision;
         if (h->precision != g->precision) {
             g->err = "grouped members must share one contraction precision";
+            return DTX_ERR_UNSUPPORTED;
+        }
+        if (h->moment_correction && h->precision != DTX_PREC_FP64) {
+            g->err = "grouped members run without the moment-rate correction";
             return DTX_ERR_UNSUPPORTED;
         }
         rc = ensure_ready(h);
@@ -1301,7 +1481,17 @@ int dtx_set_moment_weights(dtx_handle* h, const double* w) {
         int rc = dalloc(h, &h->d_eweight, h->N);
         if (rc) return rc;
     }
+    h->eweight_host.assign(w, w + h->N);
     return upload(h, h->d_eweight, w, h->N);
+}
+
+int dtx_set_moment_correction(dtx_handle* h, int32_t enable) {
+    if (!h) return DTX_ERR_INVALID;
+    if (h->binary && !h->binary_dirty)
+        return fail(h, DTX_ERR_STATE,
+                    "the moment-rate correction must be chosen before the first step");
+    h->moment_correction = enable ? 1 : 0;
+    return DTX_OK;
 }
 
 int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
@@ -1333,6 +1523,9 @@ int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
                 pc.N2 = h->sp_size[b];
                 pc.npairs = (a == b) ? (int64_t)pc.N1 * (pc.N1 + 1) / 2 : (int64_t)pc.N1 * pc.N2;
                 pc.nktiles = (pc.No + TK - 1) / TK;
+                pc.synth = true;
+                pc.synth_key = key;
+                pc.synth_scale = scale;
                 rc = alloc_channel(h, pc, chunk_table(h, pc.kind, a, b));
                 h->chans.push_back(pc);
                 if (rc) return rc;

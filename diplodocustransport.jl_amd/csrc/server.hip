@@ -274,6 +274,13 @@ static int build_request(dtx_server* sv, const dtx_request& rq, dtx_handle** out
         sv->err = dtx_last_error(nullptr);
         return rc;
     }
+    // grouped steps add the contraction's slots directly (no per-request correction launch)
+    rc = dtx_set_moment_correction(h, 0);
+    if (rc) {
+        sv->err = dtx_last_error(h);
+        dtx_destroy(h);
+        return rc;
+    }
     const int S = c.num_species;
     std::vector<SpeciesGrid> sg;
     std::vector<int> off;

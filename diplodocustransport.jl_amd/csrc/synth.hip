@@ -89,6 +89,48 @@ __global__ void synth_loss_kernel(double* D, int N, int o1, int N1, int o2, int 
     }
 }
 
+// One thread per packed element (k-tile of rows i, chunk of j, offset t, row r) of the
+// moment-rate matrices: the k-sums of the channel's pair-form gain, plain and energy-weighted.
+__global__ void synth_moment_kernel(double* AN, double* AE, int64_t nchunks, int nktiles,
+                                    int kind, int No, int N1, int N2, const double* ew,
+                                    uint64_t key, double scale) {
+    const int64_t total = (int64_t)nktiles * nchunks * KC * TK;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % TK);
+        const int t = (int)((e / TK) % KC);
+        const int64_t ch = (e / ((int64_t)TK * KC)) % nchunks;
+        const int kt = (int)(e / ((int64_t)TK * KC * nchunks));
+        const int i = kt * TK + r;
+        const int j = (int)(ch * KC) + t;
+        const bool valid = i < N1 && j < N2 && (kind != CH_TRI || j >= i);
+        double wn = 0.0, we = 0.0;
+        if (valid) {
+            for (int k = 0; k < No; ++k) {
+                double v = scale * d_lognormal(key, (uint64_t)k + (uint64_t)No *
+                                                         ((uint64_t)i + (uint64_t)N1 * j));
+                if (kind == CH_TRI && i != j)
+                    v += scale * d_lognormal(key, (uint64_t)k + (uint64_t)No *
+                                                      ((uint64_t)j + (uint64_t)N1 * i));
+                wn = wn + v;
+                if (ew) we = we + ew[k] * v;
+            }
+        }
+        AN[e] = wn;
+        if (AE) AE[e] = we;
+    }
+}
+
+hipError_t launch_synth_moment_packed(double* AN, double* AE, int64_t nchunks, int nktiles,
+                                      int kind, int No, int N1, int N2, const double* ew,
+                                      uint64_t key, double scale, hipStream_t stream) {
+    const int64_t total = (int64_t)nktiles * nchunks * KC * TK;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 16384);
+    synth_moment_kernel<<<blocks, 256, 0, stream>>>(AN, AE, nchunks, nktiles, kind, No, N1, N2,
+                                                    ew, key, scale);
+    return hipGetLastError();
+}
+
 hipError_t launch_synth_gain_packed(int precision, void* A, const uint32_t* chunks,
                                     int64_t nchunks, int No, int nktiles, int kind, int N1,
                                     int N2, int pad1, int pad2, uint64_t key, double scale,

@@ -84,6 +84,7 @@ SIGNATURES = [
     ("dtx_set_emission_blocks", I32, [C.c_void_p, PD]),
     ("dtx_set_vectors", I32, [C.c_void_p, PD, PD, PD, PD]),
     ("dtx_set_moment_weights", I32, [C.c_void_p, PD]),
+    ("dtx_set_moment_correction", I32, [C.c_void_p, I32]),
     ("dtx_synth_binary", I32, [C.c_void_p, U64, D]),
     ("dtx_step", I32, [C.c_void_p, D, D, D, I32, PD, PI32, C.POINTER(StepStats)]),
     ("dtx_solve", I32, [C.c_void_p, D, I64, PD, I32, PD, PD, PI64]),

@@ -97,7 +97,7 @@ class _SteppingStruct:
     def __init__(self, PhaseSpace: PhaseSpace, Initial, Injection, BinM: BinaryMatricesStruct,
                  EmiM: EmissionMatricesStruct, FluxM: FluxMatricesStruct,
                  Adaptive: bool = False, Backend: BackendType = None,
-                 Precision: str = "fp64"):
+                 Precision: str = "fp64", MomentCorrection: bool = True):
         if Backend is None:
             Backend = HIPBackend()
         if not isinstance(Backend, HIPBackend):
@@ -181,6 +181,9 @@ class _SteppingStruct:
         w = _f64(PhaseSpace.energy_weights())
         _lib.check(self.lib.dtx_set_moment_weights(h, _lib.dptr(w)), h,
                    "dtx_set_moment_weights")
+        # FP32 / BF16x3: FP64 number and energy rates of the gain term (no effect in FP64)
+        _lib.check(self.lib.dtx_set_moment_correction(h, int(bool(MomentCorrection))), h,
+                   "dtx_set_moment_correction")
 
     @staticmethod
     def _flux_sum(n, FluxM):

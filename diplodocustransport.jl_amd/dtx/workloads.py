@@ -211,7 +211,8 @@ def emission_dense(w: Workload) -> Optional[np.ndarray]:
     return M
 
 
-def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=None):
+def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=None,
+                moment_correction=True):
     """ForwardEulerStruct / ForwardSymplecticEulerStruct on the HIP backend for a workload."""
     from .api import (BinaryMatricesStruct, EmissionMatricesStruct, FluxMatricesStruct,
                       ForwardEulerStruct, ForwardSymplecticEulerStruct)
@@ -228,4 +229,5 @@ def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=No
                                   else [], blocks=w.emission_blocks)
     FluxM = FluxMatricesStruct(w.Ap, w.X, w.P, w.Vol)
     cls = ForwardEulerStruct if method == "euler" else ForwardSymplecticEulerStruct
-    return cls(w.ps, w.f0, w.inj, BinM, EmiM, FluxM, Precision=precision)
+    return cls(w.ps, w.f0, w.inj, BinM, EmiM, FluxM, Precision=precision,
+               MomentCorrection=moment_correction)

@@ -148,6 +148,15 @@ int dtx_set_vectors(dtx_handle* h, const double* Ap, const double* Vol, const do
  * them the energy moment is reported as 0. */
 int dtx_set_moment_weights(dtx_handle* h, const double* energy_weight);
 
+/* FP64 moment-rate correction of the FP32 / BF16x3 gain contraction (default: on).  Per gain
+ * channel the library keeps W_N[i,j] = sum_k G[k,i,j] and W_E[i,j] = sum_k E_k G[k,i,j] in FP64
+ * (E = the moment weights, when set before the first step) and evaluates the exact number and
+ * energy rates f^T W f of the gain term every step; the reduced-precision gain sums of each
+ * (cell, species) are rescaled per state by (1 + a + b E_k) so that their number and energy
+ * sums equal those rates.  No counterpart in the reference, which has no reduced-precision
+ * path; FP64 handles ignore it.  Must be called before the first step. */
+int dtx_set_moment_correction(dtx_handle* h, int32_t enable);
+
 /* Synthetic workload support: generate lognormal(0,1)*scale gain channels for every species
  * pair (a <= b) -> (a, b) directly in device memory, with number-conserving losses
  * (sum_k G[k,i,j] = L[i,j]).  No counterpart in the reference, which reads DiplodocusCollisions

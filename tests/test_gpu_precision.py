@@ -1,0 +1,84 @@
+"""GPU: reduced-precision contraction (FP32 MFMA, split-BF16 x3) against the FP64 GPU path,
+which the parity tests pin to the FP64 oracle.
+
+Bars (over 60 steps of small cases):
+* relative L-inf error of f against FP64 (over entries above 1e-8 of the maximum):
+  FP32 <= 1e-6, BF16x3 <= 1e-4;
+* with the FP64 moment-rate correction, the number of every (cell, species) follows the FP64
+  run to <= 1e-12 relative (the synthetic collisions conserve number exactly), and the energy
+  error stays within twice the uncorrected one (the energy rate is exact for the current f, so
+  what remains comes from the f error).
+"""
+import numpy as np
+import pytest
+
+from dtx import workloads
+
+pytestmark = pytest.mark.gpu
+STEPS = 60
+F_TOL = {"fp32": 1e-6, "bf16x3": 1e-4}
+
+
+def run(w, precision, correction=True, steps=STEPS):
+    m = workloads.make_struct(w, precision=precision, use_monolithic=False,
+                              moment_correction=correction)
+    t = 0.0
+    for _ in range(steps):
+        dt, _ = m(t, 1e300, w.dt, 0)
+        t += dt
+    m.synchronize()
+    f = m.f
+    num, en = m.moments()
+    m.close()
+    return f, num, en
+
+
+def rel_err_f(f, ref):
+    mask = np.abs(ref) > 1e-8 * np.max(np.abs(ref))
+    return float(np.max(np.abs(f[mask] - ref[mask]) / np.abs(ref[mask])))
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def homogeneous_channels():
+    """Channel-form tiny case without flux: collisions alone, number conserved per cell."""
+    return workloads.tiny(p_num=8, u_num=4, x_num=3, with_flux=False, monolithic=False)
+
+
+def synthetic_small():
+    """Device-generated channels (3 species), the precision configuration at small size."""
+    return workloads.precision_box(p_num=8, u_num=4, h_num=2, num_species=3, steps=STEPS)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("case", ["host_channels", "device_channels"])
+def test_moment_rate_correction(precision, case):
+    w = homogeneous_channels() if case == "host_channels" else synthetic_small()
+    print(f"[case] {case} {precision} N={w.ps.n_momentum} C={w.ps.n_space}", flush=True)
+    f64, n64, e64 = run(w, "fp64")
+    fc, nc, ec = run(w, precision, correction=True)
+    fu, nu, eu = run(w, precision, correction=False)
+    assert rel_err_f(fc, f64) <= F_TOL[precision]
+    assert rel_err_f(fu, f64) <= F_TOL[precision]
+    assert rel_err(nc, n64) <= 1e-12, (rel_err(nc, n64), rel_err(nu, n64))
+    assert rel_err(nu, n64) > rel_err(nc, n64)
+    assert rel_err(ec, e64) <= 2.0 * rel_err(eu, e64) + 1e-12
+
+
+def test_correction_off_in_fp64_is_a_no_op():
+    w = homogeneous_channels()
+    a = run(w, "fp64", correction=True, steps=10)
+    b = run(w, "fp64", correction=False, steps=10)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_correction_must_precede_first_step():
+    w = homogeneous_channels()
+    m = workloads.make_struct(w, precision="fp32", use_monolithic=False)
+    m(0.0, 1e300, w.dt, 0)
+    rc = m.lib.dtx_set_moment_correction(m._h, 0)
+    assert rc == -4  # DTX_ERR_STATE
+    m.close()
