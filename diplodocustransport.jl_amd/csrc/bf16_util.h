@@ -32,4 +32,13 @@ DTX_HD inline void split_bf16(double x, uint16_t* hi, uint16_t* lo) {
     *lo = bf16_rne((float)r);
 }
 
+// Position of operator element (row r, chunk offset t) inside one packed split-BF16 chunk half
+// (64 rows x 32 j): the order in which the 16x16x32 MFMA consumes its A operand.  Rows come in
+// blocks of 16; inside a block, lane l = (r mod 16) + 16 (t div 8) holds the 8 consecutive j of
+// its row at 16 bytes.  Staging copies a chunk to LDS in this order and every wave reads its
+// fragments as one linear 1 KB block per instruction, free of LDS bank conflicts.
+DTX_HD inline int bf16_frag_index(int r, int t) {
+    return (((r >> 4) * 64 + (r & 15) + 16 * (t >> 3)) << 3) + (t & 7);
+}
+
 }  // namespace dtx

@@ -20,7 +20,7 @@ namespace dtx {
 constexpr int TK = 64;          // output rows (k) per work item
 constexpr int KC = 32;          // j per chunk (one LDS stage)
 constexpr int SPECIES_ALIGN = 32;  // species blocks of the B operand start on this multiple
-constexpr int MAX_RANGE = 256;     // most chunks one work item covers (its table slice lives in LDS)
+constexpr int MAX_RANGE = 1024;    // most chunks one work item covers (its table slice lives in LDS)
 
 // Channel kinds: TRI = one species with itself, pairs i <= j; RECT = s1 < s2, all (i, j);
 // LOSS = rows of one species against all states, multiplied by the row's own f at the end.

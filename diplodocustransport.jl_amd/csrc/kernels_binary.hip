@@ -26,6 +26,7 @@ namespace dtx {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte load / LDS store
 
 template <int TC>
 struct Tiling {
@@ -37,24 +38,31 @@ struct Tiling {
     static constexpr int MC = WCOLS / 16;
 };
 
+// The work table's pointers are generic to the compiler, which would emit flat memory
+// instructions for them.  Flat loads count against the LDS counter as well, so every wait for an
+// LDS read would also wait for the operator loads prefetched for the next stage, and their
+// latency would never overlap the MFMAs.  All global accesses of the contraction kernels
+// therefore go through global-address-space pointers.
+#define GLB __attribute__((address_space(1)))
+
 struct ItemView {
-    const char* A;
-    const uint32_t* chunks;
-    const double* fT;
-    const void* B;
+    const GLB char* A;
+    const GLB uint32_t* chunks;
+    const GLB double* fT;
+    const GLB void* B;
     int64_t b_lo;
-    double* part;
+    GLB double* part;
     int c_begin, c_end, krow0, row_base, row_pad_base, rows, cells, ldc, ldb, N, c0, kind;
 };
 
 __device__ inline ItemView load_item(const BinWork& w, int tc, int elem_bytes) {
     ItemView v;
-    v.A = (const char*)w.A + (int64_t)w.ktile * w.a_ktile_stride * elem_bytes;
-    v.chunks = w.chunks;
-    v.fT = w.fT;
-    v.B = w.B;
+    v.A = (const GLB char*)w.A + (int64_t)w.ktile * w.a_ktile_stride * elem_bytes;
+    v.chunks = (const GLB uint32_t*)w.chunks;
+    v.fT = (const GLB double*)w.fT;
+    v.B = (const GLB void*)w.B;
     v.b_lo = w.b_lo;
-    v.part = w.part;
+    v.part = (GLB double*)w.part;
     v.c_begin = w.c_begin;
     v.c_end = w.c_end;
     v.krow0 = w.ktile * TK;
@@ -82,7 +90,7 @@ __device__ inline void store_tile(const ItemView& v, int row16, int col16, int l
                                   const double* t4) {
     const int cell = v.c0 + col16 + (lane & 15);
     if (cell >= v.cells) return;
-    double* dst = v.part + (int64_t)cell * v.N + v.row_base;
+    GLB double* dst = v.part + (int64_t)cell * v.N + v.row_base;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int krow = v.krow0 + tile_row<ROWSTEP>(row16, lane, e);
@@ -133,8 +141,8 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     const ItemView v = load_item(w, TC, 8);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
-    const double* A = (const double*)v.A + (int64_t)v.c_begin * KC * TK;
-    const double* B = (const double*)v.B;
+    const GLB double* A = (const GLB double*)v.A + (int64_t)v.c_begin * KC * TK;
+    const GLB double* B = (const GLB double*)v.B;
     const int bq = tid / BT, bcc = (tid % BT) * BP;
     const bool loss = v.kind == CH_LOSS;
     load_ctab(v, ctab);
@@ -144,15 +152,15 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     auto load_stage = [&](int st) {
         const int ch = st >> 1, h = st & 1;
         const uint32_t ce = ctab[ch];
-        const double* src = A + (int64_t)ch * KC * TK + h * HK * TK + tid * AP;
+        const GLB double* src = A + (int64_t)ch * KC * TK + h * HK * TK + tid * AP;
 #pragma unroll
         for (int q = 0; q < AP; ++q) ra[q] = src[q];
         const int j0 = (int)(ce >> 16) + h * HK;
-        const double* b = B + (int64_t)(j0 + bq) * v.ldc + v.c0 + bcc;
+        const GLB double* b = B + (int64_t)(j0 + bq) * v.ldc + v.c0 + bcc;
 #pragma unroll
         for (int q = 0; q < BP; ++q) rb[q] = b[q];
         if (!loss) {
-            const double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
+            const GLB double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
 #pragma unroll
             for (int q = 0; q < BP; ++q) rf[q] = fi[q];
         }
@@ -239,8 +247,8 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
     const ItemView v = load_item(w, TC, 4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
-    const float* A = (const float*)v.A + (int64_t)v.c_begin * KC * TK;
-    const double* B = v.fT;
+    const GLB float* A = (const GLB float*)v.A + (int64_t)v.c_begin * KC * TK;
+    const GLB double* B = v.fT;
     const int bq = tid / BT, bcc = (tid % BT) * BP;
     load_ctab(v, ctab);
 
@@ -248,11 +256,11 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
     double rb[BP], rf[BP];
     auto load_chunk = [&](int ch) {
         const uint32_t ce = ctab[ch];
-        const float* src = A + (int64_t)ch * KC * TK + tid * AP;
+        const GLB float* src = A + (int64_t)ch * KC * TK + tid * AP;
 #pragma unroll
         for (int q = 0; q < AP; ++q) ra[q] = src[q];
-        const double* b = B + (int64_t)((int)(ce >> 16) + bq) * v.ldc + v.c0 + bcc;
-        const double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
+        const GLB double* b = B + (int64_t)((int)(ce >> 16) + bq) * v.ldc + v.c0 + bcc;
+        const GLB double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
 #pragma unroll
         for (int q = 0; q < BP; ++q) {
             rb[q] = b[q];
@@ -338,16 +346,15 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
 // ------------------------------------------------------------------------ split BF16 x3 -----
 // B is the cell-major hi/lo split of f (no per-element work while staging); the row multiplier
 // f_i is staged in LDS beside each chunk and folded into FP64 when the row ends.  Packed A per
-// k-tile and chunk: hi[TK][32] then lo[TK][32] (bf16 bits).  LDS rows are padded to 40 elements
-// so the 16-byte fragment reads of 16 rows hit distinct banks.
-constexpr int BPITCH = 40;
-
+// k-tile and chunk: hi then lo, each 64 rows x 32 j in MFMA fragment order (bf16_frag_index).
+// The B tile is staged in the same order (16 cells x 8 j per lane quad), so LDS is written and
+// read as whole linear 1 KB blocks: no bank conflicts, and no padding.
 template <int TC>
 struct SmemBF16 {
-    __attribute__((aligned(16))) uint16_t Ah[2][TK * BPITCH];
-    __attribute__((aligned(16))) uint16_t Al[2][TK * BPITCH];
-    __attribute__((aligned(16))) uint16_t Bh[2][TC * BPITCH];
-    __attribute__((aligned(16))) uint16_t Bl[2][TC * BPITCH];
+    u32x4 Ah[2][TK * KC / 8];   // 16-byte lane quads: [row block][lane]
+    u32x4 Al[2][TK * KC / 8];
+    u32x4 Bh[2][TC * KC / 8];   // [cell block][lane]
+    u32x4 Bl[2][TC * KC / 8];
     double Fs[2][TC];
     uint32_t ctab[MAX_RANGE];
 };
@@ -365,34 +372,35 @@ __device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) 
     const ItemView v = load_item(w, TC, 2);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
-    const uint16_t* A = (const uint16_t*)v.A + (int64_t)v.c_begin * TK * KC * 2;
-    const uint16_t* Bhi = (const uint16_t*)v.B;
-    const uint16_t* Blo = Bhi + v.b_lo;
+    const GLB uint16_t* A = (const GLB uint16_t*)v.A + (int64_t)v.c_begin * TK * KC * 2;
+    const GLB uint16_t* Bhi = (const GLB uint16_t*)v.B;
+    const GLB uint16_t* Blo = Bhi + v.b_lo;
+    // B quad of this thread: cell block tid / 64, lane l = tid % 64 -> cell 16 (tid / 64) +
+    // l % 16 and j offset 8 (l / 16); lanes of one cell read one contiguous 64-byte segment.
     const bool bthread = tid < TC * 4;
-    const int bcc = tid >> 2, bk8 = (tid & 3) * 8;
-    const int arow = tid >> 2, ak8 = (tid & 3) * 8;
+    const int bcell = 16 * (tid >> 6) + (tid & 15), bk8 = 8 * ((tid >> 4) & 3);
     load_ctab(v, ctab);
 
-    uint4 rah, ral, rbh, rbl;
+    u32x4 rah, ral, rbh, rbl;
     double rfi = 0.0;
     auto load_chunk = [&](int ch) {
         const uint32_t ce = ctab[ch];
-        const uint4* src = (const uint4*)(A + (int64_t)ch * TK * KC * 2);
+        const GLB u32x4* src = (const GLB u32x4*)(A + (int64_t)ch * TK * KC * 2);
         rah = src[tid];
         ral = src[256 + tid];
         if (bthread) {
-            const int64_t off = (int64_t)(v.c0 + bcc) * v.ldb + (int)(ce >> 16) + bk8;
-            rbh = *(const uint4*)(Bhi + off);
-            rbl = *(const uint4*)(Blo + off);
+            const int64_t off = (int64_t)(v.c0 + bcell) * v.ldb + (int)(ce >> 16) + bk8;
+            rbh = *(const GLB u32x4*)(Bhi + off);
+            rbl = *(const GLB u32x4*)(Blo + off);
         }
         if (tid < TC) rfi = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + tid];
     };
     auto store_chunk = [&](int buf) {
-        *(uint4*)&Ah[buf][arow * BPITCH + ak8] = rah;
-        *(uint4*)&Al[buf][arow * BPITCH + ak8] = ral;
+        Ah[buf][tid] = rah;
+        Al[buf][tid] = ral;
         if (bthread) {
-            *(uint4*)&Bh[buf][bcc * BPITCH + bk8] = rbh;
-            *(uint4*)&Bl[buf][bcc * BPITCH + bk8] = rbl;
+            Bh[buf][tid] = rbh;
+            Bl[buf][tid] = rbl;
         }
         if (tid < TC) Fs[buf][tid] = rfi;
     };
@@ -414,22 +422,21 @@ __device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) 
     }
     __syncthreads();
     int buf = 0;
-    const int kq = 8 * (lane >> 4);
     for (int ch = 0; ch < nch; ++ch) {
         const bool more = ch + 1 < nch;
         if (more) load_chunk(ch + 1);
         bf16x8 ah[T::MR], al[T::MR], bh[T::MC], bl[T::MC];
 #pragma unroll
         for (int r = 0; r < T::MR; ++r) {
-            const int row = wk * T::WROWS + r * 16 + (lane & 15);
-            ah[r] = *(const bf16x8*)&Ah[buf][row * BPITCH + kq];
-            al[r] = *(const bf16x8*)&Al[buf][row * BPITCH + kq];
+            const int q = (wk * T::MR + r) * 64 + lane;  // row block wk*MR + r
+            ah[r] = __builtin_bit_cast(bf16x8, Ah[buf][q]);
+            al[r] = __builtin_bit_cast(bf16x8, Al[buf][q]);
         }
 #pragma unroll
         for (int c = 0; c < T::MC; ++c) {
-            const int col = wc * T::WCOLS + c * 16 + (lane & 15);
-            bh[c] = *(const bf16x8*)&Bh[buf][col * BPITCH + kq];
-            bl[c] = *(const bf16x8*)&Bl[buf][col * BPITCH + kq];
+            const int q = (wc * T::MC + c) * 64 + lane;  // cell block wc*MC + c
+            bh[c] = __builtin_bit_cast(bf16x8, Bh[buf][q]);
+            bl[c] = __builtin_bit_cast(bf16x8, Bl[buf][q]);
         }
 #pragma unroll
         for (int r = 0; r < T::MR; ++r)

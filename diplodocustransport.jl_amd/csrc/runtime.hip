@@ -277,7 +277,7 @@ static void release_binary_device(dtx_handle* h) {
 }
 
 static int64_t chunk_elems(int precision) {
-    // FP64/FP32: [t][r] (KC x TK);  BF16X3: hi [r][t] then lo [r][t]
+    // FP64/FP32: [t][r] (KC x TK);  BF16X3: hi then lo, each in MFMA fragment order
     return (int64_t)KC * TK * (precision == DTX_PREC_BF16X3 ? 2 : 1);
 }
 
@@ -355,8 +355,8 @@ static int pack_upload_channel(dtx_handle* h, const HostChannel& hc) {
                     } else {
                         uint16_t hi, lo;
                         split_bf16(v, &hi, &lo);
-                        ((uint16_t*)buf.data())[base + r * KC + t] = hi;
-                        ((uint16_t*)buf.data())[base + TK * KC + r * KC + t] = lo;
+                        ((uint16_t*)buf.data())[base + bf16_frag_index(r, t)] = hi;
+                        ((uint16_t*)buf.data())[base + TK * KC + bf16_frag_index(r, t)] = lo;
                     }
                 }
             }

@@ -67,8 +67,8 @@ __global__ void synth_gain_kernel(SynthArgs s) {
             uint16_t hi, lo;
             split_bf16(v, &hi, &lo);
             uint16_t* b = (uint16_t*)s.A + 2 * base;
-            b[r * KC + t] = hi;
-            b[TK * KC + r * KC + t] = lo;
+            b[bf16_frag_index(r, t)] = hi;
+            b[TK * KC + bf16_frag_index(r, t)] = lo;
         }
     }
 }
