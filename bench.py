@@ -42,7 +42,7 @@ DESCRIPTIONS = {
 # HBM bytes per launch of the FP64 contraction from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (separate runs, each scaled by the same counter over a 1 GiB device copy):
 # profiles/transport_1d_hbm_traffic.txt.
-TRAFFIC = {("transport_1d", "bin_f64_kernel<64>"): None}
+TRAFFIC = {("transport_1d", "bin_f64_kernel<64>"): 882_940_000}
 
 
 def log(*a):
