@@ -571,18 +571,26 @@ static int build_work(dtx_handle* h) {
         int rc = dalloc(h, &h->d_part, h->part_stride * h->part_slots);
         if (rc) return rc;
     }
-    // Launch order: gain units, then the short LOSS-layout units; the cell tiles of a unit
-    // (which read the same operator block) are adjacent.
-    std::vector<BinWork> items;
-    items.reserve((gain_units.size() + loss_units.size() + moment_units.size()) * ctiles);
+    // Launch order: gain units, then the short LOSS-layout units.  Workgroups are dispatched to
+    // the eight XCDs round-robin, and each XCD has its own L2: units go in groups of eight, and
+    // the item of (unit x of the group, cell tile t) is number 8t + x of the group, so all cell
+    // tiles of a unit -- which read the same operator block -- run on one XCD at about the same
+    // time and fetch the block from HBM once.
+    std::vector<const Unit*> order;
     for (const auto* list : {&gain_units, &loss_units, &moment_units})
-        for (const Unit& u : *list)
-            for (int ct = 0; ct < ctiles; ++ct) {
-                BinWork w = u.w;
+        for (const Unit& u : *list) order.push_back(&u);
+    std::vector<BinWork> items;
+    items.reserve(order.size() * ctiles);
+    for (size_t g0 = 0; g0 < order.size(); g0 += 8) {
+        const size_t g1 = std::min(order.size(), g0 + 8);
+        for (int ct = 0; ct < ctiles; ++ct)
+            for (size_t x = g0; x < g1; ++x) {
+                BinWork w = order[x]->w;
                 w.ctile = ct;
-                w.part = h->d_part + (int64_t)u.slot * h->part_stride;
+                w.part = h->d_part + (int64_t)order[x]->slot * h->part_stride;
                 items.push_back(w);
             }
+    }
     h->nitems = (int32_t)items.size();
     h->items_host = items;
     if (h->nitems) {
