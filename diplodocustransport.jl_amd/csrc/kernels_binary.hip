@@ -19,6 +19,7 @@
 // MFMA operand layout (16x16xK): lane l supplies A[l%16][kgroup(l/16)], B[kgroup(l/16)][l%16].
 // The 16x16 result tile is held as D[4*(l/16)+e][l%16] (e = 0..3) by the FP32 and BF16
 // instructions, and as D[(l/16)+4*e][l%16] by v_mfma_f64_16x16x4_f64 (measured on gfx950).
+#include "bf16_util.h"
 #include "dtx_internal.h"
 
 namespace dtx {
@@ -480,6 +481,185 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
     const BinWork& w = items[blockIdx.x];
     if (w.kind == CH_LOSS) f64_item<TC>(w, sm.d);
     else bf16x3_item<TC>(w, sm.b);
+}
+
+// --------------------------------------------------------------- wide split BF16 x3 -----
+// For many cells (C > 64) the 64 x 64 tile moves too many bytes per MFMA through global loads,
+// LDS and barriers.  The wide item covers 128 rows (two packed 64-row slices, k tiles 2w and
+// 2w+1) x 128 cells, and its four waves each own a 64 x 64 block (16 MFMA tiles), so every
+// staged byte feeds twice the MFMAs.  B is staged as B' = f_i f_j: formed in FP32 from the
+// cell-major FP32 copy of f and split into bf16 hi + lo while it is written to LDS, so no
+// row-end fold is needed.  FP32 accumulators are added into a second FP32 set every
+// WIDE_FLUSH chunks (<= 512 products per partial, <= 64 partials per item), which is written
+// to the item's FP64 slot at the end.
+constexpr int WIDE_FLUSH = 16;
+
+struct SmemWide {
+    u32x4 Ah[2][8 * 64];   // [row block 0..7][lane]: blocks 0-3 from slice 0, 4-7 from slice 1
+    u32x4 Al[2][8 * 64];
+    u32x4 Bh[2][8 * 64];   // [cell block 0..7][lane]
+    u32x4 Bl[2][8 * 64];
+    uint32_t ctab[MAX_RANGE];
+};
+
+// Four bf16 hi/lo pairs of x * y (FP32 product, RNE splits), packed two per 32-bit word.
+__device__ inline void split4(const float* fj, float fi, uint32_t* hi, uint32_t* lo) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t h2 = 0, l2 = 0;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float p = fi * fj[2 * q + e];
+            const uint16_t h = bf16_rne(p);
+            const uint16_t l = bf16_rne(p - bf16_to_f32(h));
+            h2 |= (uint32_t)h << (16 * e);
+            l2 |= (uint32_t)l << (16 * e);
+        }
+        hi[q] = h2;
+        lo[q] = l2;
+    }
+}
+
+__device__ __forceinline__ void wide_item(const BinWork& w, SmemWide& sm) {
+    auto& Ah = sm.Ah;
+    auto& Al = sm.Al;
+    auto& Bh = sm.Bh;
+    auto& Bl = sm.Bl;
+    uint32_t* ctab = sm.ctab;
+    const ItemView v = load_item(w, WIDE, 2);   // v.A: slice 0 (k tile 2 * ktile)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wk = wave & 1, wc = wave >> 1;    // this wave: rows 64 wk.., cells 64 wc..
+    // slice s of the item starts at k tile 2 * w.ktile + s (v.A already includes the factor of
+    // load_item, which applied w.ktile; the second slice is one k-tile stride further)
+    const int64_t slice = w.a_ktile_stride;     // bf16 elements per packed 64-row slice
+    const GLB uint16_t* A0 = (const GLB uint16_t*)((const GLB char*)w.A) +
+                             (int64_t)(2 * w.ktile) * slice + (int64_t)v.c_begin * TK * KC * 2;
+    const bool has1 = (2 * w.ktile + 1) * TK < v.rows;
+    const GLB float* F = (const GLB float*)v.B;   // fc32[cell * ldb + kp]
+    // B' quads of this thread: cell block tid / 32, cell 16 (tid / 32) + tid % 16, j half
+    // (tid / 16) % 2 -> j 16 jh .. 16 jh + 15 (lanes 16 (2 jh) and 16 (2 jh + 1) of the block)
+    const int bcb = tid >> 5, bcell = 16 * bcb + (tid & 15), bjh = (tid >> 4) & 1;
+    const GLB float* Frow = F + (int64_t)(v.c0 + bcell) * v.ldb;
+    load_ctab(v, ctab);
+
+    u32x4 rh0, rl0, rh1, rl1;   // A quads (slice 0 and 1)
+    f4 rf0, rf1, rf2, rf3;      // 16 f_j (FP32); named registers, not an array captured by
+                                // the lambdas (which the compiler would place in memory)
+    float rfi = 0.0f;
+    auto load_chunk = [&](int ch) {
+        const uint32_t ce = ctab[ch];
+        const GLB u32x4* s0 = (const GLB u32x4*)(A0 + (int64_t)ch * TK * KC * 2);
+        rh0 = s0[tid];
+        rl0 = s0[256 + tid];
+        // slice 1 (absent when the channel has an odd number of k tiles: then slice 0 is read
+        // again and replaced by zeros, without a branch)
+        const GLB u32x4* s1 = has1 ? (const GLB u32x4*)(A0 + slice + (int64_t)ch * TK * KC * 2)
+                                   : s0;
+        const u32x4 zero = {0u, 0u, 0u, 0u};
+        const u32x4 x1 = s1[tid], y1 = s1[256 + tid];
+        rh1 = has1 ? x1 : zero;
+        rl1 = has1 ? y1 : zero;
+        const GLB f4* fj = (const GLB f4*)(Frow + (int)(ce >> 16) + 16 * bjh);
+        rf0 = fj[0];
+        rf1 = fj[1];
+        rf2 = fj[2];
+        rf3 = fj[3];
+        rfi = Frow[ce & 0xFFFFu];
+    };
+    auto store_chunk = [&](int buf) {
+        Ah[buf][tid] = rh0;
+        Al[buf][tid] = rl0;
+        Ah[buf][256 + tid] = rh1;
+        Al[buf][256 + tid] = rl1;
+        // B' = f_i f_j split; quad k8 of this thread's j half holds j 8 k8 .. 8 k8 + 7
+        auto put = [&](int k8, const f4 x0, const f4 x1) {
+            const float fj[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+            uint32_t h[4], l[4];
+            split4(fj, rfi, h, l);
+            split4(fj + 4, rfi, h + 2, l + 2);
+            const int q = bcb * 64 + (tid & 15) + 16 * (2 * bjh + k8);
+            Bh[buf][q] = u32x4{h[0], h[1], h[2], h[3]};
+            Bl[buf][q] = u32x4{l[0], l[1], l[2], l[3]};
+        };
+        put(0, rf0, rf1);
+        put(1, rf2, rf3);
+    };
+
+    f4 acc[4][4], acc2[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = acc2[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+    const int nch = v.c_end - v.c_begin;
+    if (nch > 0) {
+        load_chunk(0);
+        store_chunk(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const bool more = ch + 1 < nch;
+        if (more) load_chunk(ch + 1);
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ah[r] = __builtin_bit_cast(bf16x8, Ah[buf][(4 * wk + r) * 64 + lane]);
+            al[r] = __builtin_bit_cast(bf16x8, Al[buf][(4 * wk + r) * 64 + lane]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bf16x8 bh = __builtin_bit_cast(bf16x8, Bh[buf][(4 * wc + c) * 64 + lane]);
+            const bf16x8 bl = __builtin_bit_cast(bf16x8, Bl[buf][(4 * wc + c) * 64 + lane]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh, acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl, acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh, acc[r][c], 0, 0, 0);
+            }
+        }
+        if ((ch + 1) % WIDE_FLUSH == 0 || !more) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    acc2[r][c] += acc[r][c];
+                    acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+                }
+        }
+        if (more) store_chunk(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // write the item's partial sums (FP32 result layout D[4 (l / 16) + e][l % 16])
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int cell = v.c0 + 64 * wc + 16 * c + (lane & 15);
+            if (cell >= v.cells) continue;
+            GLB double* dst = v.part + (int64_t)cell * v.N + v.row_base;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int krow = 2 * w.ktile * TK + 64 * wk + 16 * r + 4 * (lane >> 4) + e;
+                if (krow < v.rows) dst[krow] = (double)acc2[r][c][e];
+            }
+        }
+}
+
+__global__ __launch_bounds__(256) void bin_bf16x3_wide_kernel(const BinWork* __restrict__ items) {
+    __shared__ union {
+        SmemF64<64> d;
+        SmemWide b;
+    } sm;
+    const BinWork& w = items[blockIdx.x];
+    if (w.kind == CH_LOSS) f64_item<64>(w, sm.d);
+    else wide_item(w, sm.b);
+}
+
+hipError_t launch_binary_wide(const BinWork* items, int nitems, hipStream_t stream) {
+    if (nitems <= 0) return hipSuccess;
+    bin_bf16x3_wide_kernel<<<nitems, 256, 0, stream>>>(items);
+    return hipGetLastError();
 }
 
 // Pack rows [row0, row0+rows) of the FP64 loss matrix D (N x N, column-major) as a LOSS

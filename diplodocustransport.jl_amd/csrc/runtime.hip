@@ -97,7 +97,8 @@ struct dtx_handle {
     double* d_f[2] = {nullptr, nullptr};
     int cur = 0;
     double* d_fT = nullptr;       // FP64, state-major, padded states: [Np][ldc]
-    float* d_fT32 = nullptr;      // unused: the FP32 contraction reads the FP64 copy
+    float* d_fc32 = nullptr;      // FP32, cell-major: [ldc][Np] (wide split-BF16 contraction)
+    bool wide = false;            // split-BF16 with 128 x 128 tiles (C > 64)
     uint16_t* d_fhi = nullptr;    // BF16 split, cell-major: [ldc][Np]
     uint16_t* d_flo = nullptr;
     double *d_invap = nullptr, *d_dfinj = nullptr, *d_vol = nullptr, *d_finit = nullptr;
@@ -486,12 +487,18 @@ static int build_work(dtx_handle* h) {
     h->correct_active = false;
     const int C = h->C;
     const int ctiles = (C + h->tc - 1) / h->tc;
+    // wide split-BF16 gain items: two k tiles x 128 cells each
+    const int wctiles = (C + WIDE - 1) / WIDE;
+    auto gain_ksteps = [&](const PackedChannel& c) {
+        return h->wide ? (c.nktiles + 1) / 2 : c.nktiles;
+    };
     double total = 0;
     h->pairs_total = 0;
     h->bin_flops = 0;
     h->gain_bytes = 0;
     for (auto& c : h->chans) {
-        total += (double)c.nktiles * ctiles * c.nchunks;
+        if (c.kind != CH_LOSS && h->wide) total += (double)gain_ksteps(c) * wctiles * c.nchunks;
+        else total += (double)c.nktiles * ctiles * c.nchunks;
         if (c.kind != CH_LOSS) {
             h->pairs_total += c.npairs;
             h->bin_flops += 2.0 * c.No * (double)c.npairs * C;
@@ -500,7 +507,8 @@ static int build_work(dtx_handle* h) {
         }
         h->gain_bytes += (double)c.a_bytes;
     }
-    const int target_items = 1024;  // about four resident workgroups per CU
+    // about four resident workgroups per CU (two for the wide items, which hold 68 KB of LDS)
+    const int target_items = h->wide ? 512 : 1024;
     int64_t range = (int64_t)std::ceil(total / target_items);
     range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
     // LOSS-layout channels (losses, moment rates) are small FP64 GEMVs next to the gain
@@ -511,25 +519,27 @@ static int build_work(dtx_handle* h) {
     // A unit is (channel, chunk range, k tile); it has one item per cell tile.  Gain slots of a
     // species come first, then its loss slots, and the update kernel adds them in slot order.
     // Moment-rate slots follow all of them.
-    struct Unit { BinWork w; int slot; };
+    struct Unit { BinWork w; int slot; int nct; };
     std::vector<Unit> gain_units, loss_units, moment_units;
     const bool bf16 = h->precision == DTX_PREC_BF16X3;
     auto add_units = [&](const PackedChannel& c, int64_t rg, std::vector<Unit>& out,
                          const std::function<int()>& next) {
         const bool loss = c.kind == CH_LOSS;
+        const bool wide = h->wide && !loss;
         const int prec = loss ? DTX_PREC_FP64 : h->precision;
         const int s_row = c.moment >= 0 ? c.s1 : c.s_out;  // owner of the output rows
         for (int64_t c0 = 0; c0 < c.nchunks; c0 += rg) {
             const int64_t c1 = std::min<int64_t>(c0 + rg, c.nchunks);
             const int slot = next();
-            for (int kt = 0; kt < c.nktiles; ++kt) {
+            for (int kt = 0; kt < (wide ? gain_ksteps(c) : c.nktiles); ++kt) {
                 BinWork w{};
                 w.A = c.A;
                 w.chunks = c.chunks;
                 w.fT = h->d_fT;
                 if (prec == DTX_PREC_FP64) w.B = h->d_fT;
+                else if (wide) w.B = h->d_fc32;
                 else w.B = h->d_fhi;
-                w.b_lo = (bf16 && !loss) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
+                w.b_lo = (bf16 && !loss && !wide) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
                 w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
                 w.ktile = kt;
                 w.c_begin = (int32_t)c0;
@@ -542,7 +552,7 @@ static int build_work(dtx_handle* h) {
                 w.ldb = h->Np;
                 w.N = h->N;
                 w.kind = c.kind;
-                out.push_back({w, slot});
+                out.push_back({w, slot, wide ? wctiles : ctiles});
             }
         }
     };
@@ -583,8 +593,11 @@ static int build_work(dtx_handle* h) {
     items.reserve(order.size() * ctiles);
     for (size_t g0 = 0; g0 < order.size(); g0 += 8) {
         const size_t g1 = std::min(order.size(), g0 + 8);
-        for (int ct = 0; ct < ctiles; ++ct)
+        int nct = 0;
+        for (size_t x = g0; x < g1; ++x) nct = std::max(nct, order[x]->nct);
+        for (int ct = 0; ct < nct; ++ct)
             for (size_t x = g0; x < g1; ++x) {
+                if (ct >= order[x]->nct) continue;
                 BinWork w = order[x]->w;
                 w.ctile = ct;
                 w.part = h->d_part + (int64_t)order[x]->slot * h->part_stride;
@@ -680,7 +693,7 @@ static int finalize_binary(dtx_handle* h) {
 // Publish the current f into the contraction's B-operand copies (padded, per precision).
 static int refresh_fT(dtx_handle* h) {
     HIP_CHECK(h, launch_publish_state(h->d_f[h->cur], h->N, h->C, h->ldc, h->d_pad_index, h->Np,
-                                      h->d_fT, h->d_fT32, h->d_fhi, h->d_flo, h->stream));
+                                      h->d_fT, h->d_fc32, h->d_fhi, h->d_flo, h->stream));
     return DTX_OK;
 }
 
@@ -777,7 +790,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.f = h->d_f[h->cur];
     a.f_out = h->d_f[h->cur ^ 1];
     a.fT = h->d_fT;
-    a.fT32 = h->d_fT32;
+    a.fc32 = h->d_fc32;
     a.fhi = h->d_fhi;
     a.flo = h->d_flo;
     a.pad_index = h->d_pad_index;
@@ -853,7 +866,8 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     if (bin) {
         int rc = timed_begin(h, 0, &ev);
         if (rc) return rc;
-        HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
+        if (h->wide) HIP_CHECK(h, launch_binary_wide(h->d_items, h->nitems, h->stream));
+        else HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
         if (h->correct_active) {
             CorrectArgs ca{};
             ca.part = h->d_part;
@@ -955,7 +969,7 @@ struct Group {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<dtx_handle*> members;
-    BinWork* d_items[3] = {nullptr, nullptr, nullptr};  // members with cell tiles of 16/64/128
+    BinWork* d_items[3] = {nullptr, nullptr, nullptr};  // cell tiles of 16 / 64 / wide items
     int n_items[3] = {0, 0, 0};
     int precision = 0;
     int32_t* d_block_start = nullptr;
@@ -1021,7 +1035,7 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
     if (rc) return rc;
     group_free_tables(g);
     g->members.assign(hs, hs + count);
-    std::vector<BinWork> lists[3];  // cell tiles of 16, 64 and 128
+    std::vector<BinWork> lists[3];  // cell tiles of 16, of 64, and wide split-BF16 items
     std::vector<int32_t> starts;
     g->total_blocks = 0;
     g->max_N = 0;
@@ -1049,9 +1063,8 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
             g->err = "member stream failed";
             return DTX_ERR_HIP;
         }
-        const int wide = h->tc == 16 ? 0 : (h->tc == 64 ? 1 : 2);
-        if (h->binary)
-            lists[wide].insert(lists[wide].end(), h->items_host.begin(), h->items_host.end());
+        const int q = h->wide ? 2 : (h->tc == 16 ? 0 : 1);
+        if (h->binary) lists[q].insert(lists[q].end(), h->items_host.begin(), h->items_host.end());
         starts.push_back(g->total_blocks);
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
         g->max_N = std::max(g->max_N, h->N);
@@ -1114,9 +1127,10 @@ int group_step(Group* g, const double* dt, int verbose) {
                                   hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess) e = hipEventRecord(g->ev[slot], g->stream);
     g->ev_used[slot] = true;
-    const int ktc[3] = {16, 64, 128};
-    for (int q = 0; q < 3 && e == hipSuccess; ++q)
+    const int ktc[2] = {16, 64};
+    for (int q = 0; q < 2 && e == hipSuccess; ++q)
         e = launch_binary(g->precision, ktc[q], g->d_items[q], g->n_items[q], g->stream);
+    if (e == hipSuccess) e = launch_binary_wide(g->d_items[2], g->n_items[2], g->stream);
     if (e == hipSuccess)
         e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
                                       g->max_N, g->stream);
@@ -1184,7 +1198,9 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     h->precision = cfg->precision;
     h->method = cfg->method;
     h->tc = (h->C <= 16) ? 16 : 64;
-    h->ldc = (h->C + h->tc - 1) / h->tc * h->tc;
+    h->wide = h->precision == DTX_PREC_BF16X3 && h->C > 64;
+    const int cell_align = h->wide ? WIDE : h->tc;
+    h->ldc = (h->C + cell_align - 1) / cell_align * cell_align;
     h->sp_pad_off.assign(h->S + 1, 0);
     for (int s = 0; s < h->S; ++s)
         h->sp_pad_off[s + 1] = h->sp_pad_off[s] +
@@ -1212,7 +1228,9 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_f[0], n);
     rc |= dalloc(hp, &h->d_f[1], n);
     rc |= dalloc(hp, &h->d_fT, (int64_t)h->ldc * h->Np);
-    if (h->precision == DTX_PREC_BF16X3) {
+    if (h->wide) {
+        rc |= dalloc(hp, &h->d_fc32, (int64_t)h->ldc * h->Np);
+    } else if (h->precision == DTX_PREC_BF16X3) {
         rc |= dalloc(hp, &h->d_fhi, (int64_t)h->ldc * h->Np);
         rc |= dalloc(hp, &h->d_flo, (int64_t)h->ldc * h->Np);
     }
@@ -1252,8 +1270,8 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     if (!rc) {
         e = hipMemsetAsync(h->d_counter, 0, sizeof(uint32_t), h->stream);
         if (e == hipSuccess) e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * h->Np * 8, h->stream);
-        if (e == hipSuccess && h->d_fT32)
-            e = hipMemsetAsync(h->d_fT32, 0, (size_t)h->ldc * h->Np * 4, h->stream);
+        if (e == hipSuccess && h->d_fc32)
+            e = hipMemsetAsync(h->d_fc32, 0, (size_t)h->ldc * h->Np * 4, h->stream);
         if (e == hipSuccess && h->d_fhi)
             e = hipMemsetAsync(h->d_fhi, 0, (size_t)h->ldc * h->Np * 2, h->stream);
         if (e == hipSuccess && h->d_flo)

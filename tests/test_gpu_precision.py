@@ -52,10 +52,20 @@ def synthetic_small():
     return workloads.precision_box(p_num=8, u_num=4, h_num=2, num_species=3, steps=STEPS)
 
 
+def many_cells():
+    """80 cells (the split-BF16 path switches to its 128-cell tiles above 64 cells: one partial
+    cell tile) and 160 states per species (three 64-row k tiles: one full pair, one single)."""
+    return workloads.tiny(p_num=20, u_num=8, x_num=80, with_flux=False, monolithic=False)
+
+
+CASES = {"host_channels": homogeneous_channels, "device_channels": synthetic_small,
+         "many_cells": many_cells}
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
-@pytest.mark.parametrize("case", ["host_channels", "device_channels"])
+@pytest.mark.parametrize("case", list(CASES))
 def test_moment_rate_correction(precision, case):
-    w = homogeneous_channels() if case == "host_channels" else synthetic_small()
+    w = CASES[case]()
     print(f"[case] {case} {precision} N={w.ps.n_momentum} C={w.ps.n_space}", flush=True)
     f64, n64, e64 = run(w, "fp64")
     fc, nc, ec = run(w, precision, correction=True)
