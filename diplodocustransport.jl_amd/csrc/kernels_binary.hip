@@ -502,21 +502,19 @@ struct SmemWide {
     uint32_t ctab[MAX_RANGE];
 };
 
-// Four bf16 hi/lo pairs of x * y (FP32 product, RNE splits), packed two per 32-bit word.
+// Four bf16 hi/lo pairs of fi * fj[0..3] (FP32 products; round-to-nearest-even splits by the
+// packed FP32 -> BF16 conversion), packed two per 32-bit word.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 __device__ inline void split4(const float* fj, float fi, uint32_t* hi, uint32_t* lo) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        uint32_t h2 = 0, l2 = 0;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const float p = fi * fj[2 * q + e];
-            const uint16_t h = bf16_rne(p);
-            const uint16_t l = bf16_rne(p - bf16_to_f32(h));
-            h2 |= (uint32_t)h << (16 * e);
-            l2 |= (uint32_t)l << (16 * e);
-        }
-        hi[q] = h2;
-        lo[q] = l2;
+        const f2 p = {fi * fj[2 * q], fi * fj[2 * q + 1]};
+        const bf16x2 h = __builtin_convertvector(p, bf16x2);
+        const f2 r = p - __builtin_convertvector(h, f2);
+        const bf16x2 l = __builtin_convertvector(r, bf16x2);
+        hi[q] = __builtin_bit_cast(uint32_t, h);
+        lo[q] = __builtin_bit_cast(uint32_t, l);
     }
 }
 
