@@ -28,6 +28,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte load / LDS store
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <int TC>
 struct Tiling {
@@ -534,53 +535,63 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide& sm) {
                              (int64_t)(2 * w.ktile) * slice + (int64_t)v.c_begin * TK * KC * 2;
     const bool has1 = (2 * w.ktile + 1) * TK < v.rows;
     const GLB float* F = (const GLB float*)v.B;   // fc32[cell * ldb + kp]
-    // B' quads of this thread: cell block tid / 32, cell 16 (tid / 32) + tid % 16, j half
-    // (tid / 16) % 2 -> j 16 jh .. 16 jh + 15 (lanes 16 (2 jh) and 16 (2 jh + 1) of the block)
-    const int bcb = tid >> 5, bcell = 16 * bcb + (tid & 15), bjh = (tid >> 4) & 1;
-    const GLB float* Frow = F + (int64_t)(v.c0 + bcell) * v.ldb;
+    // B' of this thread: four cells 32 wave + 8 q + lane / 8 (q = 0..3), each at the j quad
+    // lane % 8 (j 4 (lane % 8) .. + 3), so that eight lanes read one cell's 128-byte j row and
+    // every load instruction touches whole 64-byte segments
+    const int bjq = lane & 7;
+    const GLB float* Frow = F + (int64_t)(v.c0 + 32 * wave + (lane >> 3)) * v.ldb;
+    const int64_t fstep = (int64_t)8 * v.ldb;   // next q: 8 cells further
     load_ctab(v, ctab);
 
     u32x4 rh0, rl0, rh1, rl1;   // A quads (slice 0 and 1)
-    f4 rf0, rf1, rf2, rf3;      // 16 f_j (FP32); named registers, not an array captured by
-                                // the lambdas (which the compiler would place in memory)
-    float rfi = 0.0f;
+    f4 rf0, rf1, rf2, rf3;      // f_j quads of the four cells (FP32); named registers, not an
+                                // array captured by the lambdas (which the compiler would
+                                // place in memory)
+    float rfi0 = 0.0f, rfi1 = 0.0f, rfi2 = 0.0f, rfi3 = 0.0f;
     auto load_chunk = [&](int ch) {
         const uint32_t ce = ctab[ch];
         const GLB u32x4* s0 = (const GLB u32x4*)(A0 + (int64_t)ch * TK * KC * 2);
         rh0 = s0[tid];
         rl0 = s0[256 + tid];
         // slice 1 (absent when the channel has an odd number of k tiles: then slice 0 is read
-        // again and replaced by zeros, without a branch)
+        // again, and zeros are stored in its place)
         const GLB u32x4* s1 = has1 ? (const GLB u32x4*)(A0 + slice + (int64_t)ch * TK * KC * 2)
                                    : s0;
-        const u32x4 zero = {0u, 0u, 0u, 0u};
-        const u32x4 x1 = s1[tid], y1 = s1[256 + tid];
-        rh1 = has1 ? x1 : zero;
-        rl1 = has1 ? y1 : zero;
-        const GLB f4* fj = (const GLB f4*)(Frow + (int)(ce >> 16) + 16 * bjh);
-        rf0 = fj[0];
-        rf1 = fj[1];
-        rf2 = fj[2];
-        rf3 = fj[3];
-        rfi = Frow[ce & 0xFFFFu];
+        rh1 = s1[tid];
+        rl1 = s1[256 + tid];
+        const int j0 = (int)(ce >> 16) + 4 * bjq, i = (int)(ce & 0xFFFFu);
+        rf0 = *(const GLB f4*)(Frow + j0);
+        rf1 = *(const GLB f4*)(Frow + fstep + j0);
+        rf2 = *(const GLB f4*)(Frow + 2 * fstep + j0);
+        rf3 = *(const GLB f4*)(Frow + 3 * fstep + j0);
+        rfi0 = Frow[i];
+        rfi1 = Frow[fstep + i];
+        rfi2 = Frow[2 * fstep + i];
+        rfi3 = Frow[3 * fstep + i];
     };
     auto store_chunk = [&](int buf) {
         Ah[buf][tid] = rh0;
         Al[buf][tid] = rl0;
-        Ah[buf][256 + tid] = rh1;
-        Al[buf][256 + tid] = rl1;
-        // B' = f_i f_j split; quad k8 of this thread's j half holds j 8 k8 .. 8 k8 + 7
-        auto put = [&](int k8, const f4 x0, const f4 x1) {
-            const float fj[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-            uint32_t h[4], l[4];
-            split4(fj, rfi, h, l);
-            split4(fj + 4, rfi, h + 2, l + 2);
-            const int q = bcb * 64 + (tid & 15) + 16 * (2 * bjh + k8);
-            Bh[buf][q] = u32x4{h[0], h[1], h[2], h[3]};
-            Bl[buf][q] = u32x4{l[0], l[1], l[2], l[3]};
+        // the zero select sits here, after the MFMAs, and not next to the loads: there it
+        // would make every chunk wait for its own prefetch
+        const u32x4 zero = {0u, 0u, 0u, 0u};
+        Ah[buf][256 + tid] = has1 ? rh1 : zero;
+        Al[buf][256 + tid] = has1 ? rl1 : zero;
+        // B' = f_i f_j split: the quad is one half (8 bytes) of the fragment slot of its cell
+        // and j octet (slot index 64 (cell / 16) + cell % 16 + 16 (j / 8))
+        auto put = [&](int q, const f4 x, float fi) {
+            const float fj[4] = {x[0], x[1], x[2], x[3]};
+            uint32_t h[2], l[2];
+            split4(fj, fi, h, l);
+            const int cell = 32 * wave + 8 * q + (lane >> 3);
+            const int slot = 64 * (cell >> 4) + (cell & 15) + 16 * (bjq >> 1);
+            ((u32x2*)&Bh[buf][slot])[bjq & 1] = u32x2{h[0], h[1]};
+            ((u32x2*)&Bl[buf][slot])[bjq & 1] = u32x2{l[0], l[1]};
         };
-        put(0, rf0, rf1);
-        put(1, rf2, rf3);
+        put(0, rf0, rfi0);
+        put(1, rf1, rfi1);
+        put(2, rf2, rfi2);
+        put(3, rf3, rfi3);
     };
 
     f4 acc[4][4], acc2[4][4];
