@@ -81,8 +81,6 @@ struct UpdateArgs {
     const double* f;        // current state (cell-major)
     double* f_out;          // next state
     double* fT;             // FP64 state-major copy of f_out (padded states) for the contraction
-    float* fc32;            // FP32 copy of f_out, cell-major over padded states
-                            // (fc32[cell * ldb + kp]; wide split-BF16 contraction) or null
     uint16_t* fhi;          // BF16 hi/lo split, cell-major over padded states, or null
     uint16_t* flo;
     const int32_t* pad_index;  // padded index of every state (N)
@@ -153,10 +151,10 @@ struct CorrectArgs {
 hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream);
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
-// Split-BF16 contraction with 128 x 128 (rows x cells) tiles for C > 64: gain items are wide,
-// LOSS-layout items keep the 64 x 64 FP64 body.
-constexpr int WIDE = 128;
-hipError_t launch_binary_wide(const BinWork* items, int nitems, hipStream_t stream);
+// Split-BF16 contraction with (64 ns) x WIDE (rows x cells) tiles for C > 64, ns = 2 or 4
+// k tiles per item: gain items are wide, LOSS-layout items keep the 64 x 64 FP64 body.
+constexpr int WIDE = 128;   // cells per wide item
+hipError_t launch_binary_wide(int ns, const BinWork* items, int nitems, hipStream_t stream);
 hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,
                             int rows, int nktiles, double* A, hipStream_t stream);
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);
@@ -166,7 +164,7 @@ hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* blo
                                     int count, int total_blocks, int max_N, hipStream_t stream);
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream);
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
-                                int Np, double* fT, float* fc32, uint16_t* fhi, uint16_t* flo,
+                                int Np, double* fT, uint16_t* fhi, uint16_t* flo,
                                 hipStream_t stream);
 
 // synthetic generator (synth.hip)

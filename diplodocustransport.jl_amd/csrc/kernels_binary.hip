@@ -28,7 +28,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte load / LDS store
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <int TC>
 struct Tiling {
@@ -112,7 +111,8 @@ __device__ inline double f_at(const ItemView& v, int ip, int col) {
 // Copy the item's slice of the chunk table into LDS, so the main loop never waits on a global
 // load for its control flow.
 __device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
-    for (int q = threadIdx.x; q < v.c_end - v.c_begin; q += 256) ctab[q] = v.chunks[v.c_begin + q];
+    for (int q = threadIdx.x; q < v.c_end - v.c_begin; q += blockDim.x)
+        ctab[q] = v.chunks[v.c_begin + q];
     __syncthreads();
 }
 
@@ -486,119 +486,96 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
 
 // --------------------------------------------------------------- wide split BF16 x3 -----
 // For many cells (C > 64) the 64 x 64 tile moves too many bytes per MFMA through global loads,
-// LDS and barriers.  The wide item covers 128 rows (two packed 64-row slices, k tiles 2w and
-// 2w+1) x 128 cells, and its four waves each own a 64 x 64 block (16 MFMA tiles), so every
-// staged byte feeds twice the MFMAs.  B is staged as B' = f_i f_j: formed in FP32 from the
-// cell-major FP32 copy of f and split into bf16 hi + lo while it is written to LDS, so no
-// row-end fold is needed.  FP32 accumulators are added into a second FP32 set every
-// WIDE_FLUSH chunks (<= 512 products per partial, <= 64 partials per item), which is written
-// to the item's FP64 slot at the end.
-constexpr int WIDE_FLUSH = 16;
-
+// LDS and barriers.  A wide item covers NS packed 64-row slices (k tiles NS w .. NS w + NS - 1)
+// x 128 cells with 2 NS waves, each owning a 64 x 64 block (16 MFMA tiles): every staged A
+// byte feeds two waves and every B byte NS waves.  NS = 4 (eight waves, one workgroup per CU)
+// halves the B bytes per MFMA again but needs enough items to fill the chip; NS = 2 (four
+// waves, two workgroups per CU) serves fewer cells.  Both operands are staged as plain 16-byte
+// copies in MFMA fragment order: A from the packed operator, B from the cell-major hi/lo split
+// of f.  The row multiplier f_i is staged beside each chunk and folded in FP32 when the row
+// ends, tot += f_i * acc; tot is written to the item's FP64 slot at the end.
+template <int NS>
 struct SmemWide {
-    u32x4 Ah[2][8 * 64];   // [row block 0..7][lane]: blocks 0-3 from slice 0, 4-7 from slice 1
-    u32x4 Al[2][8 * 64];
-    u32x4 Bh[2][8 * 64];   // [cell block 0..7][lane]
+    u32x4 Ah[2][4 * NS * 64];   // [row block][lane]: blocks 4s..4s+3 from slice s
+    u32x4 Al[2][4 * NS * 64];
+    u32x4 Bh[2][8 * 64];        // [cell block 0..7][lane]
     u32x4 Bl[2][8 * 64];
+    double Fs[2][WIDE];         // f_i of the chunk's row, per cell of the tile
     uint32_t ctab[MAX_RANGE];
 };
 
-// Four bf16 hi/lo pairs of fi * fj[0..3] (FP32 products; round-to-nearest-even splits by the
-// packed FP32 -> BF16 conversion), packed two per 32-bit word.
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float f2 __attribute__((ext_vector_type(2)));
-__device__ inline void split4(const float* fj, float fi, uint32_t* hi, uint32_t* lo) {
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const f2 p = {fi * fj[2 * q], fi * fj[2 * q + 1]};
-        const bf16x2 h = __builtin_convertvector(p, bf16x2);
-        const f2 r = p - __builtin_convertvector(h, f2);
-        const bf16x2 l = __builtin_convertvector(r, bf16x2);
-        hi[q] = __builtin_bit_cast(uint32_t, h);
-        lo[q] = __builtin_bit_cast(uint32_t, l);
-    }
-}
-
-__device__ __forceinline__ void wide_item(const BinWork& w, SmemWide& sm) {
+template <int NS>
+__device__ __forceinline__ void wide_item(const BinWork& w, SmemWide<NS>& sm) {
+    constexpr int NT = 128 * NS;      // threads
+    constexpr int BQ = 4 / NS;        // B cell blocks staged per thread
     auto& Ah = sm.Ah;
     auto& Al = sm.Al;
     auto& Bh = sm.Bh;
     auto& Bl = sm.Bl;
+    auto& Fs = sm.Fs;
     uint32_t* ctab = sm.ctab;
-    const ItemView v = load_item(w, WIDE, 2);   // v.A: slice 0 (k tile 2 * ktile)
+    const ItemView v = load_item(w, WIDE, 2);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wk = wave & 1, wc = wave >> 1;    // this wave: rows 64 wk.., cells 64 wc..
-    // slice s of the item starts at k tile 2 * w.ktile + s (v.A already includes the factor of
-    // load_item, which applied w.ktile; the second slice is one k-tile stride further)
+    const int wk = wave % NS, wc = wave / NS;   // this wave: rows 64 wk.., cells 64 wc..
+    // A quads of this thread: slices sa and sa + NS / 2 (sa = tid / 256), quad tid % 256 of
+    // each slice's hi and lo halves.  Slice s of the item is k tile NS w.ktile + s; a slice
+    // beyond the channel's rows reads slice 0 again, and the rows it feeds are never written.
     const int64_t slice = w.a_ktile_stride;     // bf16 elements per packed 64-row slice
-    const GLB uint16_t* A0 = (const GLB uint16_t*)((const GLB char*)w.A) +
-                             (int64_t)(2 * w.ktile) * slice + (int64_t)v.c_begin * TK * KC * 2;
-    const bool has1 = (2 * w.ktile + 1) * TK < v.rows;
-    const GLB float* F = (const GLB float*)v.B;   // fc32[cell * ldb + kp]
-    // B' of this thread: four cells 32 wave + 8 q + lane / 8 (q = 0..3), each at the j quad
-    // lane % 8 (j 4 (lane % 8) .. + 3), so that eight lanes read one cell's 128-byte j row and
-    // every load instruction touches whole 64-byte segments
-    const int bjq = lane & 7;
-    const GLB float* Frow = F + (int64_t)(v.c0 + 32 * wave + (lane >> 3)) * v.ldb;
-    const int64_t fstep = (int64_t)8 * v.ldb;   // next q: 8 cells further
+    const int kt0 = NS * w.ktile;
+    const int sa = tid >> 8, qa = tid & 255;
+    auto slice_ptr = [&](int s) {
+        const bool present = (kt0 + s) * TK < v.rows;
+        return (const GLB u32x4*)((const GLB uint16_t*)((const GLB char*)w.A) +
+                                  (int64_t)(present ? kt0 + s : kt0) * slice +
+                                  (int64_t)v.c_begin * TK * KC * 2) + qa;
+    };
+    const GLB u32x4* As0 = slice_ptr(sa);
+    const GLB u32x4* As1 = slice_ptr(sa + NS / 2);
+    const GLB uint16_t* Bhi = (const GLB uint16_t*)v.B;
+    const GLB uint16_t* Blo = Bhi + v.b_lo;
+    // B quads of this thread: cell blocks wave + 2 NS q (q < BQ), fragment lane `lane` of each
+    // (cell lane % 16 of the block, j octet lane / 16); the four lanes of one cell read one
+    // contiguous 64-byte segment, and the LDS stores are linear 1 KB blocks
+    const int64_t boff = (int64_t)(v.c0 + 16 * wave + (lane & 15)) * v.ldb + 8 * (lane >> 4);
+    const int64_t bstep = (int64_t)16 * (2 * NS) * v.ldb;
     load_ctab(v, ctab);
 
-    u32x4 rh0, rl0, rh1, rl1;   // A quads (slice 0 and 1)
-    f4 rf0, rf1, rf2, rf3;      // f_j quads of the four cells (FP32); named registers, not an
-                                // array captured by the lambdas (which the compiler would
-                                // place in memory)
-    float rfi0 = 0.0f, rfi1 = 0.0f, rfi2 = 0.0f, rfi3 = 0.0f;
+    u32x4 rh0, rl0, rh1, rl1;   // A quads (slices sa, sa + NS / 2)
+    u32x4 bh[BQ], bl[BQ];       // B quads
+    double rfi = 0.0;
     auto load_chunk = [&](int ch) {
         const uint32_t ce = ctab[ch];
-        const GLB u32x4* s0 = (const GLB u32x4*)(A0 + (int64_t)ch * TK * KC * 2);
-        rh0 = s0[tid];
-        rl0 = s0[256 + tid];
-        // slice 1 (absent when the channel has an odd number of k tiles: then slice 0 is read
-        // again, and zeros are stored in its place)
-        const GLB u32x4* s1 = has1 ? (const GLB u32x4*)(A0 + slice + (int64_t)ch * TK * KC * 2)
-                                   : s0;
-        rh1 = s1[tid];
-        rl1 = s1[256 + tid];
-        const int j0 = (int)(ce >> 16) + 4 * bjq, i = (int)(ce & 0xFFFFu);
-        rf0 = *(const GLB f4*)(Frow + j0);
-        rf1 = *(const GLB f4*)(Frow + fstep + j0);
-        rf2 = *(const GLB f4*)(Frow + 2 * fstep + j0);
-        rf3 = *(const GLB f4*)(Frow + 3 * fstep + j0);
-        rfi0 = Frow[i];
-        rfi1 = Frow[fstep + i];
-        rfi2 = Frow[2 * fstep + i];
-        rfi3 = Frow[3 * fstep + i];
+        const int64_t co = (int64_t)ch * (TK * KC * 2 / 8);   // u32x4 per slice and chunk
+        rh0 = As0[co];
+        rl0 = As0[co + 256];
+        rh1 = As1[co];
+        rl1 = As1[co + 256];
+        const int j0 = (int)(ce >> 16);
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            bh[q] = *(const GLB u32x4*)(Bhi + boff + q * bstep + j0);
+            bl[q] = *(const GLB u32x4*)(Blo + boff + q * bstep + j0);
+        }
+        if (tid < WIDE) rfi = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + tid];
     };
     auto store_chunk = [&](int buf) {
-        Ah[buf][tid] = rh0;
-        Al[buf][tid] = rl0;
-        // the zero select sits here, after the MFMAs, and not next to the loads: there it
-        // would make every chunk wait for its own prefetch
-        const u32x4 zero = {0u, 0u, 0u, 0u};
-        Ah[buf][256 + tid] = has1 ? rh1 : zero;
-        Al[buf][256 + tid] = has1 ? rl1 : zero;
-        // B' = f_i f_j split: the quad is one half (8 bytes) of the fragment slot of its cell
-        // and j octet (slot index 64 (cell / 16) + cell % 16 + 16 (j / 8))
-        auto put = [&](int q, const f4 x, float fi) {
-            const float fj[4] = {x[0], x[1], x[2], x[3]};
-            uint32_t h[2], l[2];
-            split4(fj, fi, h, l);
-            const int cell = 32 * wave + 8 * q + (lane >> 3);
-            const int slot = 64 * (cell >> 4) + (cell & 15) + 16 * (bjq >> 1);
-            ((u32x2*)&Bh[buf][slot])[bjq & 1] = u32x2{h[0], h[1]};
-            ((u32x2*)&Bl[buf][slot])[bjq & 1] = u32x2{l[0], l[1]};
-        };
-        put(0, rf0, rfi0);
-        put(1, rf1, rfi1);
-        put(2, rf2, rfi2);
-        put(3, rf3, rfi3);
+        Ah[buf][sa * 256 + qa] = rh0;
+        Al[buf][sa * 256 + qa] = rl0;
+        Ah[buf][(sa + NS / 2) * 256 + qa] = rh1;
+        Al[buf][(sa + NS / 2) * 256 + qa] = rl1;
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            Bh[buf][q * NT + tid] = bh[q];
+            Bl[buf][q * NT + tid] = bl[q];
+        }
+        if (tid < WIDE) Fs[buf][tid] = rfi;
     };
 
-    f4 acc[4][4], acc2[4][4];
+    f4 acc[4][4], tot[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = acc2[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < 4; ++c) acc[r][c] = tot[r][c] = f4{0.f, 0.f, 0.f, 0.f};
     const int nch = v.c_end - v.c_begin;
     if (nch > 0) {
         load_chunk(0);
@@ -617,23 +594,29 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide& sm) {
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const bf16x8 bh = __builtin_bit_cast(bf16x8, Bh[buf][(4 * wc + c) * 64 + lane]);
-            const bf16x8 bl = __builtin_bit_cast(bf16x8, Bl[buf][(4 * wc + c) * 64 + lane]);
+            const bf16x8 bhc = __builtin_bit_cast(bf16x8, Bh[buf][(4 * wc + c) * 64 + lane]);
+            const bf16x8 blc = __builtin_bit_cast(bf16x8, Bl[buf][(4 * wc + c) * 64 + lane]);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh, acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl, acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh, acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bhc, acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], blc, acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bhc, acc[r][c], 0, 0, 0);
             }
         }
-        if ((ch + 1) % WIDE_FLUSH == 0 || !more) {
+        const uint32_t ip = ctab[ch] & 0xFFFFu;
+        if (!more || (ctab[ch + 1] & 0xFFFFu) != ip) {
+            // the row ends: tot += f_i * acc (FP32)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) {
+                const float fi = (float)Fs[buf][64 * wc + 16 * c + (lane & 15)];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    acc2[r][c] += acc[r][c];
+                for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        tot[r][c][e] = __builtin_fmaf(fi, acc[r][c][e], tot[r][c][e]);
                     acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
                 }
+            }
         }
         if (more) store_chunk(buf ^ 1);
         __syncthreads();
@@ -649,25 +632,36 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide& sm) {
             GLB double* dst = v.part + (int64_t)cell * v.N + v.row_base;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int krow = 2 * w.ktile * TK + 64 * wk + 16 * r + 4 * (lane >> 4) + e;
-                if (krow < v.rows) dst[krow] = (double)acc2[r][c][e];
+                const int krow = kt0 * TK + 64 * wk + 16 * r + 4 * (lane >> 4) + e;
+                if (krow < v.rows) dst[krow] = (double)tot[r][c][e];
             }
         }
 }
 
-__global__ __launch_bounds__(256) void bin_bf16x3_wide_kernel(const BinWork* __restrict__ items) {
+// LOSS items take the 256-thread FP64 body.  With eight waves the upper four end at once, and
+// the barriers of the body then wait for the four that remain (a workgroup barrier counts only
+// waves that have not ended).
+template <int NS>
+__global__ __launch_bounds__(128 * NS) void bin_bf16x3_wide_kernel(
+    const BinWork* __restrict__ items) {
     __shared__ union {
         SmemF64<64> d;
-        SmemWide b;
+        SmemWide<NS> b;
     } sm;
     const BinWork& w = items[blockIdx.x];
-    if (w.kind == CH_LOSS) f64_item<64>(w, sm.d);
-    else wide_item(w, sm.b);
+    if (w.kind == CH_LOSS) {
+        if (threadIdx.x >= 256) return;
+        f64_item<64>(w, sm.d);
+    } else {
+        wide_item<NS>(w, sm.b);
+    }
 }
 
-hipError_t launch_binary_wide(const BinWork* items, int nitems, hipStream_t stream) {
+hipError_t launch_binary_wide(int ns, const BinWork* items, int nitems, hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
-    bin_bf16x3_wide_kernel<<<nitems, 256, 0, stream>>>(items);
+    if (ns == 4) bin_bf16x3_wide_kernel<4><<<nitems, 512, 0, stream>>>(items);
+    else if (ns == 2) bin_bf16x3_wide_kernel<2><<<nitems, 256, 0, stream>>>(items);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

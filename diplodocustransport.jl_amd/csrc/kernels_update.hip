@@ -117,11 +117,10 @@ __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, b
 }
 
 // Copies of a new state value for the next contraction: FP64 state-major (padded states), and
-// the FP32 copy or the BF16 hi/lo split when that precision is in use.
+// the cell-major BF16 hi/lo split when that precision is in use.
 __device__ inline void publish(const UpdateArgs& a, int c, int k, double x) {
     const int kp = a.pad_index[k];
     a.fT[(int64_t)kp * a.ldc + c] = x;
-    if (a.fc32) a.fc32[(int64_t)c * a.ldb + kp] = (float)x;
     if (a.fhi) {
         uint16_t hi, lo;
         split_bf16(x, &hi, &lo);
@@ -565,7 +564,7 @@ __global__ void publish_state_kernel(UpdateArgs a) {
 }
 
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
-                                int Np, double* fT, float* fc32, uint16_t* fhi, uint16_t* flo,
+                                int Np, double* fT, uint16_t* fhi, uint16_t* flo,
                                 hipStream_t stream) {
     UpdateArgs a{};
     a.f = f;
@@ -575,7 +574,6 @@ hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const in
     a.ldb = Np;
     a.pad_index = pad_index;
     a.fT = fT;
-    a.fc32 = fc32;
     a.fhi = fhi;
     a.flo = flo;
     const int64_t total = (int64_t)N * C;

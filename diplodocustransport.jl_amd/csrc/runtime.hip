@@ -97,8 +97,7 @@ struct dtx_handle {
     double* d_f[2] = {nullptr, nullptr};
     int cur = 0;
     double* d_fT = nullptr;       // FP64, state-major, padded states: [Np][ldc]
-    float* d_fc32 = nullptr;      // FP32, cell-major: [ldc][Np] (wide split-BF16 contraction)
-    bool wide = false;            // split-BF16 with 128 x 128 tiles (C > 64)
+    int wide = 0;                 // split-BF16 wide items (C > 64): k tiles per item, 2 or 4
     uint16_t* d_fhi = nullptr;    // BF16 split, cell-major: [ldc][Np]
     uint16_t* d_flo = nullptr;
     double *d_invap = nullptr, *d_dfinj = nullptr, *d_vol = nullptr, *d_finit = nullptr;
@@ -487,10 +486,10 @@ static int build_work(dtx_handle* h) {
     h->correct_active = false;
     const int C = h->C;
     const int ctiles = (C + h->tc - 1) / h->tc;
-    // wide split-BF16 gain items: two k tiles x 128 cells each
+    // wide split-BF16 gain items: h->wide k tiles x WIDE cells each
     const int wctiles = (C + WIDE - 1) / WIDE;
     auto gain_ksteps = [&](const PackedChannel& c) {
-        return h->wide ? (c.nktiles + 1) / 2 : c.nktiles;
+        return h->wide ? (c.nktiles + h->wide - 1) / h->wide : c.nktiles;
     };
     double total = 0;
     h->pairs_total = 0;
@@ -507,7 +506,8 @@ static int build_work(dtx_handle* h) {
         }
         h->gain_bytes += (double)c.a_bytes;
     }
-    // about four resident workgroups per CU (two for the wide items, which hold 68 KB of LDS)
+    // about four resident workgroups per CU; two per CU over both wide shapes (two k tiles:
+    // two workgroups of 68 KB LDS per CU; four k tiles: one of 102 KB)
     const int target_items = h->wide ? 512 : 1024;
     int64_t range = (int64_t)std::ceil(total / target_items);
     range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
@@ -537,9 +537,8 @@ static int build_work(dtx_handle* h) {
                 w.chunks = c.chunks;
                 w.fT = h->d_fT;
                 if (prec == DTX_PREC_FP64) w.B = h->d_fT;
-                else if (wide) w.B = h->d_fc32;
                 else w.B = h->d_fhi;
-                w.b_lo = (bf16 && !loss && !wide) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
+                w.b_lo = (bf16 && !loss) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
                 w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
                 w.ktile = kt;
                 w.c_begin = (int32_t)c0;
@@ -693,7 +692,7 @@ static int finalize_binary(dtx_handle* h) {
 // Publish the current f into the contraction's B-operand copies (padded, per precision).
 static int refresh_fT(dtx_handle* h) {
     HIP_CHECK(h, launch_publish_state(h->d_f[h->cur], h->N, h->C, h->ldc, h->d_pad_index, h->Np,
-                                      h->d_fT, h->d_fc32, h->d_fhi, h->d_flo, h->stream));
+                                      h->d_fT, h->d_fhi, h->d_flo, h->stream));
     return DTX_OK;
 }
 
@@ -790,7 +789,6 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.f = h->d_f[h->cur];
     a.f_out = h->d_f[h->cur ^ 1];
     a.fT = h->d_fT;
-    a.fc32 = h->d_fc32;
     a.fhi = h->d_fhi;
     a.flo = h->d_flo;
     a.pad_index = h->d_pad_index;
@@ -866,7 +864,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     if (bin) {
         int rc = timed_begin(h, 0, &ev);
         if (rc) return rc;
-        if (h->wide) HIP_CHECK(h, launch_binary_wide(h->d_items, h->nitems, h->stream));
+        if (h->wide) HIP_CHECK(h, launch_binary_wide(h->wide, h->d_items, h->nitems, h->stream));
         else HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
         if (h->correct_active) {
             CorrectArgs ca{};
@@ -969,8 +967,9 @@ struct Group {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<dtx_handle*> members;
-    BinWork* d_items[3] = {nullptr, nullptr, nullptr};  // cell tiles of 16 / 64 / wide items
-    int n_items[3] = {0, 0, 0};
+    // cell tiles of 16 / 64, wide items of 2 / 4 k tiles
+    BinWork* d_items[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n_items[4] = {0, 0, 0, 0};
     int precision = 0;
     int32_t* d_block_start = nullptr;
     int total_blocks = 0, max_N = 0;
@@ -984,7 +983,7 @@ struct Group {
 };
 
 static void group_free_tables(Group* g) {
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
         if (g->d_items[q]) (void)hipFree(g->d_items[q]);
         g->d_items[q] = nullptr;
         g->n_items[q] = 0;
@@ -1035,7 +1034,7 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
     if (rc) return rc;
     group_free_tables(g);
     g->members.assign(hs, hs + count);
-    std::vector<BinWork> lists[3];  // cell tiles of 16, of 64, and wide split-BF16 items
+    std::vector<BinWork> lists[4];  // cell tiles of 16, of 64, wide items of 2 and 4 k tiles
     std::vector<int32_t> starts;
     g->total_blocks = 0;
     g->max_N = 0;
@@ -1063,13 +1062,13 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
             g->err = "member stream failed";
             return DTX_ERR_HIP;
         }
-        const int q = h->wide ? 2 : (h->tc == 16 ? 0 : 1);
+        const int q = h->wide ? 1 + h->wide / 2 : (h->tc == 16 ? 0 : 1);
         if (h->binary) lists[q].insert(lists[q].end(), h->items_host.begin(), h->items_host.end());
         starts.push_back(g->total_blocks);
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
         g->max_N = std::max(g->max_N, h->N);
     }
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
         g->n_items[q] = (int)lists[q].size();
         if (!g->n_items[q]) continue;
         if (hipMalloc((void**)&g->d_items[q], lists[q].size() * sizeof(BinWork)) != hipSuccess ||
@@ -1130,7 +1129,8 @@ int group_step(Group* g, const double* dt, int verbose) {
     const int ktc[2] = {16, 64};
     for (int q = 0; q < 2 && e == hipSuccess; ++q)
         e = launch_binary(g->precision, ktc[q], g->d_items[q], g->n_items[q], g->stream);
-    if (e == hipSuccess) e = launch_binary_wide(g->d_items[2], g->n_items[2], g->stream);
+    for (int q = 2; q < 4 && e == hipSuccess; ++q)
+        e = launch_binary_wide(2 * (q - 1), g->d_items[q], g->n_items[q], g->stream);
     if (e == hipSuccess)
         e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
                                       g->max_N, g->stream);
@@ -1198,7 +1198,9 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     h->precision = cfg->precision;
     h->method = cfg->method;
     h->tc = (h->C <= 16) ? 16 : 64;
-    h->wide = h->precision == DTX_PREC_BF16X3 && h->C > 64;
+    // wide split-BF16 items above 64 cells; four k tiles per item from 512 cells on, where the
+    // launch still has enough items for one eight-wave workgroup per CU
+    if (h->precision == DTX_PREC_BF16X3 && h->C > 64) h->wide = h->C >= 512 ? 4 : 2;
     const int cell_align = h->wide ? WIDE : h->tc;
     h->ldc = (h->C + cell_align - 1) / cell_align * cell_align;
     h->sp_pad_off.assign(h->S + 1, 0);
@@ -1228,9 +1230,7 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_f[0], n);
     rc |= dalloc(hp, &h->d_f[1], n);
     rc |= dalloc(hp, &h->d_fT, (int64_t)h->ldc * h->Np);
-    if (h->wide) {
-        rc |= dalloc(hp, &h->d_fc32, (int64_t)h->ldc * h->Np);
-    } else if (h->precision == DTX_PREC_BF16X3) {
+    if (h->precision == DTX_PREC_BF16X3) {
         rc |= dalloc(hp, &h->d_fhi, (int64_t)h->ldc * h->Np);
         rc |= dalloc(hp, &h->d_flo, (int64_t)h->ldc * h->Np);
     }
@@ -1270,8 +1270,6 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     if (!rc) {
         e = hipMemsetAsync(h->d_counter, 0, sizeof(uint32_t), h->stream);
         if (e == hipSuccess) e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * h->Np * 8, h->stream);
-        if (e == hipSuccess && h->d_fc32)
-            e = hipMemsetAsync(h->d_fc32, 0, (size_t)h->ldc * h->Np * 4, h->stream);
         if (e == hipSuccess && h->d_fhi)
             e = hipMemsetAsync(h->d_fhi, 0, (size_t)h->ldc * h->Np * 2, h->stream);
         if (e == hipSuccess && h->d_flo)

@@ -58,8 +58,15 @@ def many_cells():
     return workloads.tiny(p_num=20, u_num=8, x_num=80, with_flux=False, monolithic=False)
 
 
+def wide_slices():
+    """520 cells: the split-BF16 path takes four k tiles per item from 512 cells on (five cell
+    tiles, the last with 8 cells; three k tiles per species, so one slice of each item is
+    absent)."""
+    return workloads.tiny(p_num=20, u_num=8, x_num=520, with_flux=False, monolithic=False)
+
+
 CASES = {"host_channels": homogeneous_channels, "device_channels": synthetic_small,
-         "many_cells": many_cells}
+         "many_cells": many_cells, "wide_slices": wide_slices}
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
