@@ -261,7 +261,7 @@ def main():
         "config": {"workload": f"{w.name}: {DESCRIPTIONS[w.name]}",
                    "solve_steps": w.steps,
                    "solves_per_s": round(base["steps_per_s"] / w.steps, 4),
-                   "kernels_per_step": 3},
+                   "kernels_per_step": 4},
         "precision_modes": prec_report,
         "update_kernel_us": round(base["update_ms"] * 1e3, 2),
         "roofline": roof,

@@ -99,7 +99,8 @@ struct UpdateArgs {
     const int32_t* species_off;    // (S+1)
     double* moments;        // [cell][S][2] (number, energy)
     double* mom_part;       // per-block moment partials [cell][segment][S][2]
-    double* block_part;     // per-block partials [nblocks][8]
+    double* block_part;     // per-block partials [nblocks][BP_WORDS], then the finalize
+                            // fold partials [ceil(max(nblocks, C S) / 256)][BP_WORDS]
     double* status;         // status ring [ring][16]
     uint32_t* counter;      // unused (kept zero)
     // symplectic intermediates
@@ -118,6 +119,7 @@ struct UpdateArgs {
 };
 
 constexpr int UPDATE_STATES = 64;  // states per update workgroup (four threads each)
+constexpr int UPDATE_THREADS = 256;  // threads per update and finalize workgroup
 constexpr int STATUS_RING = 4096;
 constexpr int STATUS_WORDS = 16;  // doubles per status record
 // status record layout

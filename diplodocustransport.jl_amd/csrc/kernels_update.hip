@@ -22,7 +22,7 @@ namespace dtx {
 __device__ constexpr double THR_1F_10 = (double)1e-10f;  // Julia Float32 literal 1f-10
 __device__ constexpr double THR_1F_20 = (double)1e-20f;  // Julia Float32 literal 1f-20
 
-constexpr int UB = 256;  // threads per update block
+constexpr int UB = UPDATE_THREADS;
 
 __device__ inline double strong_zero(double x, bool cond) { return cond ? x : copysign(0.0, x); }
 
@@ -140,9 +140,26 @@ __device__ inline double emission_term(const UpdateArgs& a, int c, int k, const 
 // partial words (sums first, minima after).
 constexpr int RED_VALUES = 2 * DTX_MAX_SPECIES + BP_WORDS;
 
-// Fused workgroup reductions.  Each value is first reduced over its wave by a butterfly
-// (wave_put), and after one barrier thread t combines the four wave results of value t in wave
-// order (wave_combine).  The pattern is fixed, so the reductions are deterministic.
+// Fused workgroup reductions.  Each value is first reduced over its wave by a butterfly, and
+// after one barrier thread t combines the four wave results of value t in wave order
+// (wave_combine).  The pattern is fixed, so the reductions are deterministic.
+//
+// wave_reduce_batch runs the butterflies of WB values at once, with the arithmetic of wave_sum /
+// wave_min (bit i of summask selects the sum for value i) and the shuffles of one level issued
+// together, so a batch costs about the latency of one butterfly.
+constexpr int WB = 8;
+__device__ inline void wave_reduce_batch(double* v, uint32_t summask) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        double x[WB];
+#pragma unroll
+        for (int i = 0; i < WB; ++i) x[i] = __shfl_xor(v[i], m, 64);
+#pragma unroll
+        for (int i = 0; i < WB; ++i)
+            v[i] = ((summask >> i) & 1u) ? v[i] + x[i] : cr_min(v[i], x[i]);
+    }
+}
+
 __device__ inline void wave_put(double x, int i, bool is_sum, double* red) {
     const double w = is_sum ? wave_sum(x) : wave_min(x);
     if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * RED_VALUES + i] = w;
@@ -157,7 +174,7 @@ __device__ inline double wave_combine(int t, bool is_sum, const double* red) {
 
 // Per-species moment partials of this block's new states and the per-block partials (sums and
 // Cr minima), in one fused reduction.  Thread (tid & 3) == 0 of each state holds its values;
-// finalize_step_kernel folds the partials in block order.
+// the finalize kernels fold the partials in a fixed order.
 __device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, bool mine, int k,
                              double fnew, double* red, double sumdf, double sumf, double cr,
                              double crb, double cre, double crf, double crm, double crs) {
@@ -195,10 +212,133 @@ __device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, bool
     }
 }
 
-// One workgroup: folds the per-block partials of the step (block order) into the moments of
-// every (cell, species) and the status record of the step.  Loads are issued in batches ahead
-// of the ordered adds.
-__device__ void finalize_step_body(const UpdateArgs& a, int nseg, double* red) {
+// Finalisation of a step: the per-block partials of the update grid are folded into the
+// moments of every (cell, species) and into the status record.  The fold has two fixed levels,
+// so it runs on many workgroups and is still deterministic: fold group g reduces blocks
+// [UB g, UB g + UB) (one per thread, fixed workgroup pattern) into fold partial g, and the
+// fold partials are then reduced by one workgroup (thread t takes partials t, t + UB, ... in
+// order, then the same workgroup pattern).  The moments of (cell, species) q add the segment
+// partials of the cell in segment order.
+
+// Reduce BP_WORDS values per thread over the workgroup (the sums, then the Cr minima); thread
+// q < BP_WORDS returns word q.
+__device__ inline double wg_reduce_words(const double* v, double* red) {
+    double x[WB];
+#pragma unroll
+    for (int q = 0; q < BP_WORDS; ++q) x[q] = v[q];
+    wave_reduce_batch(x, 0x3u);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < BP_WORDS; ++q) red[(threadIdx.x >> 6) * RED_VALUES + q] = x[q];
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const double r = tid < BP_WORDS ? wave_combine(tid, tid <= BP_SUMF, red) : 0.0;
+    __syncthreads();
+    return r;
+}
+
+__device__ inline void words_identity(double* v) {
+#pragma unroll
+    for (int q = 0; q < BP_WORDS; ++q) v[q] = (q <= BP_SUMF) ? 0.0 : INFINITY;
+}
+
+__device__ inline void words_fold(double* v, const double* p) {
+    v[BP_SUMDF] = v[BP_SUMDF] + p[BP_SUMDF];
+    v[BP_SUMF] = v[BP_SUMF] + p[BP_SUMF];
+#pragma unroll
+    for (int q = BP_CR; q < BP_WORDS; ++q) v[q] = cr_min(v[q], p[q]);
+}
+
+// Number of fold groups (also the number of finalize workgroups of a single simulation).
+__device__ __host__ inline int fold_groups(int C, int S, int nseg) {
+    const int64_t nb = (int64_t)C * nseg, nq = (int64_t)C * S;
+    return (int)(((nb > nq ? nb : nq) + UB - 1) / UB);
+}
+
+// Moments of (cell, species) pair q: segment partials in segment order.
+__device__ inline void fold_moments(const UpdateArgs& a, int nseg, int64_t q) {
+    constexpr int LB = 8;
+    const int64_t cc = q / a.S, sp = q % a.S;
+    double num = 0.0, en = 0.0;
+    for (int g0 = 0; g0 < nseg; g0 += LB) {
+        double pn[LB], pe[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            const int g = g0 + u;
+            const double* mp = a.mom_part + (((cc * nseg + g) * a.S) + sp) * 2;
+            pn[u] = g < nseg ? mp[0] : 0.0;
+            pe[u] = g < nseg ? mp[1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u)
+            if (g0 + u < nseg) {
+                num = num + pn[u];
+                en = en + pe[u];
+            }
+    }
+    a.moments[q * 2 + 0] = num;
+    a.moments[q * 2 + 1] = en;
+}
+
+// Fold group g: moments of pairs [UB g, UB g + UB) and fold partial g of the block partials.
+__device__ inline void fold_group(const UpdateArgs& a, int nseg, int g, double* red) {
+    const int tid = threadIdx.x;
+    const int64_t q = (int64_t)g * UB + tid;
+    if (q < (int64_t)a.C * a.S) fold_moments(a, nseg, q);
+    double v[BP_WORDS];
+    words_identity(v);
+    if (q < (int64_t)a.C * nseg) {
+        double p[BP_WORDS];
+#pragma unroll
+        for (int w = 0; w < BP_WORDS; ++w) p[w] = a.block_part[q * BP_WORDS + w];
+        words_fold(v, p);
+    }
+    const double r = wg_reduce_words(v, red);
+    double* fold = a.block_part + (int64_t)a.C * nseg * BP_WORDS;
+    if (tid < BP_WORDS) fold[(int64_t)g * BP_WORDS + tid] = r;
+}
+
+// Reduce the fold partials and write the status record of the step.
+__device__ inline void fold_status(const UpdateArgs& a, int nseg, int ngroups, double* red) {
+    const int tid = threadIdx.x;
+    const double* fold = a.block_part + (int64_t)a.C * nseg * BP_WORDS;
+    double v[BP_WORDS];
+    words_identity(v);
+    for (int g = tid; g < ngroups; g += UB) {
+        double p[BP_WORDS];
+#pragma unroll
+        for (int w = 0; w < BP_WORDS; ++w) p[w] = fold[(int64_t)g * BP_WORDS + w];
+        words_fold(v, p);
+    }
+    const double r = wg_reduce_words(v, red);
+    if (tid < BP_WORDS) red[tid] = r;
+    __syncthreads();
+    if (tid == 0) {
+        double w[BP_WORDS];
+        for (int q = 0; q < BP_WORDS; ++q) w[q] = red[q];
+        double* st = a.status + (int64_t)(a.step_index % a.ring) * STATUS_WORDS;
+        const bool cr_eval = (a.verbose >= 1 && a.verbose <= 3) && (w[BP_SUMF] != 0.0);
+        st[ST_STEP] = (double)a.step_index;
+        st[ST_FINITE] = isfinite(w[BP_SUMDF]) ? 1.0 : 0.0;
+        st[ST_CR_EVAL] = cr_eval ? 1.0 : 0.0;
+        st[ST_CR] = cr_eval ? -w[BP_CR] : 0.0;
+        st[ST_CR_BIN] = (cr_eval && a.verbose == 3 && a.binary) ? -w[BP_CR_BIN] : 0.0;
+        st[ST_CR_EMI] = (cr_eval && a.verbose == 3 && a.emission) ? -w[BP_CR_EMI] : 0.0;
+        st[ST_CR_FLUX] = (cr_eval && a.verbose == 3) ? -w[BP_CR_FLUX] : 0.0;
+        st[ST_CR_MOM] = (cr_eval && a.verbose == 3) ? -w[BP_CR_MOM] : 0.0;
+        st[ST_CR_SPACE] = (cr_eval && a.verbose == 3) ? -w[BP_CR_SPACE] : 0.0;
+        st[ST_SUMF] = w[BP_SUMF];
+        st[ST_DT] = a.dt;
+        st[ST_TSTART] = a.t_start;
+        st[ST_SAVE] = (double)a.save;
+        st[ST_SUMDF] = w[BP_SUMDF];
+    }
+}
+
+// Small grids (at most FINALIZE_SINGLE_GROUPS fold groups): one workgroup folds the per-block
+// partials (thread-strided, then the workgroup pattern) into the moments of every (cell, species)
+// and the status record of the step.  Loads are issued in batches ahead of the ordered adds.
+__device__ void finalize_single(const UpdateArgs& a, int nseg, double* red) {
     constexpr int LB = 8;
     const int tid = threadIdx.x;
     const int64_t nblocks = (int64_t)a.C * nseg;
@@ -348,9 +488,74 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
     euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
 }
 
-__global__ __launch_bounds__(UB) void finalize_step_kernel(UpdateArgs a, int nseg) {
+// The two levels are two launches: every fold group in parallel, then the final reduction
+// (the kernel boundary makes the fold partials visible to it; a device-wide fence inside one
+// launch would write back the L2 of every XCD and cost more than the second launch).
+__global__ __launch_bounds__(UB) void finalize_fold_kernel(UpdateArgs a, int nseg) {
     __shared__ double red[(UB / 64) * RED_VALUES];
-    finalize_step_body(a, nseg, red);
+    fold_group(a, nseg, blockIdx.x, red);
+}
+
+__global__ __launch_bounds__(UB) void finalize_status_kernel(UpdateArgs a, int nseg, int ngroups) {
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    fold_status(a, nseg, ngroups, red);
+}
+
+// Both levels in one workgroup, with the arithmetic of the two launches: wave w reduces its
+// 64-block slice of each fold group by the butterfly of wave_put, the four wave results of a
+// group are combined in wave order as wave_combine does, and the fold partials then go through
+// fold_status.  Groups are taken SG at a time, so their loads are in flight together.
+constexpr int SG = 4;
+
+__device__ inline void finalize_serial(const UpdateArgs& a, int nseg, double* red, double* wres) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ng = fold_groups(a.C, a.S, nseg);
+    const int64_t nblocks = (int64_t)a.C * nseg;
+    for (int64_t q = tid; q < (int64_t)a.C * a.S; q += UB) fold_moments(a, nseg, q);
+    double* fold = a.block_part + nblocks * BP_WORDS;
+    for (int g0 = 0; g0 < ng; g0 += SG) {
+        double p[SG][BP_WORDS];
+#pragma unroll
+        for (int u = 0; u < SG; ++u) {
+            const int64_t b = (int64_t)(g0 + u) * UB + tid;
+            words_identity(p[u]);
+            if (g0 + u < ng && b < nblocks) {
+                double x[BP_WORDS];
+#pragma unroll
+                for (int w = 0; w < BP_WORDS; ++w) x[w] = a.block_part[b * BP_WORDS + w];
+                words_fold(p[u], x);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SG; ++u) wave_reduce_batch(p[u], 0x3u);
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < SG; ++u)
+#pragma unroll
+                for (int q = 0; q < BP_WORDS; ++q)
+                    wres[(u * (UB / 64) + wave) * BP_WORDS + q] = p[u][q];
+        __syncthreads();
+        if (tid < SG * BP_WORDS && g0 + tid / BP_WORDS < ng) {
+            const int u = tid / BP_WORDS, q = tid % BP_WORDS;
+            double r = wres[(u * (UB / 64)) * BP_WORDS + q];
+            for (int w = 1; w < UB / 64; ++w) {
+                const double x = wres[(u * (UB / 64) + w) * BP_WORDS + q];
+                r = q <= BP_SUMF ? r + x : cr_min(r, x);
+            }
+            fold[(int64_t)(g0 + u) * BP_WORDS + q] = r;
+        }
+        __syncthreads();
+    }
+    fold_status(a, nseg, ng, red);
+}
+
+// Grids with at most this many fold groups are finalised by one workgroup (finalize_single):
+// one short launch instead of two.
+constexpr int FINALIZE_SINGLE_GROUPS = 16;
+
+__global__ __launch_bounds__(UB) void finalize_single_kernel(UpdateArgs a, int nseg) {
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    finalize_single(a, nseg, red);
 }
 
 // Grouped forms for several simulations in one launch (serving): block b of the update grid
@@ -375,10 +580,16 @@ __global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs
     euler_update_body(a, local / nseg, local % nseg, nseg, smem, red, ired);
 }
 
+// One workgroup per simulation, with the arithmetic the simulation gets when stepped alone
+// (finalize_single for small grids, the serial form of the two-level fold otherwise), so its
+// status and moments do not depend on whether it was stepped alone or in a group.
 __global__ __launch_bounds__(UB) void finalize_group_kernel(const UpdateArgs* table) {
     __shared__ double red[(UB / 64) * RED_VALUES];
+    __shared__ double wres[SG * (UB / 64) * BP_WORDS];
     const UpdateArgs a = table[blockIdx.x];
-    finalize_step_body(a, (a.N + UPDATE_STATES - 1) / UPDATE_STATES, red);
+    const int nseg = (a.N + UPDATE_STATES - 1) / UPDATE_STATES;
+    if (fold_groups(a.C, a.S, nseg) <= FINALIZE_SINGLE_GROUPS) finalize_single(a, nseg, red);
+    else finalize_serial(a, nseg, red, wres);
 }
 
 // ------------------------------------------------------------- ForwardSymplecticEuler ------
@@ -515,11 +726,11 @@ __global__ __launch_bounds__(UB) void gain_correct_kernel(CorrectArgs a) {
             const double* p = a.part + (int64_t)slot * a.part_stride + base;
             for (int r = r0 + tid; r < r1; r += UB) rate[m] = rate[m] + p[r];
         }
-    wave_put(s0, 0, true, red);
-    wave_put(s1, 1, true, red);
-    wave_put(s2, 2, true, red);
-    wave_put(rate[0], 3, true, red);
-    wave_put(rate[1], 4, true, red);
+    double v[WB] = {s0, s1, s2, rate[0], rate[1], 0.0, 0.0, 0.0};
+    wave_reduce_batch(v, 0xFFu);
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) red[(tid >> 6) * RED_VALUES + i] = v[i];
     __syncthreads();
     const double r = tid < 5 ? wave_combine(tid, true, red) : 0.0;
     __syncthreads();
@@ -597,7 +808,15 @@ static dim3 update_grid(const UpdateArgs& a) {
 }
 
 static hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream) {
-    finalize_step_kernel<<<1, UB, 0, stream>>>(a, (int)update_grid(a).y);
+    const int nseg = (int)update_grid(a).y, ng = fold_groups(a.C, a.S, nseg);
+    if (ng <= FINALIZE_SINGLE_GROUPS) {
+        finalize_single_kernel<<<1, UB, 0, stream>>>(a, nseg);
+        return hipGetLastError();
+    }
+    finalize_fold_kernel<<<ng, UB, 0, stream>>>(a, nseg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    finalize_status_kernel<<<1, UB, 0, stream>>>(a, nseg, ng);
     return hipGetLastError();
 }
 

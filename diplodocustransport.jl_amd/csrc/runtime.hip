@@ -1244,7 +1244,10 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_species_off, h->S + 1);
     rc |= dalloc(hp, &h->d_moments, C * h->S * 2);
     const int64_t nseg = (N + UPDATE_STATES - 1) / UPDATE_STATES;  // update workgroups per cell
-    rc |= dalloc(hp, &h->d_block_part, C * nseg * BP_WORDS);
+    // per-block partials, then the fold partials of the finalize kernel
+    const int64_t nfold =
+        (std::max(C * nseg, C * (int64_t)h->S) + UPDATE_THREADS - 1) / UPDATE_THREADS;
+    rc |= dalloc(hp, &h->d_block_part, (C * nseg + nfold) * BP_WORDS);
     rc |= dalloc(hp, &h->d_mom_part, C * nseg * h->S * 2);
     rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
     rc |= dalloc(hp, &h->d_counter, 1);

@@ -50,3 +50,19 @@ def test_number_conserved_per_request():
     for s, r, f in zip(specs, res, states):
         assert abs(r.number_total - f.sum()) <= 1e-12 * abs(f.sum())
         assert abs(f.sum() / (s.species * s.x_num) - 1.0) < 0.05
+
+
+def test_large_request_in_group():
+    """A request large enough for the two-level fold of the step partials (more than 16 fold
+    groups: 520 cells x 8 update segments), run alone and next to a small one: same state, and
+    the device moments still sum the state."""
+    big = serving.RequestSpec(p_num=32, u_num=8, h_num=1, x_num=520, species=2, steps=12,
+                              seed=11)
+    small = small_mix()[0]
+    mixed, _, f_mixed = serving.run([big, small], precision="fp64", realtime=False, fetch=True)
+    alone, _, f_alone = serving.run([big], precision="fp64", realtime=False, fetch=True)
+    np.testing.assert_array_equal(f_alone[0], f_mixed[0])
+    assert alone[0].number_total == mixed[0].number_total
+    f = f_alone[0]
+    assert abs(alone[0].number_total - f.sum()) <= 1e-12 * abs(f.sum())
+
