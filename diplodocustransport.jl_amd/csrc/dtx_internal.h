@@ -24,7 +24,9 @@ constexpr int MAX_RANGE = 1024;    // most chunks one work item covers (its tabl
 
 // Channel kinds: TRI = one species with itself, pairs i <= j; RECT = s1 < s2, all (i, j);
 // LOSS = rows of one species against all states, multiplied by the row's own f at the end.
-enum ChannelKind : int32_t { CH_TRI = 0, CH_RECT = 1, CH_LOSS = 2 };
+// CH_MOMENT items are LOSS-layout FP64 items of a moment-rate matrix W: they write, per cell,
+// the sum over their rows of f_row (W f)_row instead of the rows themselves.
+enum ChannelKind : int32_t { CH_TRI = 0, CH_RECT = 1, CH_LOSS = 2, CH_MOMENT = 3 };
 
 // A work item of the grouped contraction kernel (one workgroup).  Items of several problems
 // (serving) can be mixed in one launch; everything an item needs is reachable from it.
@@ -36,6 +38,7 @@ struct BinWork {
                               // BF16 -> hi bits, cell-major: B[cell * ldb + kp]
     int64_t b_lo;             // BF16: elements from the hi to the lo array
     double* part;             // partial sums of this slot: part[cell * N + row]
+                              // (CH_MOMENT: the item's per-cell sums, part[cell])
     int64_t a_ktile_stride;   // elements between consecutive k-tiles of A
     int32_t ktile;            // k-tile inside the channel
     int32_t ctile;            // first cell of the tile = ctile * TC
@@ -133,8 +136,8 @@ enum BlockSlot { BP_SUMDF = 0, BP_SUMF, BP_CR, BP_CR_BIN, BP_CR_EMI, BP_CR_FLUX,
 
 // FP64 moment-rate correction of a reduced-precision gain term, one workgroup per (cell, output
 // species s).  dBg = sum of the species' gain slots, dBl = sum of its loss slots; R_N and R_E are
-// the exact FP64 number and energy rates of the gain term (sums of the moment-rate slots listed
-// for (s, moment) in mom_off / mom_list).  The corrected binary sum dbsum = dBg (1 + a + b E) +
+// the exact FP64 number and energy rates of the gain term (sums of the moment-rate item sums
+// listed for (s, moment) in mom_off / mom_list).  The corrected binary sum dbsum = dBg (1 + a + b E) +
 // dBl, with (a, b) chosen so that sum dBg' = R_N and sum E dBg' = R_E.
 struct CorrectArgs {
     const double* part;        // partial slots [slot][cell][N]
@@ -144,9 +147,10 @@ struct CorrectArgs {
     const int32_t* gslots;     // gain slots per species (S)
     const int32_t* nslots;     // gain + loss slots per species (S)
     const double* eweight;     // per-state energy weight (N) or null (number only)
+    const double* mpart;       // moment-rate item sums [index][ldc]
     const int32_t* mom_off;    // (2S+1) offsets into mom_list for (s, moment)
-    const int32_t* mom_list;   // triples (slot, row begin, row end)
-    int32_t N, C, S;
+    const int32_t* mom_list;   // mpart indices, in summation order
+    int32_t N, C, S, ldc;
 };
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------

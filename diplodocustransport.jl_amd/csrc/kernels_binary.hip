@@ -146,7 +146,7 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     const GLB double* A = (const GLB double*)v.A + (int64_t)v.c_begin * KC * TK;
     const GLB double* B = (const GLB double*)v.B;
     const int bq = tid / BT, bcc = (tid % BT) * BP;
-    const bool loss = v.kind == CH_LOSS;
+    const bool loss = v.kind >= CH_LOSS;   // B = f_j; LOSS and MOMENT rows take their own f
     load_ctab(v, ctab);
 
     double ra[AP], rb[BP], rf[BP];
@@ -208,6 +208,39 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
         if (more) store_stage(buf ^ 1);
         __syncthreads();
         buf ^= 1;
+    }
+    if (v.kind == CH_MOMENT) {
+        // per cell, the sum over the item's rows of f_row (W f)_row: each lane adds its rows of
+        // the column (tiles r, elements e in order), the four lanes of a column combine by
+        // butterfly, and the WK waves of a column in wave order through LDS (As is free now)
+        double cs[T::MC];
+#pragma unroll
+        for (int c = 0; c < T::MC; ++c) {
+            const int cell = v.c0 + wc * T::WCOLS + c * 16 + (lane & 15);
+            cs[c] = 0.0;
+#pragma unroll
+            for (int r = 0; r < T::MR; ++r)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int krow = v.krow0 + tile_row<4>(wk * T::WROWS + r * 16, lane, e);
+                    if (krow < v.rows && cell < v.cells)
+                        cs[c] = cs[c] +
+                                acc[r][c][e] * v.fT[(int64_t)(v.row_pad_base + krow) * v.ldc + cell];
+                }
+            cs[c] = cs[c] + __shfl_xor(cs[c], 16, 64);
+            cs[c] = cs[c] + __shfl_xor(cs[c], 32, 64);
+        }
+        double* msum = &As[0][0];
+        if (lane < 16)
+#pragma unroll
+            for (int c = 0; c < T::MC; ++c) msum[wk * TC + wc * T::WCOLS + c * 16 + lane] = cs[c];
+        __syncthreads();
+        if (tid < TC && v.c0 + tid < v.cells) {
+            double t = msum[tid];
+            for (int k = 1; k < T::WK; ++k) t = t + msum[k * TC + tid];
+            v.part[v.c0 + tid] = t;
+        }
+        return;
     }
 #pragma unroll
     for (int r = 0; r < T::MR; ++r)
@@ -341,7 +374,7 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
         SmemF32<TC> s;
     } sm;
     const BinWork& w = items[blockIdx.x];
-    if (w.kind == CH_LOSS) f64_item<TC>(w, sm.d);
+    if (w.kind >= CH_LOSS) f64_item<TC>(w, sm.d);
     else f32_item<TC>(w, sm.s);
 }
 
@@ -480,7 +513,7 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
         SmemBF16<TC> b;
     } sm;
     const BinWork& w = items[blockIdx.x];
-    if (w.kind == CH_LOSS) f64_item<TC>(w, sm.d);
+    if (w.kind >= CH_LOSS) f64_item<TC>(w, sm.d);
     else bf16x3_item<TC>(w, sm.b);
 }
 
@@ -649,7 +682,7 @@ __global__ __launch_bounds__(128 * NS) void bin_bf16x3_wide_kernel(
         SmemWide<NS> b;
     } sm;
     const BinWork& w = items[blockIdx.x];
-    if (w.kind == CH_LOSS) {
+    if (w.kind >= CH_LOSS) {
         if (threadIdx.x >= 256) return;
         f64_item<64>(w, sm.d);
     } else {

@@ -717,14 +717,22 @@ __global__ __launch_bounds__(UB) void gain_correct_kernel(CorrectArgs a) {
         s1 = s1 + e * gv;
         s2 = s2 + (e * e) * gv;
     }
+    // R_N, R_E of this cell: the listed moment-rate item sums in list order (thread 0; every
+    // other thread adds zeros, which leaves the sum exact through the reduction below)
     double rate[2] = {0.0, 0.0};
+    if (tid == 0)
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
-        for (int t = a.mom_off[2 * s + m]; t < a.mom_off[2 * s + m + 1]; ++t) {
-            const int slot = a.mom_list[3 * t];
-            const int r0 = a.mom_list[3 * t + 1], r1 = a.mom_list[3 * t + 2];
-            const double* p = a.part + (int64_t)slot * a.part_stride + base;
-            for (int r = r0 + tid; r < r1; r += UB) rate[m] = rate[m] + p[r];
+        for (int m = 0; m < 2; ++m) {
+            const int t0 = a.mom_off[2 * s + m], t1 = a.mom_off[2 * s + m + 1];
+            for (int t = t0; t < t1; t += LB) {
+                double v[LB];
+#pragma unroll
+                for (int u = 0; u < LB; ++u)
+                    v[u] = t + u < t1 ? a.mpart[(int64_t)a.mom_list[t + u] * a.ldc + c] : 0.0;
+#pragma unroll
+                for (int u = 0; u < LB; ++u)
+                    if (t + u < t1) rate[m] = rate[m] + v[u];
+            }
         }
     double v[WB] = {s0, s1, s2, rate[0], rate[1], 0.0, 0.0, 0.0};
     wave_reduce_batch(v, 0xFFu);

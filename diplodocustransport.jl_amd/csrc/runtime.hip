@@ -88,6 +88,8 @@ struct dtx_handle {
     std::vector<BinWork> items_host;  // host copy (grouped launches)
     int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0, part_slots = 0;
     double* d_part = nullptr;
+    double* d_mpart = nullptr;    // per-cell sums of the moment-rate items [n_mpart][ldc]
+    int32_t n_mpart = 0;
     int64_t part_stride = 0;
     int32_t* d_nslot = nullptr;
     int64_t pairs_total = 0;
@@ -480,7 +482,9 @@ static int build_work(dtx_handle* h) {
     dfree(h, &h->d_gslots, h->S);
     dfree(h, &h->d_nslots_s, h->S);
     dfree(h, &h->d_mom_off, 2 * h->S + 1);
-    dfree(h, &h->d_mom_list, 3 * h->n_mom_list);
+    dfree(h, &h->d_mom_list, h->n_mom_list);
+    dfree(h, &h->d_mpart, (int64_t)h->n_mpart * h->ldc);
+    h->n_mpart = 0;
     h->nitems = 0;
     h->max_slots = h->part_slots = 0;
     h->correct_active = false;
@@ -550,7 +554,7 @@ static int build_work(dtx_handle* h) {
                 w.ldc = h->ldc;
                 w.ldb = h->Np;
                 w.N = h->N;
-                w.kind = c.kind;
+                w.kind = c.moment >= 0 ? CH_MOMENT : c.kind;
                 out.push_back({w, slot, wide ? wctiles : ctiles});
             }
         }
@@ -563,19 +567,24 @@ static int build_work(dtx_handle* h) {
         if (pass == 0) gain_slots = next_slot;  // gain slots of species s: [0, gain_slots[s])
     }
     for (int s = 0; s < h->S; ++s) h->max_slots = std::max(h->max_slots, next_slot[s]);
-    // moment-rate slots: one per (moment channel, chunk range), listed per (species, moment)
-    int n_moment = 0;
+    // moment-rate items write per-cell sums: one row of ldc values per (moment channel, chunk
+    // range, k tile), listed per (species, moment) in summation order
+    int n_mpart = 0;
     std::vector<std::vector<int32_t>> mom(2 * h->S);
     for (auto& c : h->mchans)
         add_units(c, loss_range, moment_units, [&]() {
-            const int slot = h->max_slots + n_moment++;
+            const int base = n_mpart;
+            n_mpart += c.nktiles;
             auto& l = mom[2 * c.s_out + c.moment];
-            l.push_back(slot);
-            l.push_back(h->sp_off[c.s1]);
-            l.push_back(h->sp_off[c.s1] + c.N1);
-            return slot;
+            for (int kt = 0; kt < c.nktiles; ++kt) l.push_back(base + kt);
+            return base;
         });
-    h->part_slots = h->max_slots + n_moment;
+    h->part_slots = h->max_slots;
+    h->n_mpart = n_mpart;
+    if (n_mpart > 0) {
+        int rc = dalloc(h, &h->d_mpart, (int64_t)n_mpart * h->ldc);
+        if (rc) return rc;
+    }
     if (h->part_slots > 0) {
         int rc = dalloc(h, &h->d_part, h->part_stride * h->part_slots);
         if (rc) return rc;
@@ -599,7 +608,9 @@ static int build_work(dtx_handle* h) {
                 if (ct >= order[x]->nct) continue;
                 BinWork w = order[x]->w;
                 w.ctile = ct;
-                w.part = h->d_part + (int64_t)order[x]->slot * h->part_stride;
+                w.part = w.kind == CH_MOMENT
+                             ? h->d_mpart + (int64_t)(order[x]->slot + w.ktile) * h->ldc
+                             : h->d_part + (int64_t)order[x]->slot * h->part_stride;
                 items.push_back(w);
             }
     }
@@ -620,11 +631,11 @@ static int build_work(dtx_handle* h) {
     // correction tables
     std::vector<int32_t> ones(h->N, 1), moff(2 * h->S + 1, 0), mlist;
     for (int q = 0; q < 2 * h->S; ++q) {
-        moff[q] = (int32_t)(mlist.size() / 3);
+        moff[q] = (int32_t)mlist.size();
         mlist.insert(mlist.end(), mom[q].begin(), mom[q].end());
     }
-    moff[2 * h->S] = (int32_t)(mlist.size() / 3);
-    h->n_mom_list = (int32_t)(mlist.size() / 3);
+    moff[2 * h->S] = (int32_t)mlist.size();
+    h->n_mom_list = (int32_t)mlist.size();
     h->max_ns = 0;
     for (int s = 0; s < h->S; ++s) h->max_ns = std::max(h->max_ns, h->sp_size[s]);
     std::vector<int32_t> nsl(next_slot.begin(), next_slot.end());
@@ -633,12 +644,12 @@ static int build_work(dtx_handle* h) {
     if (!rc) rc = dalloc(h, &h->d_gslots, h->S);
     if (!rc) rc = dalloc(h, &h->d_nslots_s, h->S);
     if (!rc) rc = dalloc(h, &h->d_mom_off, 2 * h->S + 1);
-    if (!rc) rc = dalloc(h, &h->d_mom_list, 3 * h->n_mom_list);
+    if (!rc) rc = dalloc(h, &h->d_mom_list, h->n_mom_list);
     if (!rc) rc = upload(h, h->d_ones, ones.data(), h->N);
     if (!rc) rc = upload(h, h->d_gslots, gain_slots.data(), h->S);
     if (!rc) rc = upload(h, h->d_nslots_s, nsl.data(), h->S);
     if (!rc) rc = upload(h, h->d_mom_off, moff.data(), 2 * h->S + 1);
-    if (!rc) rc = upload(h, h->d_mom_list, mlist.data(), 3 * h->n_mom_list);
+    if (!rc) rc = upload(h, h->d_mom_list, mlist.data(), h->n_mom_list);
     if (rc) return rc;
     h->correct_active = true;
     return DTX_OK;
@@ -875,11 +886,13 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             ca.gslots = h->d_gslots;
             ca.nslots = h->d_nslots_s;
             ca.eweight = h->d_eweight;
+            ca.mpart = h->d_mpart;
             ca.mom_off = h->d_mom_off;
             ca.mom_list = h->d_mom_list;
             ca.N = h->N;
             ca.C = h->C;
             ca.S = h->S;
+            ca.ldc = h->ldc;
             HIP_CHECK(h, launch_gain_correct(ca, h->max_ns, h->stream));
         }
         rc = timed_end(h, &ev);
