@@ -42,7 +42,7 @@ DESCRIPTIONS = {
 # HBM bytes per launch of the FP64 contraction from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (separate runs, each scaled by the same counter over a 1 GiB device copy):
 # profiles/transport_1d_hbm_traffic.txt.
-TRAFFIC = {("transport_1d", "bin_f64_kernel<64>"): 882_940_000}
+TRAFFIC = {("transport_1d", "bin_f64_wide_kernel"): 503_160_000}
 
 
 def log(*a):
@@ -223,7 +223,9 @@ def main():
     # The contraction is MFMA-bound when its arithmetic intensity exceeds the FP64 ridge
     # (78.6 TFLOP/s / 8 TB/s), else HBM-bound (single-cell boxes stream the whole operator).
     ai = base["bin_flops"] / base["bin_bytes"]
-    kernel = f"bin_f64_kernel<{16 if w.ps.n_space <= 16 else 64}>"
+    C = w.ps.n_space
+    kernel = ("bin_f64_kernel<16>" if C <= 16 else
+              "bin_f64_wide_kernel" if 64 < C <= 256 else "bin_f64_kernel<64>")
     if ai >= PEAK_TFLOPS["fp64"] * 1e12 / (PEAK_HBM_GBS * 1e9):
         achieved = base["bin_flops"] / (base["bin_ms"] * 1e-3) / 1e12
         roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["fp64"],

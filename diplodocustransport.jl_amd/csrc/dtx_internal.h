@@ -157,10 +157,11 @@ struct CorrectArgs {
 hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream);
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
-// Split-BF16 contraction with (64 ns) x WIDE (rows x cells) tiles for C > 64, ns = 2 or 4
-// k tiles per item: gain items are wide, LOSS-layout items keep the 64 x 64 FP64 body.
+// Contraction with (64 ns) x WIDE (rows x cells) gain items for C > 64: split-BF16 with ns = 2
+// or 4 k tiles per item, FP64 with ns = 2.  LOSS-layout items keep the 64 x 64 FP64 body.
 constexpr int WIDE = 128;   // cells per wide item
-hipError_t launch_binary_wide(int ns, const BinWork* items, int nitems, hipStream_t stream);
+hipError_t launch_binary_wide(int precision, int ns, const BinWork* items, int nitems,
+                              hipStream_t stream);
 hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,
                             int rows, int nktiles, double* A, hipStream_t stream);
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);

@@ -690,9 +690,133 @@ __global__ __launch_bounds__(128 * NS) void bin_bf16x3_wide_kernel(
     }
 }
 
-hipError_t launch_binary_wide(int ns, const BinWork* items, int nitems, hipStream_t stream) {
+
+// ------------------------------------------------------------------------- wide FP64 -----
+// FP64 gain items for C > 64: two packed 64-row slices (k tiles 2w and 2w+1) x 128 cells, four
+// waves of 64 x 64 (16 tiles of v_mfma_f64_16x16x4_f64 each), so every staged byte feeds twice
+// the MFMAs of a 64 x 64 item.  B' = f_i f_j is formed while staging, as in f64_item.  Both
+// tiles sit in LDS in MFMA operand order: one 64-double block per (k step of 4, 16 rows or 16
+// cells), element l of a block being what lane l supplies (A[row l % 16][k l / 16],
+// B[k l / 16][cell l % 16]), so every LDS store and operand read of a wave is one linear
+// 512-byte block.
+struct SmemWide64 {
+    static constexpr int HK = KC / 2;   // j per LDS stage
+    double As[2][HK * 2 * TK];          // block ks * 8 + row block (0..7)
+    double Bs[2][HK * WIDE];            // block ks * 8 + cell block (0..7)
+    uint32_t ctab[MAX_RANGE];
+};
+
+__device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64& sm) {
+    constexpr int HK = SmemWide64::HK;
+    auto& As = sm.As;
+    auto& Bs = sm.Bs;
+    uint32_t* ctab = sm.ctab;
+    const ItemView v = load_item(w, WIDE, 8);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wk = wave & 1, wc = wave >> 1;    // this wave: rows 64 wk.., cells 64 wc..
+    // Element q 256 + tid (q = 0..7) of a stage tile is in block 4 q + wave = 8 ks + b with
+    // ks = q / 2 and b = 4 (q % 2) + wave, at lane `lane`; its k is 4 ks + lane / 16.  A: row
+    // block b is slice q % 2, rows 16 wave + lane % 16 of the slice.  B: cell 16 b + lane % 16.
+    // A slice beyond the channel's rows (odd k-tile count) reads slice 0 again, and the rows it
+    // feeds are never written.
+    const int64_t slice = w.a_ktile_stride;   // doubles per packed 64-row slice
+    const int kt0 = 2 * w.ktile;
+    const bool has1 = (kt0 + 1) * TK < v.rows;
+    const int kl = lane >> 4;
+    const GLB double* A0 = (const GLB double*)w.A + (int64_t)kt0 * slice +
+                           (int64_t)v.c_begin * KC * TK + 16 * wave + (lane & 15);
+    const GLB double* A1 = has1 ? A0 + slice : A0;
+    const int cell0 = 16 * wave + (lane & 15), cell1 = cell0 + 64;   // q even, q odd
+    load_ctab(v, ctab);
+
+    double ra[8], rb[8], rf0 = 0.0, rf1 = 0.0;
+    // stage st = half (st & 1) of local chunk (st >> 1)
+    auto load_stage = [&](int st) {
+        const int ch = st >> 1, h = st & 1;
+        const uint32_t ce = ctab[ch];
+        const int64_t ao = (int64_t)ch * KC * TK + (int64_t)(h * HK + kl) * TK;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ra[q] = ((q & 1) ? A1 : A0)[ao + 4 * (q / 2) * TK];
+        const GLB double* b = (const GLB double*)v.B +
+                              (int64_t)((int)(ce >> 16) + h * HK + kl) * v.ldc + v.c0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            rb[q] = b[(int64_t)(4 * (q / 2)) * v.ldc + ((q & 1) ? cell1 : cell0)];
+        const GLB double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0;
+        rf0 = fi[cell0];
+        rf1 = fi[cell1];
+    };
+    auto store_stage = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) As[buf][q * 256 + tid] = ra[q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Bs[buf][q * 256 + tid] = ((q & 1) ? rf1 : rf0) * rb[q];
+    };
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+    const int nst = 2 * (v.c_end - v.c_begin);
+    if (nst > 0) {
+        load_stage(0);
+        store_stage(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int st = 0; st < nst; ++st) {
+        const bool more = st + 1 < nst;
+        if (more) load_stage(st + 1);
+#pragma unroll
+        for (int ks = 0; ks < HK / 4; ++ks) {
+            double a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = As[buf][(8 * ks + 4 * wk + r) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = Bs[buf][(8 * ks + 4 * wc + c) * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
+        if (more) store_stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // write the item's partial sums (FP64 result layout D[(l / 16) + 4 e][l % 16])
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int cell = v.c0 + 64 * wc + 16 * c + (lane & 15);
+            if (cell >= v.cells) continue;
+            GLB double* dst = v.part + (int64_t)cell * v.N + v.row_base;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int krow = kt0 * TK + 64 * wk + 16 * r + (lane >> 4) + 4 * e;
+                if (krow < v.rows) dst[krow] = acc[r][c][e];
+            }
+        }
+}
+
+__global__ __launch_bounds__(256) void bin_f64_wide_kernel(const BinWork* __restrict__ items) {
+    __shared__ union {
+        SmemF64<64> d;
+        SmemWide64 w;
+    } sm;
+    const BinWork& w = items[blockIdx.x];
+    if (w.kind >= CH_LOSS) f64_item<64>(w, sm.d);
+    else wide64_item(w, sm.w);
+}
+
+hipError_t launch_binary_wide(int precision, int ns, const BinWork* items, int nitems,
+                              hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
-    if (ns == 4) bin_bf16x3_wide_kernel<4><<<nitems, 512, 0, stream>>>(items);
+    if (precision == DTX_PREC_FP64 && ns == 2) bin_f64_wide_kernel<<<nitems, 256, 0, stream>>>(items);
+    else if (precision != DTX_PREC_BF16X3) return hipErrorInvalidValue;
+    else if (ns == 4) bin_bf16x3_wide_kernel<4><<<nitems, 512, 0, stream>>>(items);
     else if (ns == 2) bin_bf16x3_wide_kernel<2><<<nitems, 256, 0, stream>>>(items);
     else return hipErrorInvalidValue;
     return hipGetLastError();

@@ -99,7 +99,8 @@ struct dtx_handle {
     double* d_f[2] = {nullptr, nullptr};
     int cur = 0;
     double* d_fT = nullptr;       // FP64, state-major, padded states: [Np][ldc]
-    int wide = 0;                 // split-BF16 wide items (C > 64): k tiles per item, 2 or 4
+    int wide = 0;                 // wide gain items (C > 64): k tiles per item (BF16x3 2 or 4,
+                                  // FP64 2), 0 for 64-cell tiles
     uint16_t* d_fhi = nullptr;    // BF16 split, cell-major: [ldc][Np]
     uint16_t* d_flo = nullptr;
     double *d_invap = nullptr, *d_dfinj = nullptr, *d_vol = nullptr, *d_finit = nullptr;
@@ -490,7 +491,7 @@ static int build_work(dtx_handle* h) {
     h->correct_active = false;
     const int C = h->C;
     const int ctiles = (C + h->tc - 1) / h->tc;
-    // wide split-BF16 gain items: h->wide k tiles x WIDE cells each
+    // wide gain items: h->wide k tiles x WIDE cells each
     const int wctiles = (C + WIDE - 1) / WIDE;
     auto gain_ksteps = [&](const PackedChannel& c) {
         return h->wide ? (c.nktiles + h->wide - 1) / h->wide : c.nktiles;
@@ -875,7 +876,8 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     if (bin) {
         int rc = timed_begin(h, 0, &ev);
         if (rc) return rc;
-        if (h->wide) HIP_CHECK(h, launch_binary_wide(h->wide, h->d_items, h->nitems, h->stream));
+        if (h->wide)
+            HIP_CHECK(h, launch_binary_wide(h->precision, h->wide, h->d_items, h->nitems, h->stream));
         else HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
         if (h->correct_active) {
             CorrectArgs ca{};
@@ -1143,7 +1145,7 @@ int group_step(Group* g, const double* dt, int verbose) {
     for (int q = 0; q < 2 && e == hipSuccess; ++q)
         e = launch_binary(g->precision, ktc[q], g->d_items[q], g->n_items[q], g->stream);
     for (int q = 2; q < 4 && e == hipSuccess; ++q)
-        e = launch_binary_wide(2 * (q - 1), g->d_items[q], g->n_items[q], g->stream);
+        e = launch_binary_wide(g->precision, 2 * (q - 1), g->d_items[q], g->n_items[q], g->stream);
     if (e == hipSuccess)
         e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
                                       g->max_N, g->stream);
@@ -1211,9 +1213,13 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     h->precision = cfg->precision;
     h->method = cfg->method;
     h->tc = (h->C <= 16) ? 16 : 64;
-    // wide split-BF16 items above 64 cells; four k tiles per item from 512 cells on, where the
-    // launch still has enough items for one eight-wave workgroup per CU
+    // wide gain items above 64 cells: split-BF16 with four k tiles per item from 512 cells on,
+    // where the launch still has enough items for one eight-wave workgroup per CU.  FP64 with
+    // two k tiles up to 256 cells: with many cells the 64-cell items, four workgroups per CU,
+    // measured faster (the wide FP64 item holds two), and with few the halved operator reads win
+    // (the FP32 contraction keeps 64-cell tiles)
     if (h->precision == DTX_PREC_BF16X3 && h->C > 64) h->wide = h->C >= 512 ? 4 : 2;
+    if (h->precision == DTX_PREC_FP64 && h->C > 64 && h->C <= 256) h->wide = 2;
     const int cell_align = h->wide ? WIDE : h->tc;
     h->ldc = (h->C + cell_align - 1) / cell_align * cell_align;
     h->sp_pad_off.assign(h->S + 1, 0);

@@ -136,6 +136,15 @@ def test_medium_transport_parity_fp64():
         assert rel_linf(out.f[i], fo[i]) < TOL64, i
 
 
+def test_many_cells_parity_fp64():
+    # 130 cells: the FP64 gain items take two k tiles x 128 cells (the second cell tile holds 2
+    # cells); 2 species x 96 states, so the second k tile of each item is partial
+    w = workloads.transport_1d(p_num=12, u_num=8, x_num=130, steps=12, monolithic=False)
+    m, out, fo, to, _ = _solve_both(w, n_save=2, mono=False)
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, i
+
+
 @pytest.mark.parametrize("path", __import__("fixture_util").fixtures(), ids=lambda p: p.stem)
 def test_golden_fixtures_fp64(path):
     """The HIP path against the committed golden outputs (oracle + brute force agreed on them)."""
