@@ -66,3 +66,17 @@ def test_large_request_in_group():
     f = f_alone[0]
     assert abs(alone[0].number_total - f.sum()) <= 1e-12 * abs(f.sum())
 
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_wide_items_in_group(precision):
+    """128 cells: FP64 gain items take two k tiles x 128 cells; alone and mixed with a
+    64-cell-tile request the state is the same."""
+    req = serving.RequestSpec(p_num=16, u_num=8, h_num=1, x_num=128, species=2, steps=10,
+                              seed=5)
+    small = small_mix()[1]
+    mixed, _, f_mixed = serving.run([req, small], precision=precision, realtime=False,
+                                    fetch=True)
+    alone, _, f_alone = serving.run([req], precision=precision, realtime=False, fetch=True)
+    np.testing.assert_array_equal(f_alone[0], f_mixed[0])
+    assert alone[0].number_total == mixed[0].number_total
+
