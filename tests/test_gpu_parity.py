@@ -136,10 +136,13 @@ def test_medium_transport_parity_fp64():
         assert rel_linf(out.f[i], fo[i]) < TOL64, i
 
 
-def test_many_cells_parity_fp64():
-    # 130 cells: the FP64 gain items take two k tiles x 128 cells (the second cell tile holds 2
-    # cells); 2 species x 96 states, so the second k tile of each item is partial
-    w = workloads.transport_1d(p_num=12, u_num=8, x_num=130, steps=12, monolithic=False)
+@pytest.mark.parametrize("shape", [(12, 8, 130), (8, 4, 100)], ids=["2ktiles", "1ktile"])
+def test_many_cells_parity_fp64(shape):
+    # more than 64 cells: the FP64 gain items take two k tiles x 128 cells.  (12, 8, 130): 96
+    # states per species, so the second k tile of each item is partial, and the second cell
+    # tile holds 2 cells; (8, 4, 100): 32 states per species, so the second k tile is absent
+    p_num, u_num, x_num = shape
+    w = workloads.transport_1d(p_num=p_num, u_num=u_num, x_num=x_num, steps=12, monolithic=False)
     m, out, fo, to, _ = _solve_both(w, n_save=2, mono=False)
     for i in range(len(out.f)):
         assert rel_linf(out.f[i], fo[i]) < TOL64, i
