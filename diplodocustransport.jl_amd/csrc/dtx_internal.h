@@ -105,7 +105,6 @@ struct UpdateArgs {
     double* block_part;     // per-block partials [nblocks][BP_WORDS], then the finalize
                             // fold partials [ceil(max(nblocks, C S) / 256)][BP_WORDS]
     double* status;         // status ring [ring][16]
-    uint32_t* counter;      // unused (kept zero)
     // symplectic intermediates
     double* f_mom;          // f after the momentum update
     double* df_mom;         // df_Momentum (scaled)

@@ -108,7 +108,6 @@ struct dtx_handle {
     int32_t *d_state_species = nullptr, *d_species_off = nullptr;
     double *d_moments = nullptr, *d_mom_part = nullptr, *d_block_part = nullptr,
            *d_status = nullptr;
-    uint32_t* d_counter = nullptr;
     uint8_t* d_active = nullptr;
     EllMatrix ell[3];
     double* d_emi = nullptr;
@@ -517,8 +516,10 @@ static int build_work(dtx_handle* h) {
     int64_t range = (int64_t)std::ceil(total / target_items);
     range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
     // LOSS-layout channels (losses, moment rates) are small FP64 GEMVs next to the gain
-    // contraction: short ranges keep their items from forming the tail of the launch.
-    const int64_t loss_range = std::min<int64_t>(range, 4);
+    // contraction: ranges of at most 16 chunks keep their items from forming the tail of the
+    // launch (shorter ones measured slower at transport_2d: each range is a partial slot that
+    // is written and read back)
+    const int64_t loss_range = std::min<int64_t>(range, 16);
     std::vector<int> next_slot(h->S, 0), gain_slots(h->S, 0);
     h->part_stride = (int64_t)C * h->N;
     // A unit is (channel, chunk range, k tile); it has one item per cell tile.  Gain slots of a
@@ -829,7 +830,6 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.mom_part = h->d_mom_part;
     a.block_part = h->d_block_part;
     a.status = h->d_status;
-    a.counter = h->d_counter;
     a.f_mom = h->d_fmom;
     a.df_mom = h->d_dfmom;
     a.dpflux = h->d_dpflux;
@@ -1269,7 +1269,6 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_block_part, (C * nseg + nfold) * BP_WORDS);
     rc |= dalloc(hp, &h->d_mom_part, C * nseg * h->S * 2);
     rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
-    rc |= dalloc(hp, &h->d_counter, 1);
     if (h->method == DTX_METHOD_FORWARD_SYMPLECTIC_EULER) {
         rc |= dalloc(hp, &h->d_fmom, n);
         rc |= dalloc(hp, &h->d_dfmom, n);
@@ -1290,8 +1289,7 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     if (!rc) rc = upload(hp, h->d_unpad, h->unpad.data(), h->Np);
     if (!rc) rc = upload(hp, h->d_species_off, h->sp_off.data(), h->S + 1);
     if (!rc) {
-        e = hipMemsetAsync(h->d_counter, 0, sizeof(uint32_t), h->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * h->Np * 8, h->stream);
+        e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * h->Np * 8, h->stream);
         if (e == hipSuccess && h->d_fhi)
             e = hipMemsetAsync(h->d_fhi, 0, (size_t)h->ldc * h->Np * 2, h->stream);
         if (e == hipSuccess && h->d_flo)
