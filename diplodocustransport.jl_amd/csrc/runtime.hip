@@ -99,6 +99,7 @@ struct dtx_handle {
     double* d_f[2] = {nullptr, nullptr};
     int cur = 0;
     double* d_fT = nullptr;       // FP64, state-major, padded states: [Np][ldc]
+    int num_cu = 256;             // compute units of the device (work-item sizing)
     int wide = 0;                 // wide gain items (C > 64): k tiles per item (BF16x3 2 or 4,
                                   // FP64 2), 0 for 64-cell tiles
     uint16_t* d_fhi = nullptr;    // BF16 split, cell-major: [ldc][Np]
@@ -510,16 +511,40 @@ static int build_work(dtx_handle* h) {
         }
         h->gain_bytes += (double)c.a_bytes;
     }
-    // about four resident workgroups per CU; two per CU over both wide shapes (two k tiles:
-    // two workgroups of 68 KB LDS per CU; four k tiles: one of 102 KB)
-    const int target_items = h->wide ? 512 : 1024;
-    int64_t range = (int64_t)std::ceil(total / target_items);
-    range = std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, range));
     // LOSS-layout channels (losses, moment rates) are small FP64 GEMVs next to the gain
     // contraction: ranges of at most 16 chunks keep their items from forming the tail of the
     // launch (shorter ones measured slower at transport_2d: each range is a partial slot that
     // is written and read back)
-    const int64_t loss_range = std::min<int64_t>(range, 16);
+    auto loss_range_of = [](int64_t rg) { return std::min<int64_t>(rg, 16); };
+    auto launch_items = [&](int64_t rg) {
+        const int64_t lr = loss_range_of(rg);
+        int64_t n = 0;
+        for (auto& c : h->chans)
+            n += c.kind == CH_LOSS
+                     ? (c.nchunks + lr - 1) / lr * c.nktiles * ctiles
+                     : (c.nchunks + rg - 1) / rg * gain_ksteps(c) * (h->wide ? wctiles : ctiles);
+        for (auto& c : h->mchans) n += (c.nchunks + lr - 1) / lr * c.nktiles * ctiles;
+        return n;
+    };
+    // Chunks per gain item.  64-cell launches (about four resident workgroups per CU) take
+    // ranges for about 1024 items.  The wide launch with two k tiles holds two workgroups per
+    // CU and runs fastest when all its items fit one round of them: the smallest range with at
+    // most 2 x CUs items (at transport_1d, FP64 and split-BF16, a round and a bit more measured
+    // 10-20% slower, and fewer, longer items slower too).  With four k tiles (one workgroup per
+    // CU) the items cannot fit one round; about three rounds of gain items measured best.
+    auto clamp_range = [](double r) {
+        return std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, (int64_t)std::ceil(r)));
+    };
+    int64_t range = clamp_range(total / (h->wide == 4 ? 768 : h->wide ? 512 : 1024));
+    if (h->wide == 2) {
+        const int64_t round = 2 * (int64_t)h->num_cu;
+        for (int64_t rg = 8; rg <= MAX_RANGE; ++rg)
+            if (launch_items(rg) <= round) {
+                range = rg;
+                break;
+            }
+    }
+    const int64_t loss_range = loss_range_of(range);
     std::vector<int> next_slot(h->S, 0), gain_slots(h->S, 0);
     h->part_stride = (int64_t)C * h->N;
     // A unit is (channel, chunk range, k tile); it has one item per cell tile.  Gain slots of a
@@ -1243,6 +1268,12 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess)
         return fail(nullptr, DTX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device) ==
+                hipSuccess && cus > 0)
+            h->num_cu = cus;
+    }
     dtx_handle* hp = h.get();
     int rc = 0;
     const int64_t n = h->n;
