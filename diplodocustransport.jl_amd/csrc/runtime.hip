@@ -512,10 +512,13 @@ static int build_work(dtx_handle* h) {
         h->gain_bytes += (double)c.a_bytes;
     }
     // LOSS-layout channels (losses, moment rates) are small FP64 GEMVs next to the gain
-    // contraction: ranges of at most 16 chunks keep their items from forming the tail of the
-    // launch (shorter ones measured slower at transport_2d: each range is a partial slot that
-    // is written and read back)
-    auto loss_range_of = [](int64_t rg) { return std::min<int64_t>(rg, 16); };
+    // contraction: short ranges keep their items from forming the tail of the launch.  At most
+    // 4 chunks next to 64-cell gain items, 16 next to wide ones (measured: each range is a
+    // partial slot that is written and read back, and with 4 the wide launches were 3-6%
+    // slower at transport_1d and transport_2d, while the 64-cell FP32 launch at transport_1d
+    // was 5% faster)
+    const int64_t loss_cap = h->wide ? 16 : 4;
+    auto loss_range_of = [&](int64_t rg) { return std::min<int64_t>(rg, loss_cap); };
     auto launch_items = [&](int64_t rg) {
         const int64_t lr = loss_range_of(rg);
         int64_t n = 0;
