@@ -51,6 +51,7 @@ struct BinWork {
     int32_t ldb;              // row pitch of the BF16 B arrays (padded states)
     int32_t N;                // states per cell (partial buffer row pitch)
     int32_t kind;             // ChannelKind
+    int32_t tc;               // cells per tile of the item (16, 64, or WIDE)
 };
 
 // Packed operator of one channel on the device.
@@ -156,6 +157,10 @@ struct CorrectArgs {
 hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream);
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
+// Items of both 64-row tile shapes (tc 16 and 64, per item) in one launch: grouped steps mix
+// single-cell requests (operator streaming, HBM-bound) with many-cell ones (MFMA-bound).
+hipError_t launch_binary_mixed(int precision, const BinWork* items, int nitems,
+                               hipStream_t stream);
 // Contraction with (64 ns) x WIDE (rows x cells) gain items for C > 64: split-BF16 with ns = 2
 // or 4 k tiles per item, FP64 with ns = 2.  LOSS-layout items keep the 64 x 64 FP64 body.
 constexpr int WIDE = 128;   // cells per wide item

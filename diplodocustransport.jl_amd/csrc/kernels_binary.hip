@@ -847,6 +847,41 @@ hipError_t launch_pack_loss(const double* D, int N, const int32_t* unpad, int Np
     return hipGetLastError();
 }
 
+// One launch over items of both narrow tile shapes: each workgroup takes the body of its
+// item's shape (the same device code as the single-shape kernels, so a result does not depend
+// on the launch it ran in).
+template <int P>
+__global__ __launch_bounds__(256) void bin_mixed_kernel(const BinWork* __restrict__ items) {
+    __shared__ union {
+        SmemF64<16> d16;
+        SmemF64<64> d64;
+        SmemF32<16> s16;
+        SmemF32<64> s64;
+        SmemBF16<16> b16;
+        SmemBF16<64> b64;
+    } sm;
+    const BinWork& w = items[blockIdx.x];
+    const bool fp64_body = P == DTX_PREC_FP64 || w.kind >= CH_LOSS;
+    if (w.tc == 16) {
+        if (fp64_body) f64_item<16>(w, sm.d16);
+        else if (P == DTX_PREC_FP32) f32_item<16>(w, sm.s16);
+        else bf16x3_item<16>(w, sm.b16);
+    } else {
+        if (fp64_body) f64_item<64>(w, sm.d64);
+        else if (P == DTX_PREC_FP32) f32_item<64>(w, sm.s64);
+        else bf16x3_item<64>(w, sm.b64);
+    }
+}
+
+hipError_t launch_binary_mixed(int precision, const BinWork* items, int nitems,
+                               hipStream_t stream) {
+    if (nitems <= 0) return hipSuccess;
+    if (precision == DTX_PREC_FP64) bin_mixed_kernel<DTX_PREC_FP64><<<nitems, 256, 0, stream>>>(items);
+    else if (precision == DTX_PREC_FP32) bin_mixed_kernel<DTX_PREC_FP32><<<nitems, 256, 0, stream>>>(items);
+    else bin_mixed_kernel<DTX_PREC_BF16X3><<<nitems, 256, 0, stream>>>(items);
+    return hipGetLastError();
+}
+
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;

@@ -585,6 +585,7 @@ static int build_work(dtx_handle* h) {
                 w.ldb = h->Np;
                 w.N = h->N;
                 w.kind = c.moment >= 0 ? CH_MOMENT : c.kind;
+                w.tc = wide ? WIDE : h->tc;
                 out.push_back({w, slot, wide ? wctiles : ctiles});
             }
         }
@@ -1010,7 +1011,7 @@ struct Group {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<dtx_handle*> members;
-    // cell tiles of 16 / 64, wide items of 2 / 4 k tiles
+    // narrow items of both shapes (list 0; list 1 stays empty), wide items of 2 / 4 k tiles
     BinWork* d_items[4] = {nullptr, nullptr, nullptr, nullptr};
     int n_items[4] = {0, 0, 0, 0};
     int precision = 0;
@@ -1111,6 +1112,22 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
         g->max_N = std::max(g->max_N, h->N);
     }
+    // Both 64-row narrow shapes go into one launch (list 0): 16-cell items of single-cell
+    // requests stream their operator (HBM-bound) while 64-cell items are MFMA-bound, so the two
+    // are interleaved in proportion to run side by side.  Each member's items keep their order.
+    {
+        std::vector<BinWork> mixed;
+        mixed.reserve(lists[0].size() + lists[1].size());
+        const size_t n0 = lists[0].size(), n1 = lists[1].size();
+        size_t i0 = 0, i1 = 0;
+        while (i0 < n0 || i1 < n1) {
+            // take from the list that is behind its share
+            if (i1 >= n1 || (i0 < n0 && i0 * n1 <= i1 * n0)) mixed.push_back(lists[0][i0++]);
+            else mixed.push_back(lists[1][i1++]);
+        }
+        lists[0].swap(mixed);
+        lists[1].clear();
+    }
     for (int q = 0; q < 4; ++q) {
         g->n_items[q] = (int)lists[q].size();
         if (!g->n_items[q]) continue;
@@ -1169,9 +1186,7 @@ int group_step(Group* g, const double* dt, int verbose) {
                                   hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess) e = hipEventRecord(g->ev[slot], g->stream);
     g->ev_used[slot] = true;
-    const int ktc[2] = {16, 64};
-    for (int q = 0; q < 2 && e == hipSuccess; ++q)
-        e = launch_binary(g->precision, ktc[q], g->d_items[q], g->n_items[q], g->stream);
+    if (e == hipSuccess) e = launch_binary_mixed(g->precision, g->d_items[0], g->n_items[0], g->stream);
     for (int q = 2; q < 4 && e == hipSuccess; ++q)
         e = launch_binary_wide(g->precision, 2 * (q - 1), g->d_items[q], g->n_items[q], g->stream);
     if (e == hipSuccess)
