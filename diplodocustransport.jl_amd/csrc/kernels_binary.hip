@@ -121,6 +121,12 @@ __device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
 // (LOSS channels stage f_j^c and multiply by their row's f at the end), so the accumulators
 // hold the whole item's contribution and no epilogue runs inside the loop.  Each 32-j chunk is
 // staged through LDS in two halves of 16 j, double-buffered (32 KB of LDS).
+//
+// With 64-cell tiles both tiles sit in LDS in MFMA operand order, as in the wide FP64 item: one
+// 64-double block per (k step of 4, 16 rows or cells), element l of a block being what lane l
+// supplies, so every LDS store and operand read is a linear 512-byte block (the row-major
+// staging stores cost about 7 bank-conflict cycles per instruction; the 16-cell shape, which
+// streams its operator from HBM, keeps them, as operand order measured 3% slower there).
 template <int TC>
 struct SmemF64 {
     static constexpr int HK = KC / 2;   // j per LDS stage
@@ -147,6 +153,11 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     const GLB double* B = (const GLB double*)v.B;
     const int bq = tid / BT, bcc = (tid % BT) * BP;
     const bool loss = v.kind >= CH_LOSS;   // B = f_j; LOSS and MOMENT rows take their own f
+    // operand order (TC = 64): element q 256 + tid of a stage tile is in block 4 q + wave = k step
+    // q, row / cell block `wave`, at lane `lane`; its row 16 wave + lane % 16 (A) or cell
+    // 16 wave + lane % 16 (B), its k 4 q + lane / 16
+    constexpr bool frag = TC == 64;
+    const int orow = 16 * wave + (lane & 15), kl = lane >> 4;
     load_ctab(v, ctab);
 
     double ra[AP], rb[BP], rf[BP];
@@ -154,24 +165,42 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     auto load_stage = [&](int st) {
         const int ch = st >> 1, h = st & 1;
         const uint32_t ce = ctab[ch];
-        const GLB double* src = A + (int64_t)ch * KC * TK + h * HK * TK + tid * AP;
-#pragma unroll
-        for (int q = 0; q < AP; ++q) ra[q] = src[q];
         const int j0 = (int)(ce >> 16) + h * HK;
-        const GLB double* b = B + (int64_t)(j0 + bq) * v.ldc + v.c0 + bcc;
+        if constexpr (frag) {
+            const GLB double* src = A + (int64_t)ch * KC * TK + (int64_t)(h * HK + kl) * TK + orow;
 #pragma unroll
-        for (int q = 0; q < BP; ++q) rb[q] = b[q];
-        if (!loss) {
-            const GLB double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
+            for (int q = 0; q < AP; ++q) ra[q] = src[4 * q * TK];
+            const GLB double* b = B + (int64_t)(j0 + kl) * v.ldc + v.c0 + orow;
 #pragma unroll
-            for (int q = 0; q < BP; ++q) rf[q] = fi[q];
+            for (int q = 0; q < BP; ++q) rb[q] = b[(int64_t)(4 * q) * v.ldc];
+            if (!loss) rf[0] = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + orow];
+        } else {
+            const GLB double* src = A + (int64_t)ch * KC * TK + h * HK * TK + tid * AP;
+#pragma unroll
+            for (int q = 0; q < AP; ++q) ra[q] = src[q];
+            const GLB double* b = B + (int64_t)(j0 + bq) * v.ldc + v.c0 + bcc;
+#pragma unroll
+            for (int q = 0; q < BP; ++q) rb[q] = b[q];
+            if (!loss) {
+                const GLB double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcc;
+#pragma unroll
+                for (int q = 0; q < BP; ++q) rf[q] = fi[q];
+            }
         }
     };
     auto store_stage = [&](int buf) {
+        if constexpr (frag) {
 #pragma unroll
-        for (int q = 0; q < AP; ++q) As[buf][tid * AP + q] = ra[q];
+            for (int q = 0; q < AP; ++q) As[buf][q * 256 + tid] = ra[q];
 #pragma unroll
-        for (int q = 0; q < BP; ++q) Bs[buf][bq * TC + bcc + q] = loss ? rb[q] : rf[q] * rb[q];
+            for (int q = 0; q < BP; ++q) Bs[buf][q * 256 + tid] = loss ? rb[q] : rf[0] * rb[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < AP; ++q) As[buf][tid * AP + q] = ra[q];
+#pragma unroll
+            for (int q = 0; q < BP; ++q)
+                Bs[buf][bq * TC + bcc + q] = loss ? rb[q] : rf[q] * rb[q];
+        }
     };
 
     d4 acc[T::MR][T::MC];
@@ -195,10 +224,12 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
             double a[T::MR], b[T::MC];
 #pragma unroll
             for (int r = 0; r < T::MR; ++r)
-                a[r] = As[buf][kr * TK + wk * T::WROWS + r * 16 + (lane & 15)];
+                a[r] = frag ? As[buf][((kk / 4) * (TK / 16) + wk * T::MR + r) * 64 + lane]
+                            : As[buf][kr * TK + wk * T::WROWS + r * 16 + (lane & 15)];
 #pragma unroll
             for (int c = 0; c < T::MC; ++c)
-                b[c] = Bs[buf][kr * TC + wc * T::WCOLS + c * 16 + (lane & 15)];
+                b[c] = frag ? Bs[buf][((kk / 4) * (TC / 16) + wc * T::MC + c) * 64 + lane]
+                            : Bs[buf][kr * TC + wc * T::WCOLS + c * 16 + (lane & 15)];
 #pragma unroll
             for (int r = 0; r < T::MR; ++r)
 #pragma unroll
