@@ -38,7 +38,8 @@ struct BinWork {
                               // BF16 -> hi bits, cell-major: B[cell * ldb + kp]
     int64_t b_lo;             // BF16: elements from the hi to the lo array
     double* part;             // partial sums of this slot: part[cell * N + row]
-                              // (CH_MOMENT: the item's per-cell sums, part[cell])
+                              // (CH_MOMENT: the item's per-cell sums, part[cell]; wide
+                              // split-BF16 gain items: FP32 sums, ((float*)part)[cell * N + row])
     int64_t a_ktile_stride;   // elements between consecutive k-tiles of A
     int32_t ktile;            // k-tile inside the channel
     int32_t ctile;            // first cell of the tile = ctile * TC
@@ -94,6 +95,7 @@ struct UpdateArgs {
     const double* vol;
     const double* part;     // binary partial sums [slot][cell][N]
     const int32_t* nslot;   // slots per state (N)
+    const int32_t* nslot32; // leading slots of each state holding FP32 sums (N), or null
     int64_t part_stride;    // doubles between slots
     const double* emi;      // emission blocks [cell][N x N] or null
     const uint8_t* active;  // binary domain per cell (1 = evaluated) or null = all
@@ -146,6 +148,7 @@ struct CorrectArgs {
     const int32_t* species_off;  // (S+1)
     const int32_t* gslots;     // gain slots per species (S)
     const int32_t* nslots;     // gain + loss slots per species (S)
+    int32_t gain_f32;          // the gain slots hold FP32 sums (wide split-BF16 items)
     const double* eweight;     // per-state energy weight (N) or null (number only)
     const double* mpart;       // moment-rate item sums [index][ldc]
     const int32_t* mom_off;    // (2S+1) offsets into mom_list for (s, moment)

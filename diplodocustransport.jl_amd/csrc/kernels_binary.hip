@@ -686,18 +686,19 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide<NS>& sm) {
         __syncthreads();
         buf ^= 1;
     }
-    // write the item's partial sums (FP32 result layout D[4 (l / 16) + e][l % 16])
+    // write the item's partial sums, FP32 as they are (FP32 result layout
+    // D[4 (l / 16) + e][l % 16]): the slot is read as floats
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int cell = v.c0 + 64 * wc + 16 * c + (lane & 15);
             if (cell >= v.cells) continue;
-            GLB double* dst = v.part + (int64_t)cell * v.N + v.row_base;
+            GLB float* dst = (GLB float*)v.part + (int64_t)cell * v.N + v.row_base;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int krow = kt0 * TK + 64 * wk + 16 * r + 4 * (lane >> 4) + e;
-                if (krow < v.rows) dst[krow] = (double)tot[r][c][e];
+                if (krow < v.rows) dst[krow] = tot[r][c][e];
             }
         }
 }

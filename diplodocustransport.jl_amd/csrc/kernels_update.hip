@@ -98,8 +98,25 @@ __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, b
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
     if (valid) {
-        const int ns = a.nslot[k];
-        for (int s0 = q; s0 < ns; s0 += 4 * SB) {
+        // slots q, q + 4, ... in order: the leading FP32 ones, then the FP64 ones
+        const int ns = a.nslot[k], n32 = a.nslot32 ? a.nslot32[k] : 0;
+        int s0 = q;
+        for (; s0 < n32; s0 += 4 * SB) {
+            // past the last FP32 slot the load repeats slot s0 (valid, unused), so the loads
+            // carry no branch and stay in flight together
+            float v[SB];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int s = s0 + 4 * u < n32 ? s0 + 4 * u : s0;
+                v[u] = ((const float*)(a.part + (int64_t)s * a.part_stride))[idx];
+            }
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                if (s0 + 4 * u < n32) g = g + (double)v[u];
+        }
+        // first FP64 slot of this thread: the smallest q + 4 i >= n32
+        s0 = n32 + ((q - n32) % 4 + 4) % 4;
+        for (; s0 < ns; s0 += 4 * SB) {
             double v[SB];
 #pragma unroll
             for (int u = 0; u < SB; ++u) {
@@ -699,16 +716,40 @@ __global__ __launch_bounds__(UB) void gain_correct_kernel(CorrectArgs a) {
     for (int q = tid; q < ns; q += UB) {
         const int64_t idx = base + k0 + q;
         double gv = 0.0, lv = 0.0;
-        for (int b0 = 0; b0 < nsl; b0 += LB) {
+        // gain slots (FP32 sums for wide split-BF16 items), then loss slots, each in order
+        if (a.gain_f32) {
+            for (int b0 = 0; b0 < gs; b0 += LB) {
+                // past the last gain slot the load repeats slot b0 (valid, unused): no branch
+                // between the loads, which stay in flight together
+                float v[LB];
+#pragma unroll
+                for (int u = 0; u < LB; ++u) {
+                    const int s = b0 + u < gs ? b0 + u : b0;
+                    v[u] = ((const float*)(a.part + (int64_t)s * a.part_stride))[idx];
+                }
+#pragma unroll
+                for (int u = 0; u < LB; ++u)
+                    if (b0 + u < gs) gv = gv + (double)v[u];
+            }
+        } else {
+            for (int b0 = 0; b0 < gs; b0 += LB) {
+                double v[LB];
+#pragma unroll
+                for (int u = 0; u < LB; ++u)
+                    v[u] = b0 + u < gs ? a.part[(int64_t)(b0 + u) * a.part_stride + idx] : 0.0;
+#pragma unroll
+                for (int u = 0; u < LB; ++u)
+                    if (b0 + u < gs) gv = gv + v[u];
+            }
+        }
+        for (int b0 = gs; b0 < nsl; b0 += LB) {
             double v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u)
                 v[u] = b0 + u < nsl ? a.part[(int64_t)(b0 + u) * a.part_stride + idx] : 0.0;
 #pragma unroll
-            for (int u = 0; u < LB; ++u) {
-                if (b0 + u < gs) gv = gv + v[u];
-                else if (b0 + u < nsl) lv = lv + v[u];
-            }
+            for (int u = 0; u < LB; ++u)
+                if (b0 + u < nsl) lv = lv + v[u];
         }
         g[q] = gv;
         l[q] = lv;

@@ -92,6 +92,7 @@ struct dtx_handle {
     int32_t n_mpart = 0;
     int64_t part_stride = 0;
     int32_t* d_nslot = nullptr;
+    int32_t* d_nslot32 = nullptr;  // leading FP32 slots per state (wide split-BF16), or null
     int64_t pairs_total = 0;
     double bin_flops = 0, gain_bytes = 0;
 
@@ -478,6 +479,7 @@ static int build_work(dtx_handle* h) {
     dfree(h, &h->d_items, h->nitems);
     dfree(h, &h->d_part, h->part_stride * h->part_slots);
     dfree(h, &h->d_nslot, h->N);
+    dfree(h, &h->d_nslot32, h->N);
     dfree(h, &h->d_dbsum, h->n);
     dfree(h, &h->d_ones, h->N);
     dfree(h, &h->d_gslots, h->S);
@@ -658,6 +660,14 @@ static int build_work(dtx_handle* h) {
     int rc = dalloc(h, &h->d_nslot, h->N);
     if (!rc) rc = upload(h, h->d_nslot, nslot.data(), h->N);
     if (rc) return rc;
+    if (h->wide && bf16) {  // wide split-BF16 gain items write FP32 sums: the gain slots
+        std::vector<int32_t> n32(h->N, 0);
+        for (int s = 0; s < h->S; ++s)
+            for (int k = 0; k < h->sp_size[s]; ++k) n32[h->sp_off[s] + k] = gain_slots[s];
+        rc = dalloc(h, &h->d_nslot32, h->N);
+        if (!rc) rc = upload(h, h->d_nslot32, n32.data(), h->N);
+        if (rc) return rc;
+    }
     if (h->mchans.empty()) return DTX_OK;
     // correction tables
     std::vector<int32_t> ones(h->N, 1), moff(2 * h->S + 1, 0), mlist;
@@ -841,9 +851,11 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     if (h->correct_active) {  // one corrected binary sum per state
         a.part = h->d_dbsum;
         a.nslot = h->d_ones;
+        a.nslot32 = nullptr;
     } else {
         a.part = h->d_part;
         a.nslot = h->d_nslot;
+        a.nslot32 = h->d_nslot32;
     }
     a.part_stride = h->part_stride;
     a.emi = h->d_emi;
@@ -916,6 +928,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             ca.species_off = h->d_species_off;
             ca.gslots = h->d_gslots;
             ca.nslots = h->d_nslots_s;
+            ca.gain_f32 = h->wide && h->precision == DTX_PREC_BF16X3;
             ca.eweight = h->d_eweight;
             ca.mpart = h->d_mpart;
             ca.mom_off = h->d_mom_off;
