@@ -42,7 +42,7 @@ DESCRIPTIONS = {
 # HBM bytes per launch of the FP64 contraction from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (separate runs, each scaled by the same counter over a 1 GiB device copy):
 # profiles/transport_1d_hbm_traffic.txt.
-TRAFFIC = {("transport_1d", "bin_f64_wide_kernel"): 503_160_000}
+TRAFFIC = {("transport_1d", "bin_f64_wide_kernel"): 500_060_000}
 
 
 def log(*a):
