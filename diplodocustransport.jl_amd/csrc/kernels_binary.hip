@@ -10,12 +10,15 @@
 // channels (the diagonal part, FP64 in every mode) accumulate sum_j D[k,j] f_j^c over all chunks
 // and are multiplied by the row's own f_k^c.
 //
-// Every workgroup takes one work item (channel, 64-row tile, cell tile, chunk range) from a work
-// table and writes its FP64 partial sum to its own slot, so the reduction order is fixed by the
-// item list alone: a simulation gives bit-identical results alone or grouped with others.  LOSS
-// items share the launch with the gain items of any precision (they take the FP64 body).
+// Every workgroup takes one work item (channel, row tile, cell tile, chunk range) from a work
+// table and writes its partial sum to its own slot (FP64; FP32 for wide split-BF16 gain items,
+// whose sums are FP32), so the reduction order is fixed by the item list alone: a simulation
+// gives bit-identical results alone or grouped with others.  LOSS-layout items share the launch
+// with the gain items of any precision (they take the FP64 body).
 //
-// Wave tiling: 4 waves = WK (rows) x WC (cells); each wave owns MR x MC 16x16 MFMA tiles.
+// Item shapes: 64 rows x 16 or 64 cells (four waves = WK (rows) x WC (cells), each owning
+// MR x MC 16x16 MFMA tiles), and for more than 64 cells the wide shapes, 128 or 256 rows x 128
+// cells with 64 x 64 per wave (wide FP64 and split-BF16 sections below).
 // MFMA operand layout (16x16xK): lane l supplies A[l%16][kgroup(l/16)], B[kgroup(l/16)][l%16].
 // The 16x16 result tile is held as D[4*(l/16)+e][l%16] (e = 0..3) by the FP32 and BF16
 // instructions, and as D[(l/16)+4*e][l%16] by v_mfma_f64_16x16x4_f64 (measured on gfx950).
