@@ -224,7 +224,7 @@ def main():
     # (78.6 TFLOP/s / 8 TB/s), else HBM-bound (single-cell boxes stream the whole operator).
     ai = base["bin_flops"] / base["bin_bytes"]
     C = w.ps.n_space
-    kernel = ("bin_f64_kernel<16>" if C <= 16 else
+    kernel = ("bin_gemv_kernel<0>" if C == 1 else "bin_f64_kernel<16>" if C <= 16 else
               "bin_f64_wide_kernel" if 64 < C <= 256 else "bin_f64_kernel<64>")
     if ai >= PEAK_TFLOPS["fp64"] * 1e12 / (PEAK_HBM_GBS * 1e9):
         achieved = base["bin_flops"] / (base["bin_ms"] * 1e-3) / 1e12

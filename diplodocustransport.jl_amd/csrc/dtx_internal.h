@@ -52,7 +52,7 @@ struct BinWork {
     int32_t ldb;              // row pitch of the BF16 B arrays (padded states)
     int32_t N;                // states per cell (partial buffer row pitch)
     int32_t kind;             // ChannelKind
-    int32_t tc;               // cells per tile of the item (16, 64, or WIDE)
+    int32_t tc;               // cells per tile of the item (1: GEMV, 16, 64, or WIDE)
 };
 
 // Packed operator of one channel on the device.
@@ -93,7 +93,10 @@ struct UpdateArgs {
     const double* inv_ap;
     const double* df_inj;
     const double* vol;
-    const double* part;     // binary partial sums [slot][cell][N]
+    const double* part;     // binary partial sums [slot][cell][N] (moment-rate corrected:
+                            // the loss sums [cell][N])
+    const double* gsum;     // moment-rate corrected: gain sums [cell][N], else null
+    const double* coef;     // moment-rate corrected: (alpha, beta) [cell][S][2], else null
     const int32_t* nslot;   // slots per state (N)
     const int32_t* nslot32; // leading slots of each state holding FP32 sums (N), or null
     int64_t part_stride;    // doubles between slots
@@ -136,15 +139,20 @@ enum StatusSlot {
 enum BlockSlot { BP_SUMDF = 0, BP_SUMF, BP_CR, BP_CR_BIN, BP_CR_EMI, BP_CR_FLUX, BP_CR_MOM,
                  BP_CR_SPACE, BP_WORDS };
 
-// FP64 moment-rate correction of a reduced-precision gain term, one workgroup per (cell, output
-// species s).  dBg = sum of the species' gain slots, dBl = sum of its loss slots; R_N and R_E are
+// FP64 moment-rate correction of a reduced-precision gain term, per (cell, output species s).
+// dBg = sum of a state's gain slots (gsum), dBl = sum of its loss slots (lsum); R_N and R_E are
 // the exact FP64 number and energy rates of the gain term (sums of the moment-rate item sums
-// listed for (s, moment) in mom_off / mom_list).  The corrected binary sum dbsum = dBg (1 + a + b E) +
-// dBl, with (a, b) chosen so that sum dBg' = R_N and sum E dBg' = R_E.
+// listed for (s, moment) in mom_off / mom_list).  The update kernels take the corrected binary
+// sum dBg (1 + a + b E) + dBl, with (a, b) = coef[c][s] chosen so that sum dBg' = R_N and
+// sum E dBg' = R_E over the species.
 struct CorrectArgs {
     const double* part;        // partial slots [slot][cell][N]
     int64_t part_stride;
-    double* dbsum;             // output [cell][N]
+    double* gsum;              // gain sums [cell][N]
+    double* lsum;              // loss sums [cell][N]
+    double* cpart;             // moment partials of the gain sums [cell][S][zb][3]
+    double* coef;              // (alpha, beta) [cell][S][2]
+    int32_t zb;                // workgroups per species in the sums launch: ceil(max ns / 64)
     const int32_t* species_off;  // (S+1)
     const int32_t* gslots;     // gain slots per species (S)
     const int32_t* nslots;     // gain + loss slots per species (S)
@@ -157,11 +165,11 @@ struct CorrectArgs {
 };
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------
-hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream);
+hipError_t launch_gain_correct(const CorrectArgs& a, hipStream_t stream);
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
-// Items of both 64-row tile shapes (tc 16 and 64, per item) in one launch: grouped steps mix
-// single-cell requests (operator streaming, HBM-bound) with many-cell ones (MFMA-bound).
+// Items of the narrow 64-row shapes (tc 1, 16 and 64, per item) in one launch: grouped steps
+// mix single-cell requests (operator streaming, HBM-bound) with many-cell ones (MFMA-bound).
 hipError_t launch_binary_mixed(int precision, const BinWork* items, int nitems,
                                hipStream_t stream);
 // Contraction with (64 ns) x WIDE (rows x cells) gain items for C > 64: split-BF16 with ns = 2

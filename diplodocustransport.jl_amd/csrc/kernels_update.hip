@@ -97,7 +97,15 @@ __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, b
     constexpr int SB = 8;  // slot loads in flight per thread
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
-    if (valid) {
+    if (a.coef) {
+        // moment-rate corrected: the state's gain sum rescaled by (1 + alpha + beta E_k) of its
+        // (cell, species), plus its loss sum (thread q = 0; the others add zeros)
+        if (valid && q == 0) {
+            const double* cf = a.coef + ((int64_t)c * a.S + a.state_species[k]) * 2;
+            const double e = a.eweight ? a.eweight[k] : 0.0;
+            g = a.gsum[idx] * (1.0 + (cf[0] + cf[1] * e)) + a.part[idx];
+        }
+    } else if (valid) {
         // slots q, q + 4, ... in order: the leading FP32 ones, then the FP64 ones
         const int ns = a.nslot[k], n32 = a.nslot32 ? a.nslot32[k] : 0;
         int s0 = q;
@@ -695,124 +703,129 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
 }
 
 // --------------------------------------------------------- FP64 moment-rate correction -----
-// One workgroup per (cell c, output species s).  The reduced-precision gain contraction carries
-// a relative error of its arithmetic in every term; the exact FP64 number and energy rates of
-// the gain term, R_N = sum_ij W_N[i,j] f_i f_j and R_E likewise, come from the moment-rate
-// slots.  The gain sums of the species are rescaled per state by (1 + alpha + beta E_k) with
-// (alpha, beta) such that their number and energy sums equal R_N and R_E (number only when no
-// energy weights are set), and the loss sums are added unchanged.
-__global__ __launch_bounds__(UB) void gain_correct_kernel(CorrectArgs a) {
-    extern __shared__ double gl[];  // gain sums [ns], then loss sums [ns]
-    __shared__ double red[(UB / 64) * RED_VALUES];
-    __shared__ double coef[2];
+// The reduced-precision gain contraction carries a relative error of its arithmetic in every
+// term; the exact FP64 number and energy rates of the gain term, R_N = sum_ij W_N[i,j] f_i f_j
+// and R_E likewise, come from the moment-rate slots.  Per (cell c, output species s) the gain
+// sums of the species are rescaled per state by (1 + alpha + beta E_k) with (alpha, beta) such
+// that their number and energy sums equal R_N and R_E (number only when no energy weights are
+// set), and the loss sums are added unchanged.  Two launches: the slot sums of every state
+// with per-workgroup moment partials (many workgroups, the slot reads in flight together),
+// then one wave per (c, s) for the coefficients; the update kernels apply them
+// (binary_term).
+
+// Sum of slots b + q, b + q + 4, ... below e of element idx (in order; FP32 slots if f32).
+// FP32 slots: past the last slot a load repeats slot s0 (valid, unused), so the loads of a
+// batch carry no branch and stay in flight together (a conditional load followed by its
+// conversion waits for that load alone).
+__device__ inline double slot_sum(const CorrectArgs& a, int64_t idx, int q, int b, int e,
+                                  bool f32) {
     constexpr int LB = 8;
-    const int c = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
-    const int k0 = a.species_off[s], ns = a.species_off[s + 1] - k0;
-    double* g = gl;
-    double* l = gl + ns;
-    const int gs = a.gslots[s], nsl = a.nslots[s];
-    const int64_t base = (int64_t)c * a.N;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int q = tid; q < ns; q += UB) {
-        const int64_t idx = base + k0 + q;
-        double gv = 0.0, lv = 0.0;
-        // gain slots (FP32 sums for wide split-BF16 items), then loss slots, each in order
-        if (a.gain_f32) {
-            for (int b0 = 0; b0 < gs; b0 += LB) {
-                // past the last gain slot the load repeats slot b0 (valid, unused): no branch
-                // between the loads, which stay in flight together
-                float v[LB];
+    double sum = 0.0;
+    for (int s0 = b + q; s0 < e; s0 += 4 * LB) {
+        if (f32) {
+            float v[LB];
 #pragma unroll
-                for (int u = 0; u < LB; ++u) {
-                    const int s = b0 + u < gs ? b0 + u : b0;
-                    v[u] = ((const float*)(a.part + (int64_t)s * a.part_stride))[idx];
-                }
-#pragma unroll
-                for (int u = 0; u < LB; ++u)
-                    if (b0 + u < gs) gv = gv + (double)v[u];
+            for (int u = 0; u < LB; ++u) {
+                const int sl = s0 + 4 * u < e ? s0 + 4 * u : s0;
+                v[u] = ((const float*)(a.part + (int64_t)sl * a.part_stride))[idx];
             }
+#pragma unroll
+            for (int u = 0; u < LB; ++u)
+                if (s0 + 4 * u < e) sum = sum + (double)v[u];
         } else {
-            for (int b0 = 0; b0 < gs; b0 += LB) {
-                double v[LB];
-#pragma unroll
-                for (int u = 0; u < LB; ++u)
-                    v[u] = b0 + u < gs ? a.part[(int64_t)(b0 + u) * a.part_stride + idx] : 0.0;
-#pragma unroll
-                for (int u = 0; u < LB; ++u)
-                    if (b0 + u < gs) gv = gv + v[u];
-            }
-        }
-        for (int b0 = gs; b0 < nsl; b0 += LB) {
             double v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u)
-                v[u] = b0 + u < nsl ? a.part[(int64_t)(b0 + u) * a.part_stride + idx] : 0.0;
+                v[u] = s0 + 4 * u < e ? a.part[(int64_t)(s0 + 4 * u) * a.part_stride + idx] : 0.0;
 #pragma unroll
             for (int u = 0; u < LB; ++u)
-                if (b0 + u < nsl) lv = lv + v[u];
+                if (s0 + 4 * u < e) sum = sum + v[u];
         }
-        g[q] = gv;
-        l[q] = lv;
-        const double e = a.eweight ? a.eweight[k0 + q] : 0.0;
-        s0 = s0 + gv;
-        s1 = s1 + e * gv;
-        s2 = s2 + (e * e) * gv;
     }
-    // R_N, R_E of this cell: the listed moment-rate item sums in list order (thread 0; every
-    // other thread adds zeros, which leaves the sum exact through the reduction below)
-    double rate[2] = {0.0, 0.0};
-    if (tid == 0)
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int t0 = a.mom_off[2 * s + m], t1 = a.mom_off[2 * s + m + 1];
-            for (int t = t0; t < t1; t += LB) {
-                double v[LB];
-#pragma unroll
-                for (int u = 0; u < LB; ++u)
-                    v[u] = t + u < t1 ? a.mpart[(int64_t)a.mom_list[t + u] * a.ldc + c] : 0.0;
-#pragma unroll
-                for (int u = 0; u < LB; ++u)
-                    if (t + u < t1) rate[m] = rate[m] + v[u];
-            }
-        }
-    double v[WB] = {s0, s1, s2, rate[0], rate[1], 0.0, 0.0, 0.0};
+    return sum;
+}
+
+// Grid (C, S, ceil(max species states / 64)): four threads per state q = 0..3, each summing
+// slots q, q + 4, ... of the gain slots (FP32 sums for wide split-BF16 items) and of the loss
+// slots; the four sums are combined as (s0 + s1) + (s2 + s3).  Gain sums go to gsum, loss sums
+// to lsum, and the workgroup's number and energy moments of its gain sums (sum g, sum E g,
+// sum E^2 g) to cpart[c][s][z].
+__global__ __launch_bounds__(UB) void correct_sums_kernel(CorrectArgs a) {
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    const int c = blockIdx.x, s = blockIdx.y, z = blockIdx.z, tid = threadIdx.x, q = tid & 3;
+    const int k0 = a.species_off[s], ns = a.species_off[s + 1] - k0;
+    const int kb = z * (UB / 4);
+    if (kb >= ns) return;  // the whole workgroup
+    const int kk = kb + (tid >> 2);
+    const bool valid = kk < ns;
+    const int64_t idx = (int64_t)c * a.N + k0 + (valid ? kk : 0);
+    const int gs = a.gslots[s], nsl = a.nslots[s];
+    double g = slot_sum(a, idx, q, 0, gs, a.gain_f32 != 0);
+    double l = slot_sum(a, idx, q, gs, nsl, false);
+    g = g + __shfl_xor(g, 1, 64);
+    g = g + __shfl_xor(g, 2, 64);
+    l = l + __shfl_xor(l, 1, 64);
+    l = l + __shfl_xor(l, 2, 64);
+    const bool mine = valid && q == 0;
+    if (mine) {
+        a.gsum[idx] = g;
+        a.lsum[idx] = l;
+    }
+    const double e = (mine && a.eweight) ? a.eweight[k0 + kk] : 0.0;
+    const double gm = mine ? g : 0.0;
+    double v[WB] = {gm, e * gm, (e * e) * gm, 0.0, 0.0, 0.0, 0.0, 0.0};
     wave_reduce_batch(v, 0xFFu);
     if ((tid & 63) == 0)
 #pragma unroll
-        for (int i = 0; i < 5; ++i) red[(tid >> 6) * RED_VALUES + i] = v[i];
+        for (int i = 0; i < 3; ++i) red[(tid >> 6) * RED_VALUES + i] = v[i];
     __syncthreads();
-    const double r = tid < 5 ? wave_combine(tid, true, red) : 0.0;
-    __syncthreads();
-    if (tid < 5) red[tid] = r;
-    __syncthreads();
-    if (tid == 0) {
-        const double S0 = red[0], S1 = red[1], S2 = red[2], RN = red[3], RE = red[4];
-        double alpha = 0.0, beta = 0.0;
-        if (S0 != 0.0) {
-            const double dn = RN - S0, de = RE - S1;
-            const double det = S0 * S2 - S1 * S1;
-            if (a.eweight && fabs(det) > 1e-12 * fabs(S0 * S2)) {
-                alpha = (dn * S2 - de * S1) / det;
-                beta = (de * S0 - dn * S1) / det;
-            } else {
-                alpha = dn / S0;
-            }
-            if (!isfinite(alpha) || !isfinite(beta)) alpha = beta = 0.0;
-        }
-        coef[0] = alpha;
-        coef[1] = beta;
-    }
-    __syncthreads();
-    const double alpha = coef[0], beta = coef[1];
-    for (int q = tid; q < ns; q += UB) {
-        const double e = a.eweight ? a.eweight[k0 + q] : 0.0;
-        a.dbsum[base + k0 + q] = g[q] * (1.0 + (alpha + beta * e)) + l[q];
-    }
+    if (tid < 3)
+        a.cpart[(((int64_t)c * a.S + s) * a.zb + z) * 3 + tid] = wave_combine(tid, true, red);
 }
 
-hipError_t launch_gain_correct(const CorrectArgs& a, int max_species_states, hipStream_t stream) {
-    gain_correct_kernel<<<dim3(a.C, a.S), UB, (size_t)2 * max_species_states * sizeof(double),
-                          stream>>>(a);
+// One wave per (cell c, species s): the moments of the gain sums (the workgroup partials in
+// order, lane-strided), the rates R_N and R_E (the listed moment-rate item sums, lane-strided
+// in list order), one butterfly, then (alpha, beta) into coef[c][s].
+__global__ __launch_bounds__(UB) void correct_coef_kernel(CorrectArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * (UB / 64) + (threadIdx.x >> 6);
+    if (p >= (int64_t)a.C * a.S) return;  // the whole wave
+    const int c = (int)(p / a.S), s = (int)(p % a.S);
+    const int ns = a.species_off[s + 1] - a.species_off[s];
+    const int nz = (ns + UB / 4 - 1) / (UB / 4);
+    double v[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int z = lane; z < nz; z += 64)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] = v[i] + a.cpart[(p * a.zb + z) * 3 + i];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int t0 = a.mom_off[2 * s + m], t1 = a.mom_off[2 * s + m + 1];
+        for (int t = t0 + lane; t < t1; t += 64)
+            v[3 + m] = v[3 + m] + a.mpart[(int64_t)a.mom_list[t] * a.ldc + c];
+    }
+    wave_reduce_batch(v, 0xFFu);
+    if (lane != 0) return;
+    const double S0 = v[0], S1 = v[1], S2 = v[2], RN = v[3], RE = v[4];
+    double alpha = 0.0, beta = 0.0;
+    if (S0 != 0.0) {
+        const double dn = RN - S0, de = RE - S1;
+        const double det = S0 * S2 - S1 * S1;
+        if (a.eweight && fabs(det) > 1e-12 * fabs(S0 * S2)) {
+            alpha = (dn * S2 - de * S1) / det;
+            beta = (de * S0 - dn * S1) / det;
+        } else {
+            alpha = dn / S0;
+        }
+        if (!isfinite(alpha) || !isfinite(beta)) alpha = beta = 0.0;
+    }
+    a.coef[p * 2] = alpha;
+    a.coef[p * 2 + 1] = beta;
+}
+
+hipError_t launch_gain_correct(const CorrectArgs& a, hipStream_t stream) {
+    correct_sums_kernel<<<dim3(a.C, a.S, a.zb), UB, 0, stream>>>(a);
+    const int64_t pairs = (int64_t)a.C * a.S;
+    correct_coef_kernel<<<(unsigned)((pairs + UB / 64 - 1) / (UB / 64)), UB, 0, stream>>>(a);
     return hipGetLastError();
 }
 

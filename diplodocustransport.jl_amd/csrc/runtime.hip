@@ -78,7 +78,9 @@ struct dtx_handle {
     bool correct_active = false;
     std::vector<PackedChannel> mchans;
     std::vector<double> eweight_host;
-    double* d_dbsum = nullptr;
+    // moment-rate correction: gain and loss sums per state, moment partials, coefficients
+    double *d_gsum = nullptr, *d_lsum = nullptr, *d_cpart = nullptr, *d_coef = nullptr;
+    int32_t zb = 0;
     int32_t *d_ones = nullptr, *d_gslots = nullptr, *d_nslots_s = nullptr, *d_mom_off = nullptr,
             *d_mom_list = nullptr;
     int32_t n_mom_list = 0, max_ns = 0;
@@ -480,7 +482,10 @@ static int build_work(dtx_handle* h) {
     dfree(h, &h->d_part, h->part_stride * h->part_slots);
     dfree(h, &h->d_nslot, h->N);
     dfree(h, &h->d_nslot32, h->N);
-    dfree(h, &h->d_dbsum, h->n);
+    dfree(h, &h->d_gsum, h->n);
+    dfree(h, &h->d_lsum, h->n);
+    dfree(h, &h->d_cpart, (int64_t)h->C * h->S * h->zb * 3);
+    dfree(h, &h->d_coef, (int64_t)h->C * h->S * 2);
     dfree(h, &h->d_ones, h->N);
     dfree(h, &h->d_gslots, h->S);
     dfree(h, &h->d_nslots_s, h->S);
@@ -680,7 +685,11 @@ static int build_work(dtx_handle* h) {
     h->max_ns = 0;
     for (int s = 0; s < h->S; ++s) h->max_ns = std::max(h->max_ns, h->sp_size[s]);
     std::vector<int32_t> nsl(next_slot.begin(), next_slot.end());
-    rc = dalloc(h, &h->d_dbsum, h->n);
+    h->zb = (h->max_ns + UPDATE_STATES - 1) / UPDATE_STATES;
+    rc = dalloc(h, &h->d_gsum, h->n);
+    if (!rc) rc = dalloc(h, &h->d_lsum, h->n);
+    if (!rc) rc = dalloc(h, &h->d_cpart, (int64_t)h->C * h->S * h->zb * 3);
+    if (!rc) rc = dalloc(h, &h->d_coef, (int64_t)h->C * h->S * 2);
     if (!rc) rc = dalloc(h, &h->d_ones, h->N);
     if (!rc) rc = dalloc(h, &h->d_gslots, h->S);
     if (!rc) rc = dalloc(h, &h->d_nslots_s, h->S);
@@ -848,10 +857,12 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.inv_ap = h->d_invap;
     a.df_inj = h->d_dfinj;
     a.vol = h->d_vol;
-    if (h->correct_active) {  // one corrected binary sum per state
-        a.part = h->d_dbsum;
+    if (h->correct_active) {  // gain sums rescaled per (cell, species), plus loss sums
+        a.part = h->d_lsum;
         a.nslot = h->d_ones;
         a.nslot32 = nullptr;
+        a.gsum = h->d_gsum;
+        a.coef = h->d_coef;
     } else {
         a.part = h->d_part;
         a.nslot = h->d_nslot;
@@ -924,7 +935,11 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             CorrectArgs ca{};
             ca.part = h->d_part;
             ca.part_stride = h->part_stride;
-            ca.dbsum = h->d_dbsum;
+            ca.gsum = h->d_gsum;
+            ca.lsum = h->d_lsum;
+            ca.cpart = h->d_cpart;
+            ca.coef = h->d_coef;
+            ca.zb = h->zb;
             ca.species_off = h->d_species_off;
             ca.gslots = h->d_gslots;
             ca.nslots = h->d_nslots_s;
@@ -937,7 +952,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             ca.C = h->C;
             ca.S = h->S;
             ca.ldc = h->ldc;
-            HIP_CHECK(h, launch_gain_correct(ca, h->max_ns, h->stream));
+            HIP_CHECK(h, launch_gain_correct(ca, h->stream));
         }
         rc = timed_end(h, &ev);
         if (rc) return rc;
@@ -1119,15 +1134,15 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
             g->err = "member stream failed";
             return DTX_ERR_HIP;
         }
-        const int q = h->wide ? 1 + h->wide / 2 : (h->tc == 16 ? 0 : 1);
+        const int q = h->wide ? 1 + h->wide / 2 : (h->tc <= 16 ? 0 : 1);
         if (h->binary) lists[q].insert(lists[q].end(), h->items_host.begin(), h->items_host.end());
         starts.push_back(g->total_blocks);
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
         g->max_N = std::max(g->max_N, h->N);
     }
-    // Both 64-row narrow shapes go into one launch (list 0): 16-cell items of single-cell
-    // requests stream their operator (HBM-bound) while 64-cell items are MFMA-bound, so the two
-    // are interleaved in proportion to run side by side.  Each member's items keep their order.
+    // The narrow shapes go into one launch (list 0): GEMV items of single-cell requests and
+    // 16-cell items stream their operator (HBM-bound) while 64-cell items are MFMA-bound, so the
+    // two lists are interleaved in proportion to run side by side.  Each member's items keep their order.
     {
         std::vector<BinWork> mixed;
         mixed.reserve(lists[0].size() + lists[1].size());
@@ -1268,7 +1283,8 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
                     (long long)h->n);
     h->precision = cfg->precision;
     h->method = cfg->method;
-    h->tc = (h->C <= 16) ? 16 : 64;
+    // one cell: GEMV items that stream the operator (MFMA tiles would be 15/16 padding)
+    h->tc = h->C == 1 ? 1 : (h->C <= 16) ? 16 : 64;
     // wide gain items above 64 cells: split-BF16 with four k tiles per item from 512 cells on,
     // where the launch still has enough items for one eight-wave workgroup per CU.  FP64 with
     // two k tiles up to 256 cells: with many cells the 64-cell items, four workgroups per CU,
@@ -1276,7 +1292,7 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     // (the FP32 contraction keeps 64-cell tiles)
     if (h->precision == DTX_PREC_BF16X3 && h->C > 64) h->wide = h->C >= 512 ? 4 : 2;
     if (h->precision == DTX_PREC_FP64 && h->C > 64 && h->C <= 256) h->wide = 2;
-    const int cell_align = h->wide ? WIDE : h->tc;
+    const int cell_align = h->wide ? WIDE : std::max(h->tc, 16);
     h->ldc = (h->C + cell_align - 1) / cell_align * cell_align;
     h->sp_pad_off.assign(h->S + 1, 0);
     for (int s = 0; s < h->S; ++s)

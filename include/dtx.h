@@ -41,7 +41,9 @@ enum dtx_status {
 };
 
 /* Arithmetic of the binary contraction.  f, df, the Euler update, losses and the moments are
- * FP64 in every mode. */
+ * FP64 in every mode.  With one spatial cell the contraction is a GEMV (no MFMA): the operator
+ * is stored in the mode's format (FP64, FP32, or BF16 hi + lo) and its products are summed in
+ * FP64 against the FP64 f. */
 enum dtx_precision {
     DTX_PREC_FP64 = 0,   /* FP64 MFMA                                                      */
     DTX_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulation within a pair block, FP64 across   */

@@ -148,6 +148,18 @@ def test_many_cells_parity_fp64(shape):
         assert rel_linf(out.f[i], fo[i]) < TOL64, i
 
 
+@pytest.mark.parametrize("kw", [dict(p_num=20, u_num=8, monolithic=False),
+                                dict(num_species=3, emission=True, injection=True)],
+                         ids=["3ktiles", "3species"])
+def test_single_cell_parity_fp64(kw):
+    # one cell: the contraction runs as GEMV items.  (20, 8): 160 states per species, so three
+    # 64-row k tiles, the last partial
+    w = workloads.tiny(x_num=1, steps=20, **kw)
+    m, out, fo, to, _ = _solve_both(w, n_save=2, mono=kw.get("monolithic", True))
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, (kw, i)
+
+
 @pytest.mark.parametrize("path", __import__("fixture_util").fixtures(), ids=lambda p: p.stem)
 def test_golden_fixtures_fp64(path):
     """The HIP path against the committed golden outputs (oracle + brute force agreed on them)."""
