@@ -9,8 +9,9 @@ are resident in HBM before the timed region.
 
 Prints ONE JSON line: steps/s of the FP64 path (value), with the FP32-MFMA and split-BF16 paths
 and their relative error against FP64 beside it, the live roofline of the dominant kernel (the
-binary contraction, HIP events on the library's stream) and the FP64 CPU oracle timed on a
-bounded sample on the host cores.
+binary contraction, HIP events on the library's stream around the kernels of every 8th timed
+step: events around every step would add about 14 us to each) and the FP64 CPU oracle timed on
+a bounded sample on the host cores.
 """
 import argparse
 import json
@@ -25,6 +26,7 @@ sys.path.insert(0, str(ROOT / "diplodocustransport.jl_amd"))
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 PEAK_HBM_GBS = 8000.0
+TIME_EVERY = 8   # kernel durations from HIP events on every 8th timed step
 
 DESCRIPTIONS = {
     "transport_1d": "2 species x 256 states (32 p x 8 u x 1 phi), 128 periodic x cells, 65536 "
@@ -70,7 +72,7 @@ def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True):
         dt, _ = m(t, big, w.dt, verbose)
         t += dt
     m.synchronize()
-    m.kernel_timing(enable=True, reset=True)
+    m.kernel_timing(enable=True, reset=True, every=TIME_EVERY if steps >= 10 * TIME_EVERY else 1)
     t0 = time.perf_counter()
     for _ in range(steps):
         dt, _ = m(t, big, w.dt, verbose)

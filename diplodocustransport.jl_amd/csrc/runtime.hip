@@ -39,8 +39,8 @@ struct HostChannel {
 };
 
 struct TimedEvent {
-    hipEvent_t start, stop;
-    int cls;  // 0 binary, 1 update
+    hipEvent_t start = nullptr, stop = nullptr;
+    int cls = 0;  // 0 binary, 1 update
 };
 
 }  // namespace
@@ -127,6 +127,7 @@ struct dtx_handle {
     std::vector<void*> allocations;
 
     bool timing = false;
+    int32_t timing_every = 1;   // time the kernels of every timing_every-th step
     std::vector<TimedEvent> events;
     dtx_kernel_times times{};
 };
@@ -905,7 +906,8 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
 }
 
 static int timed_begin(dtx_handle* h, int cls, TimedEvent* ev) {
-    if (!h->timing) return DTX_OK;
+    *ev = TimedEvent{};
+    if (!h->timing || h->step_counter % h->timing_every != 0) return DTX_OK;
     ev->cls = cls;
     HIP_CHECK(h, hipEventCreate(&ev->start));
     HIP_CHECK(h, hipEventCreate(&ev->stop));
@@ -914,7 +916,7 @@ static int timed_begin(dtx_handle* h, int cls, TimedEvent* ev) {
 }
 
 static int timed_end(dtx_handle* h, TimedEvent* ev) {
-    if (!h->timing) return DTX_OK;
+    if (!ev->start) return DTX_OK;
     HIP_CHECK(h, hipEventRecord(ev->stop, h->stream));
     h->events.push_back(*ev);
     return DTX_OK;
@@ -1876,7 +1878,8 @@ int dtx_kernel_timing(dtx_handle* h, int32_t enable, int32_t reset, dtx_kernel_t
     h->events.clear();
     if (out) *out = h->times;
     if (reset) h->times = dtx_kernel_times{};
-    h->timing = enable != 0;
+    h->timing = enable > 0;
+    h->timing_every = enable > 0 ? enable : 1;
     return drain_status(h);
 }
 

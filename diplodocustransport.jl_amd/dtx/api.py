@@ -249,9 +249,12 @@ class _SteppingStruct:
         _lib.check(self.lib.dtx_work(self._h, C.byref(w)), self._h, "dtx_work")
         return w
 
-    def kernel_timing(self, enable: bool, reset: bool = True) -> _lib.KernelTimes:
+    def kernel_timing(self, enable: bool, reset: bool = True, every: int = 1) -> _lib.KernelTimes:
+        """Collect the kernel times of the window so far; from now on time the kernels of every
+        `every`-th step (enable) or none."""
         t = _lib.KernelTimes()
-        _lib.check(self.lib.dtx_kernel_timing(self._h, int(enable), int(reset), C.byref(t)),
+        _lib.check(self.lib.dtx_kernel_timing(self._h, max(1, int(every)) if enable else 0,
+                                              int(reset), C.byref(t)),
                    self._h, "dtx_kernel_timing")
         return t
 

@@ -210,7 +210,9 @@ typedef struct {
 } dtx_work_info;
 int dtx_work(const dtx_handle* h, dtx_work_info* out);
 /* Device time of the last synchronised window, per kernel class, in milliseconds (HIP events
- * on the handle's stream); reset by passing reset != 0. */
+ * on the handle's stream); reset by passing reset != 0.  enable = k > 0 times the kernels of
+ * every k-th step from then on (1: every step; each timed step costs the stream a few
+ * microseconds of event records), 0 stops timing. */
 typedef struct {
     double bin_ms, update_ms;
     int64_t bin_launches, update_launches;

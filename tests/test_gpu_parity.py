@@ -171,3 +171,23 @@ def test_golden_fixtures_fp64(path):
     np.testing.assert_array_equal(out.t, z["t_out"])
     for i in range(len(out.f)):
         assert rel_linf(out.f[i], z["f_out"][i]) < TOL64, (path.stem, i)
+
+
+def test_sampled_kernel_timing():
+    """Kernel timing of every k-th step records k-times fewer launches, and timing does not
+    change the state."""
+    w = workloads.tiny(steps=12)
+    a = workloads.make_struct(w)
+    b = workloads.make_struct(w)
+    a.kernel_timing(enable=True, reset=True, every=4)
+    t = 0.0
+    for _ in range(12):
+        dt, _ = a(t, 1e300, w.dt, 0)
+        b(t, 1e300, w.dt, 0)
+        t += dt
+    kt = a.kernel_timing(enable=False, reset=True)
+    assert kt.bin_launches == 3 and kt.update_launches == 3
+    assert kt.bin_ms > 0.0 and kt.update_ms > 0.0
+    np.testing.assert_array_equal(a.f, b.f)
+    a.close()
+    b.close()
