@@ -179,7 +179,14 @@ hipError_t launch_binary_wide(int precision, int ns, const BinWork* items, int n
                               hipStream_t stream);
 hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,
                             int rows, int nktiles, double* A, hipStream_t stream);
-hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);
+hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);  // update + finalisation
+// Forward Euler update without its finalisation, running the single-workgroup finalisation of
+// the previous step (prev, or none) beside it; only for grids where finalize_is_single holds.
+// The two steps' block partials must be in different buffers.
+hipError_t launch_euler_update_deferred(const UpdateArgs& a, const UpdateArgs* prev,
+                                        hipStream_t stream);
+bool finalize_is_single(const UpdateArgs& a);
+hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* block_start,

@@ -506,10 +506,17 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
                  INFINITY, INFINITY);
 }
 
-__global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
+// Grid (C, nseg), or (C + 1, nseg) with the previous step's finalisation folded in: workgroup
+// (C, 0) runs the single-workgroup finalisation of `fin` (that step's arguments; its block
+// partials are in the other of two buffers) beside the update, and (C, seg > 0) exit.
+__global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a, UpdateArgs fin) {
     extern __shared__ double smem[];
     __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
+    if ((int)blockIdx.x == a.C) {
+        if (blockIdx.y == 0) finalize_single(fin, gridDim.y, red);
+        return;
+    }
     euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
 }
 
@@ -869,7 +876,11 @@ static dim3 update_grid(const UpdateArgs& a) {
     return dim3(a.C, (a.N + UPDATE_STATES - 1) / UPDATE_STATES);
 }
 
-static hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream) {
+bool finalize_is_single(const UpdateArgs& a) {
+    return fold_groups(a.C, a.S, (int)update_grid(a).y) <= FINALIZE_SINGLE_GROUPS;
+}
+
+hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream) {
     const int nseg = (int)update_grid(a).y, ng = fold_groups(a.C, a.S, nseg);
     if (ng <= FINALIZE_SINGLE_GROUPS) {
         finalize_single_kernel<<<1, UB, 0, stream>>>(a, nseg);
@@ -883,9 +894,17 @@ static hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
-    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
+    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a, a);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize(a, stream);
+}
+
+hipError_t launch_euler_update_deferred(const UpdateArgs& a, const UpdateArgs* prev,
+                                        hipStream_t stream) {
+    dim3 g = update_grid(a);
+    if (prev) g.x += 1;
+    euler_update_kernel<<<g, UB, update_smem(a), stream>>>(a, prev ? *prev : a);
+    return hipGetLastError();
 }
 
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream) {

@@ -112,6 +112,13 @@ struct dtx_handle {
     int32_t *d_state_species = nullptr, *d_species_off = nullptr;
     double *d_moments = nullptr, *d_mom_part = nullptr, *d_block_part = nullptr,
            *d_status = nullptr;
+    // Forward Euler on small grids: a step's finalisation runs inside the next step's update
+    // launch (or before the next host synchronisation), so consecutive steps alternate between
+    // two buffers of block partials
+    int64_t bp_words = 0, mp_words = 0;   // doubles per buffer of block_part / mom_part
+    int bp_parity = 0;
+    bool fin_pending = false;
+    UpdateArgs fin_args{};
     uint8_t* d_active = nullptr;
     EllMatrix ell[3];
     double* d_emi = nullptr;
@@ -183,12 +190,23 @@ static void dfree(dtx_handle* h, T** p, int64_t count) {
     }
 }
 
+// Synchronise the handle's stream after running a finalisation still pending from the last
+// step, so the status ring and the moments are complete when the host reads them.
+static hipError_t stream_sync(dtx_handle* h) {
+    if (h->fin_pending) {
+        h->fin_pending = false;
+        hipError_t e = launch_finalize(h->fin_args, h->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(h->stream);
+}
+
 template <typename T>
 static int upload(dtx_handle* h, T* dst, const T* src, int64_t count) {
     if (count <= 0) return DTX_OK;
     HIP_CHECK(h, hipMemcpyAsync(dst, src, (size_t)count * sizeof(T), hipMemcpyHostToDevice,
                                 h->stream));
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     return DTX_OK;
 }
 
@@ -473,7 +491,7 @@ static int pack_loss_channels(dtx_handle* h) {
         HIP_CHECK(h, launch_pack_loss(h->d_loss, h->N, h->d_unpad, h->Np, h->sp_off[s], pc.No,
                                       pc.nktiles, (double*)pc.A, h->stream));
     }
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     return DTX_OK;
 }
 
@@ -963,7 +981,17 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     if (rc) return rc;
     if (h->method == DTX_METHOD_FORWARD_EULER) {
         UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_F);
-        HIP_CHECK(h, launch_euler_update(a, h->stream));
+        if (finalize_is_single(a)) {
+            a.block_part = h->d_block_part + h->bp_parity * h->bp_words;
+            a.mom_part = h->d_mom_part + h->bp_parity * h->mp_words;
+            HIP_CHECK(h, launch_euler_update_deferred(a, h->fin_pending ? &h->fin_args : nullptr,
+                                                      h->stream));
+            h->fin_args = a;
+            h->fin_pending = true;
+            h->bp_parity ^= 1;
+        } else {
+            HIP_CHECK(h, launch_euler_update(a, h->stream));
+        }
     } else {
         UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_P);
         HIP_CHECK(h, launch_symplectic_momentum(a, h->stream));
@@ -977,7 +1005,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     h->cur ^= 1;
     h->step_counter++;
     if (h->step_counter - h->reported >= STATUS_RING / 2) {
-        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+        HIP_CHECK(h, stream_sync(h));
         return drain_status(h);
     }
     return DTX_OK;
@@ -1132,7 +1160,7 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
             g->err = h->err;
             return rc;
         }
-        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        if (stream_sync(h) != hipSuccess) {
             g->err = "member stream failed";
             return DTX_ERR_HIP;
         }
@@ -1346,8 +1374,10 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     // per-block partials, then the fold partials of the finalize kernel
     const int64_t nfold =
         (std::max(C * nseg, C * (int64_t)h->S) + UPDATE_THREADS - 1) / UPDATE_THREADS;
-    rc |= dalloc(hp, &h->d_block_part, (C * nseg + nfold) * BP_WORDS);
-    rc |= dalloc(hp, &h->d_mom_part, C * nseg * h->S * 2);
+    h->bp_words = (C * nseg + nfold) * BP_WORDS;
+    h->mp_words = C * nseg * h->S * 2;
+    rc |= dalloc(hp, &h->d_block_part, 2 * h->bp_words);
+    rc |= dalloc(hp, &h->d_mom_part, 2 * h->mp_words);
     rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
     if (h->method == DTX_METHOD_FORWARD_SYMPLECTIC_EULER) {
         rc |= dalloc(hp, &h->d_fmom, n);
@@ -1665,7 +1695,7 @@ int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
                                                loss_on_first, h->stream));
             }
         }
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     h->synth_binary = true;
     h->binary = true;
     h->binary_dirty = true;
@@ -1675,7 +1705,7 @@ int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale) {
 int dtx_reset(dtx_handle* h) {
     if (!h) return DTX_ERR_INVALID;
     if (!h->vectors_set) return fail(h, DTX_ERR_STATE, "dtx_set_vectors not called");
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     int rc = drain_status(h);
     if (rc) return rc;
     h->cur = 0;
@@ -1683,7 +1713,7 @@ int dtx_reset(dtx_handle* h) {
                                 hipMemcpyDeviceToDevice, h->stream));
     rc = refresh_fT(h);
     if (rc) return rc;
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     h->step_counter = 0;
     h->reported = 0;
     return DTX_OK;
@@ -1693,7 +1723,7 @@ int dtx_get_f(dtx_handle* h, double* f) {
     if (!h || !f) return DTX_ERR_INVALID;
     HIP_CHECK(h, hipMemcpyAsync(f, h->d_f[h->cur], (size_t)h->n * 8, hipMemcpyDeviceToHost,
                                 h->stream));
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     return drain_status(h);
 }
 
@@ -1703,16 +1733,17 @@ int dtx_set_f(dtx_handle* h, const double* f) {
     if (rc) return rc;
     rc = refresh_fT(h);
     if (rc) return rc;
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     return DTX_OK;
 }
 
 int dtx_moments(dtx_handle* h, double* number, double* energy) {
     if (!h || !number || !energy) return DTX_ERR_INVALID;
     std::vector<double> m((size_t)h->C * h->S * 2);
+    HIP_CHECK(h, stream_sync(h));   // the moments of the last step are complete
     HIP_CHECK(h, hipMemcpyAsync(m.data(), h->d_moments, m.size() * 8, hipMemcpyDeviceToHost,
                                 h->stream));
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     for (size_t i = 0; i < (size_t)h->C * h->S; ++i) {
         number[i] = m[2 * i];
         energy[i] = m[2 * i + 1];
@@ -1722,7 +1753,7 @@ int dtx_moments(dtx_handle* h, double* number, double* energy) {
 
 int dtx_synchronize(dtx_handle* h) {
     if (!h) return DTX_ERR_INVALID;
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     return drain_status(h);
 }
 
@@ -1744,7 +1775,7 @@ int dtx_step(dtx_handle* h, double t_start, double t_stop, double dt, int32_t ve
     if (dt_out) *dt_out = dt;
     if (save_out) *save_out = save;
     if (stats) {
-        HIP_CHECK(h, hipStreamSynchronize(h->stream));
+        HIP_CHECK(h, stream_sync(h));
         double st[STATUS_WORDS];
         HIP_CHECK(h, hipMemcpy(st, h->d_status + (idx % STATUS_RING) * STATUS_WORDS,
                                sizeof st, hipMemcpyDeviceToHost));
@@ -1803,7 +1834,7 @@ int dtx_solve(dtx_handle* h, double dt_initial, int64_t n_save, const double* t_
         if (e != hipSuccess) rc = fail(h, DTX_ERR_HIP, "snapshot copy: %s", hipGetErrorString(e));
         t_out[i] = t_stop;
     }
-    hipError_t e = hipStreamSynchronize(h->stream);
+    hipError_t e = stream_sync(h);
     if (rc == DTX_OK && e != hipSuccess)
         rc = fail(h, DTX_ERR_HIP, "solve: %s", hipGetErrorString(e));
     if (rc == DTX_OK) {
@@ -1817,7 +1848,7 @@ int dtx_solve(dtx_handle* h, double dt_initial, int64_t n_save, const double* t_
 
 int dtx_recent_stats(dtx_handle* h, dtx_step_stats* out, int32_t max_steps, int32_t* count) {
     if (!h || !out || !count) return DTX_ERR_INVALID;
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     std::vector<double> ring((size_t)STATUS_RING * STATUS_WORDS);
     HIP_CHECK(h, hipMemcpy(ring.data(), h->d_status, ring.size() * 8, hipMemcpyDeviceToHost));
     int64_t avail = std::min<int64_t>(h->step_counter, STATUS_RING);
@@ -1861,7 +1892,7 @@ int dtx_work(const dtx_handle* hc, dtx_work_info* out) {
 
 int dtx_kernel_timing(dtx_handle* h, int32_t enable, int32_t reset, dtx_kernel_times* out) {
     if (!h) return DTX_ERR_INVALID;
-    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    HIP_CHECK(h, stream_sync(h));
     for (auto& ev : h->events) {
         float ms = 0;
         HIP_CHECK(h, hipEventElapsedTime(&ms, ev.start, ev.stop));

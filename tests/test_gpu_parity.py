@@ -191,3 +191,29 @@ def test_sampled_kernel_timing():
     np.testing.assert_array_equal(a.f, b.f)
     a.close()
     b.close()
+
+
+def test_status_of_unsynchronised_steps():
+    """Steps enqueued without host synchronisation (each step's finalisation then runs inside
+    the next step's update launch) report the same per-step status, moments and state as steps
+    synchronised one by one."""
+    w = workloads.tiny(steps=10, injection=True)
+    a = workloads.make_struct(w)
+    b = workloads.make_struct(w)
+    t = 0.0
+    sync_stats = []
+    for _ in range(10):
+        dt, _, st = a(t, 1e300, w.dt, 3, stats=True)
+        b(t, 1e300, w.dt, 3)
+        sync_stats.append(st)
+        t += dt
+    later = b.recent_stats(10)
+    assert len(later) == 10
+    for x, y in zip(sync_stats, later):
+        for name, _ in x._fields_:
+            assert getattr(x, name) == getattr(y, name), name
+    np.testing.assert_array_equal(a.f, b.f)
+    for u, v in zip(a.moments(), b.moments()):
+        np.testing.assert_array_equal(u, v)
+    a.close()
+    b.close()
