@@ -42,6 +42,10 @@ struct Tiling {
     static constexpr int MC = WCOLS / 16;
 };
 
+// Row pitch of the epilogue's cell-major transpose tiles in LDS: 64 rows + 1 against bank
+// conflicts of the result-layout stores.
+constexpr int EPI_PITCH = 65;
+
 // The work table's pointers are generic to the compiler, which would emit flat memory
 // instructions for them.  Flat loads count against the LDS counter as well, so every wait for an
 // LDS read would also wait for the operator loads prefetched for the next stage, and their
@@ -88,21 +92,41 @@ __device__ inline int tile_row(int row16, int lane, int e) {
     return row16 + (ROWSTEP == 1 ? 4 * (lane >> 4) + e : (lane >> 4) + 4 * e);
 }
 
-// Write one tile of FP64 partial sums (LOSS items are first multiplied by f of their row).
-template <int ROWSTEP>
-__device__ inline void store_tile(const ItemView& v, int row16, int col16, int lane,
-                                  const double* t4) {
-    const int cell = v.c0 + col16 + (lane & 15);
-    if (cell >= v.cells) return;
-    GLB double* dst = v.part + (int64_t)cell * v.N + v.row_base;
+// Write an item's 64 x TC tile of FP64 partial sums through LDS.  Per column c of 16-cell
+// tiles, every wave puts its result tiles (val(r, c, e) at row tile_row<ROWSTEP>, cell l % 16;
+// LOSS rows multiplied by their own f) into T cell-major; then lane l writes row l of one cell
+// per store, 512 contiguous bytes instead of 16 pieces of 32.  T (WC x 16 x EPI_PITCH doubles)
+// may alias the staging tiles once the main loop has ended with a barrier.
+template <int TC, int ROWSTEP, typename Get>
+__device__ inline void store_item(const ItemView& v, double (*T)[16 * EPI_PITCH], Get val) {
+    using TL = Tiling<TC>;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wk = wave % TL::WK, wc = wave / TL::WK;
+    const bool loss = v.kind == CH_LOSS;
+    const int krow_out = v.krow0 + lane;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int krow = v.krow0 + tile_row<ROWSTEP>(row16, lane, e);
-        if (krow < v.rows) {
-            double x = t4[e];
-            if (v.kind == CH_LOSS) x = x * v.fT[(int64_t)(v.row_pad_base + krow) * v.ldc + cell];
-            dst[krow] = x;
+    for (int c = 0; c < TL::MC; ++c) {
+        const int cl = wc * TL::WCOLS + c * 16 + (lane & 15);   // cell within the tile
+        double* t = T[wc] + (lane & 15) * EPI_PITCH;
+#pragma unroll
+        for (int r = 0; r < TL::MR; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = tile_row<ROWSTEP>(wk * TL::WROWS + r * 16, lane, e);
+                double x = val(r, c, e);
+                if (loss && v.krow0 + row < v.rows && v.c0 + cl < v.cells)
+                    x = x * v.fT[(int64_t)(v.row_pad_base + v.krow0 + row) * v.ldc + v.c0 + cl];
+                t[row] = x;
+            }
+        __syncthreads();
+        // the 16 WC cells of this column of tiles, every fourth to each wave
+        for (int q = wave; q < 16 * TL::WC; q += 4) {
+            const int g = q >> 4, ci = q & 15;
+            const int cell = v.c0 + g * TL::WCOLS + c * 16 + ci;
+            if (cell < v.cells && krow_out < v.rows)
+                v.part[(int64_t)cell * v.N + v.row_base + krow_out] = T[g][ci * EPI_PITCH + lane];
         }
+        __syncthreads();
     }
 }
 
@@ -133,8 +157,13 @@ __device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
 template <int TC>
 struct SmemF64 {
     static constexpr int HK = KC / 2;   // j per LDS stage
-    double As[2][HK * TK];
-    double Bs[2][HK * TC];
+    union {
+        struct {
+            double As[2][HK * TK];
+            double Bs[2][HK * TC];
+        } s;
+        double T[Tiling<TC>::WC][16 * EPI_PITCH];   // epilogue transpose
+    };
     uint32_t ctab[MAX_RANGE];
 };
 
@@ -145,8 +174,8 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     constexpr int AP = HK * TK / 256;   // 4 A elements per thread
     constexpr int BP = HK * TC / 256;   // B elements per thread (consecutive cells)
     constexpr int BT = TC / BP;         // threads per B row
-    auto& As = sm.As;
-    auto& Bs = sm.Bs;
+    auto& As = sm.s.As;
+    auto& Bs = sm.s.Bs;
     uint32_t* ctab = sm.ctab;
 
     const ItemView v = load_item(w, TC, 8);
@@ -276,13 +305,7 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
         }
         return;
     }
-#pragma unroll
-    for (int r = 0; r < T::MR; ++r)
-#pragma unroll
-        for (int c = 0; c < T::MC; ++c) {
-            const double t4[4] = {acc[r][c][0], acc[r][c][1], acc[r][c][2], acc[r][c][3]};
-            store_tile<4>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, t4);
-        }
+    store_item<TC, 4>(v, sm.T, [&](int r, int c, int e) { return acc[r][c][e]; });
 }
 
 template <int TC>
@@ -298,8 +321,13 @@ constexpr int FLUSH_CHUNKS = 16;
 
 template <int TC>
 struct SmemF32 {
-    float As[2][KC * TK];
-    float Bs[2][KC * TC];
+    union {
+        struct {
+            float As[2][KC * TK];
+            float Bs[2][KC * TC];
+        } s;
+        double T[Tiling<TC>::WC][16 * EPI_PITCH];   // epilogue transpose
+    };
     uint32_t ctab[MAX_RANGE];
 };
 
@@ -309,8 +337,8 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
     constexpr int AP = KC * TK / 256;   // 8
     constexpr int BP = KC * TC / 256;
     constexpr int BT = TC / BP;
-    auto& As = sm.As;
-    auto& Bs = sm.Bs;
+    auto& As = sm.s.As;
+    auto& Bs = sm.s.Bs;
     uint32_t* ctab = sm.ctab;
 
     const ItemView v = load_item(w, TC, 4);
@@ -393,11 +421,7 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
         __syncthreads();
         buf ^= 1;
     }
-#pragma unroll
-    for (int r = 0; r < T::MR; ++r)
-#pragma unroll
-        for (int c = 0; c < T::MC; ++c)
-            store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, tot[r][c]);
+    store_item<TC, 1>(v, sm.T, [&](int r, int c, int e) { return tot[r][c][e]; });
 }
 
 // LOSS items (FP64 in every mode) run the FP64 body inside the same launch.
@@ -420,22 +444,27 @@ __global__ __launch_bounds__(256) void bin_f32_kernel(const BinWork* __restrict_
 // read as whole linear 1 KB blocks: no bank conflicts, and no padding.
 template <int TC>
 struct SmemBF16 {
-    u32x4 Ah[2][TK * KC / 8];   // 16-byte lane quads: [row block][lane]
-    u32x4 Al[2][TK * KC / 8];
-    u32x4 Bh[2][TC * KC / 8];   // [cell block][lane]
-    u32x4 Bl[2][TC * KC / 8];
-    double Fs[2][TC];
+    union {
+        struct {
+            u32x4 Ah[2][TK * KC / 8];   // 16-byte lane quads: [row block][lane]
+            u32x4 Al[2][TK * KC / 8];
+            u32x4 Bh[2][TC * KC / 8];   // [cell block][lane]
+            u32x4 Bl[2][TC * KC / 8];
+            double Fs[2][TC];
+        } s;
+        double T[Tiling<TC>::WC][16 * EPI_PITCH];   // epilogue transpose
+    };
     uint32_t ctab[MAX_RANGE];
 };
 
 template <int TC>
 __device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) {
     using T = Tiling<TC>;
-    auto& Ah = sm.Ah;
-    auto& Al = sm.Al;
-    auto& Bh = sm.Bh;
-    auto& Bl = sm.Bl;
-    auto& Fs = sm.Fs;
+    auto& Ah = sm.s.Ah;
+    auto& Al = sm.s.Al;
+    auto& Bh = sm.s.Bh;
+    auto& Bl = sm.s.Bl;
+    auto& Fs = sm.s.Fs;
     uint32_t* ctab = sm.ctab;
 
     const ItemView v = load_item(w, TC, 2);
@@ -533,11 +562,7 @@ __device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) 
         __syncthreads();
         buf ^= 1;
     }
-#pragma unroll
-    for (int r = 0; r < T::MR; ++r)
-#pragma unroll
-        for (int c = 0; c < T::MC; ++c)
-            store_tile<1>(v, wk * T::WROWS + r * 16, wc * T::WCOLS + c * 16, lane, tot[r][c]);
+    store_item<TC, 1>(v, sm.T, [&](int r, int c, int e) { return tot[r][c][e]; });
 }
 
 template <int TC>
@@ -561,10 +586,6 @@ __global__ __launch_bounds__(256) void bin_bf16x3_kernel(const BinWork* __restri
 // copies in MFMA fragment order: A from the packed operator, B from the cell-major hi/lo split
 // of f.  The row multiplier f_i is staged beside each chunk and folded in FP32 when the row
 // ends, tot += f_i * acc; tot is written to the item's FP64 slot at the end.
-// Row pitch of the epilogue's cell-major transpose tiles in LDS: 64 rows + 1 against bank
-// conflicts of the result-layout stores.
-constexpr int EPI_PITCH = 65;
-
 template <int NS>
 struct SmemWide {
     union {
