@@ -28,6 +28,7 @@
 namespace dtx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte load / LDS store
@@ -155,7 +156,7 @@ __device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
 // staging stores cost about 7 bank-conflict cycles per instruction; the 16-cell shape, which
 // streams its operator from HBM, keeps them, as operand order measured 3% slower there).
 template <int TC>
-struct SmemF64 {
+struct alignas(16) SmemF64 {
     static constexpr int HK = KC / 2;   // j per LDS stage
     union {
         struct {
@@ -185,27 +186,30 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     const GLB double* B = (const GLB double*)v.B;
     const int bq = tid / BT, bcc = (tid % BT) * BP;
     const bool loss = v.kind >= CH_LOSS;   // B = f_j; LOSS and MOMENT rows take their own f
-    // operand order (TC = 64): element q 256 + tid of a stage tile is in block 4 q + wave = k step
-    // q, row / cell block `wave`, at lane `lane`; its row 16 wave + lane % 16 (A) or cell
-    // 16 wave + lane % 16 (B), its k 4 q + lane / 16
+    // operand order (TC = 64), in 16-byte pieces: elements q 512 + 2 tid and the next (q = 0, 1)
+    // of a stage tile are in block 8 q + tid / 32 = 4 ks + b with k step ks = 2 q + tid / 128 and
+    // row / cell block b = (tid / 32) % 4, at lanes l0 = 2 (tid % 32) and l0 + 1: the same k,
+    // 4 ks + l0 / 16, and consecutive rows (A: 16 b + l0 % 16) or cells (B, likewise)
     constexpr bool frag = TC == 64;
-    const int orow = 16 * wave + (lane & 15), kl = lane >> 4;
+    const int l0 = 2 * (tid & 31);
+    const int prow = 16 * ((tid >> 5) & 3) + (l0 & 15), pk = 4 * (tid >> 7) + (l0 >> 4);
     load_ctab(v, ctab);
 
     double ra[AP], rb[BP], rf[BP];
+    d2 pa[2], pb[2], pf = d2{0.0, 0.0};
     // stage st = half (st & 1) of local chunk (st >> 1)
     auto load_stage = [&](int st) {
         const int ch = st >> 1, h = st & 1;
         const uint32_t ce = ctab[ch];
         const int j0 = (int)(ce >> 16) + h * HK;
         if constexpr (frag) {
-            const GLB double* src = A + (int64_t)ch * KC * TK + (int64_t)(h * HK + kl) * TK + orow;
+            const GLB double* src = A + (int64_t)ch * KC * TK + (int64_t)(h * HK + pk) * TK + prow;
 #pragma unroll
-            for (int q = 0; q < AP; ++q) ra[q] = src[4 * q * TK];
-            const GLB double* b = B + (int64_t)(j0 + kl) * v.ldc + v.c0 + orow;
+            for (int q = 0; q < 2; ++q) pa[q] = *(const GLB d2*)(src + 8 * q * TK);
+            const GLB double* b = B + (int64_t)(j0 + pk) * v.ldc + v.c0 + prow;
 #pragma unroll
-            for (int q = 0; q < BP; ++q) rb[q] = b[(int64_t)(4 * q) * v.ldc];
-            if (!loss) rf[0] = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + orow];
+            for (int q = 0; q < 2; ++q) pb[q] = *(const GLB d2*)(b + (int64_t)(8 * q) * v.ldc);
+            if (!loss) pf = *(const GLB d2*)(v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + prow);
         } else {
             const GLB double* src = A + (int64_t)ch * KC * TK + h * HK * TK + tid * AP;
 #pragma unroll
@@ -223,9 +227,9 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     auto store_stage = [&](int buf) {
         if constexpr (frag) {
 #pragma unroll
-            for (int q = 0; q < AP; ++q) As[buf][q * 256 + tid] = ra[q];
+            for (int q = 0; q < 2; ++q) *(d2*)&As[buf][q * 512 + 2 * tid] = pa[q];
 #pragma unroll
-            for (int q = 0; q < BP; ++q) Bs[buf][q * 256 + tid] = loss ? rb[q] : rf[0] * rb[q];
+            for (int q = 0; q < 2; ++q) *(d2*)&Bs[buf][q * 512 + 2 * tid] = loss ? pb[q] : pf * pb[q];
         } else {
 #pragma unroll
             for (int q = 0; q < AP; ++q) As[buf][tid * AP + q] = ra[q];
@@ -770,7 +774,7 @@ __global__ __launch_bounds__(128 * NS) void bin_bf16x3_wide_kernel(
 // cells), element l of a block being what lane l supplies (A[row l % 16][k l / 16],
 // B[k l / 16][cell l % 16]), so every LDS store and operand read of a wave is one linear
 // 512-byte block.
-struct SmemWide64 {
+struct alignas(16) SmemWide64 {
     static constexpr int HK = KC / 2;   // j per LDS stage
     union {
         struct {
@@ -790,43 +794,40 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64& sm) {
     const ItemView v = load_item(w, WIDE, 8);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave & 1, wc = wave >> 1;    // this wave: rows 64 wk.., cells 64 wc..
-    // Element q 256 + tid (q = 0..7) of a stage tile is in block 4 q + wave = 8 ks + b with
-    // ks = q / 2 and b = 4 (q % 2) + wave, at lane `lane`; its k is 4 ks + lane / 16.  A: row
-    // block b is slice q % 2, rows 16 wave + lane % 16 of the slice.  B: cell 16 b + lane % 16.
-    // A slice beyond the channel's rows (odd k-tile count) reads slice 0 again, and the rows it
-    // feeds are never written.
+    // 16-byte pieces: elements q 512 + 2 tid and the next (q = 0..3) of a stage tile are in
+    // block 8 q + tid / 32 = 8 ks + b (ks = q) at lanes l0 = 2 (tid % 32) and l0 + 1: the same k,
+    // 4 ks + l0 / 16, and consecutive rows or cells.  A: row block b is slice b / 4, rows
+    // 16 (b % 4) + l0 % 16 of the slice.  B: cells 16 b + l0 % 16.  A slice beyond the channel's
+    // rows (odd k-tile count) reads slice 0 again, and the rows it feeds are never written.
     const int64_t slice = w.a_ktile_stride;   // doubles per packed 64-row slice
     const int kt0 = 2 * w.ktile;
     const bool has1 = (kt0 + 1) * TK < v.rows;
-    const int kl = lane >> 4;
-    const GLB double* A0 = (const GLB double*)w.A + (int64_t)kt0 * slice +
-                           (int64_t)v.c_begin * KC * TK + 16 * wave + (lane & 15);
-    const GLB double* A1 = has1 ? A0 + slice : A0;
-    const int cell0 = 16 * wave + (lane & 15), cell1 = cell0 + 64;   // q even, q odd
+    const int b8 = tid >> 5, l0 = 2 * (tid & 31), kl = l0 >> 4;
+    const GLB double* Ab = (const GLB double*)w.A +
+                           (int64_t)(kt0 + ((b8 >> 2) && has1 ? 1 : 0)) * slice +
+                           (int64_t)v.c_begin * KC * TK + 16 * (b8 & 3) + (l0 & 15);
+    const int bcell = 16 * b8 + (l0 & 15);
     load_ctab(v, ctab);
 
-    double ra[8], rb[8], rf0 = 0.0, rf1 = 0.0;
+    d2 ra[4], rb[4], rf;
     // stage st = half (st & 1) of local chunk (st >> 1)
     auto load_stage = [&](int st) {
         const int ch = st >> 1, h = st & 1;
         const uint32_t ce = ctab[ch];
         const int64_t ao = (int64_t)ch * KC * TK + (int64_t)(h * HK + kl) * TK;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) ra[q] = ((q & 1) ? A1 : A0)[ao + 4 * (q / 2) * TK];
-        const GLB double* b = (const GLB double*)v.B +
-                              (int64_t)((int)(ce >> 16) + h * HK + kl) * v.ldc + v.c0;
+        for (int q = 0; q < 4; ++q) ra[q] = *(const GLB d2*)(Ab + ao + 4 * q * TK);
+        const GLB double* bp = (const GLB double*)v.B +
+                               (int64_t)((int)(ce >> 16) + h * HK + kl) * v.ldc + v.c0 + bcell;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            rb[q] = b[(int64_t)(4 * (q / 2)) * v.ldc + ((q & 1) ? cell1 : cell0)];
-        const GLB double* fi = v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0;
-        rf0 = fi[cell0];
-        rf1 = fi[cell1];
+        for (int q = 0; q < 4; ++q) rb[q] = *(const GLB d2*)(bp + (int64_t)(4 * q) * v.ldc);
+        rf = *(const GLB d2*)(v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcell);
     };
     auto store_stage = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) As[buf][q * 256 + tid] = ra[q];
+        for (int q = 0; q < 4; ++q) *(d2*)&As[buf][q * 512 + 2 * tid] = ra[q];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) Bs[buf][q * 256 + tid] = ((q & 1) ? rf1 : rf0) * rb[q];
+        for (int q = 0; q < 4; ++q) *(d2*)&Bs[buf][q * 512 + 2 * tid] = rf * rb[q];
     };
 
     d4 acc[4][4];
