@@ -41,4 +41,15 @@ DTX_HD inline int bf16_frag_index(int r, int t) {
     return (((r >> 4) * 64 + (r & 15) + 16 * (t >> 3)) << 3) + (t & 7);
 }
 
+// Position (in bf16 elements) of the split of f for cell c and padded state kp inside the
+// fragment-ordered copy that the split-BF16 contraction reads as its B operand.  Per 32-state
+// chunk and 16-cell block there is one 1 KB block in the order the 16x16x32 MFMA consumes B:
+// lane (c mod 16) + 16 ((kp mod 32) div 8) holds the 8 consecutive states of its octet, so the
+// B quad of any lane is 16 contiguous bytes and a wave loads a whole block with one
+// instruction.  cblocks = 16-cell blocks per chunk (padded cells / 16).
+DTX_HD inline int64_t fb_index(int c, int kp, int cblocks) {
+    return ((((int64_t)(kp >> 5) * cblocks + (c >> 4)) * 64 + (c & 15) + 16 * ((kp >> 3) & 3))
+            << 3) + (kp & 7);
+}
+
 }  // namespace dtx

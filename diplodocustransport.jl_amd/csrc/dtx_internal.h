@@ -35,7 +35,7 @@ struct BinWork {
     const uint32_t* chunks;   // chunk table: i_pad | (j0_pad << 16)  (padded state indices)
     const double* fT;         // FP64 f, state-major over padded states: fT[kp * ldc + cell]
     const void* B;            // B operand: FP64 / FP32 -> fT (FP64 f is read directly),
-                              // BF16 -> hi bits, cell-major: B[cell * ldb + kp]
+                              // BF16 -> hi bits in fragment order: B[fb_index(cell, kp, ldb)]
     int64_t b_lo;             // BF16: elements from the hi to the lo array
     double* part;             // partial sums of this slot: part[cell * N + row]
                               // (CH_MOMENT: the item's per-cell sums, part[cell]; wide
@@ -49,7 +49,7 @@ struct BinWork {
     int32_t rows;             // valid output rows of the channel (No)
     int32_t cells;            // valid cells of the problem (C)
     int32_t ldc;              // leading dimension of fT (cells, padded)
-    int32_t ldb;              // row pitch of the BF16 B arrays (padded states)
+    int32_t ldb;              // 16-cell blocks of the BF16 B arrays (ldc / 16)
     int32_t N;                // states per cell (partial buffer row pitch)
     int32_t kind;             // ChannelKind
     int32_t tc;               // cells per tile of the item (1: GEMV, 16, 64, or WIDE)
@@ -86,10 +86,10 @@ struct UpdateArgs {
     const double* f;        // current state (cell-major)
     double* f_out;          // next state
     double* fT;             // FP64 state-major copy of f_out (padded states) for the contraction
-    uint16_t* fhi;          // BF16 hi/lo split, cell-major over padded states, or null
+    uint16_t* fhi;          // BF16 hi/lo split in fragment order (fb_index), or null
     uint16_t* flo;
     const int32_t* pad_index;  // padded index of every state (N)
-    int32_t ldb;            // row pitch of fhi / flo
+    int32_t ldb;            // 16-cell blocks of fhi / flo (ldc / 16; fb_index)
     const double* inv_ap;
     const double* df_inj;
     const double* vol;

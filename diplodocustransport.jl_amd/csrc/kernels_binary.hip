@@ -500,10 +500,10 @@ __device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) 
     const GLB uint16_t* A = (const GLB uint16_t*)v.A + (int64_t)v.c_begin * TK * KC * 2;
     const GLB uint16_t* Bhi = (const GLB uint16_t*)v.B;
     const GLB uint16_t* Blo = Bhi + v.b_lo;
-    // B quad of this thread: cell block tid / 64, lane l = tid % 64 -> cell 16 (tid / 64) +
-    // l % 16 and j offset 8 (l / 16); lanes of one cell read one contiguous 64-byte segment.
+    // B quad of this thread: lane tid % 64 of cell block tid / 64 in the fragment-ordered split
+    // of f (fb_index): one linear 1 KB block per wave.
     const bool bthread = tid < TC * 4;
-    const int bcell = 16 * (tid >> 6) + (tid & 15), bk8 = 8 * ((tid >> 4) & 3);
+    const int64_t bq0 = (int64_t)(v.c0 >> 4) * 64 + tid;
     load_ctab(v, ctab);
 
     u32x4 rah, ral, rbh, rbl;
@@ -514,9 +514,9 @@ __device__ __forceinline__ void bf16x3_item(const BinWork& w, SmemBF16<TC>& sm) 
         rah = src[tid];
         ral = src[256 + tid];
         if (bthread) {
-            const int64_t off = (int64_t)(v.c0 + bcell) * v.ldb + (int)(ce >> 16) + bk8;
-            rbh = *(const GLB u32x4*)(Bhi + off);
-            rbl = *(const GLB u32x4*)(Blo + off);
+            const int64_t off = (int64_t)((ce >> 16) >> 5) * v.ldb * 64 + bq0;
+            rbh = ((const GLB u32x4*)Bhi)[off];
+            rbl = ((const GLB u32x4*)Blo)[off];
         }
         if (tid < TC) rfi = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + tid];
     };
@@ -660,8 +660,8 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide<NS>& sm) {
     // B quads of this thread: cell blocks wave + 2 NS q (q < BQ), fragment lane `lane` of each
     // (cell lane % 16 of the block, j octet lane / 16); the four lanes of one cell read one
     // contiguous 64-byte segment, and the LDS stores are linear 1 KB blocks
-    const int64_t boff = (int64_t)(v.c0 + 16 * wave + (lane & 15)) * v.ldb + 8 * (lane >> 4);
-    const int64_t bstep = (int64_t)16 * (2 * NS) * v.ldb;
+    const int64_t boff = ((int64_t)(v.c0 >> 4) + wave) * 64 + lane;   // u32x4 of quad 0
+    constexpr int bstep = 2 * NS * 64;                                 // u32x4 between quads
     load_ctab(v, ctab);
 
     u32x4 rh0, rl0, rh1, rl1;   // A quads (slices sa, sa + NS / 2)
@@ -677,8 +677,9 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide<NS>& sm) {
         const int j0 = (int)(ce >> 16);
 #pragma unroll
         for (int q = 0; q < BQ; ++q) {
-            bh[q] = *(const GLB u32x4*)(Bhi + boff + q * bstep + j0);
-            bl[q] = *(const GLB u32x4*)(Blo + boff + q * bstep + j0);
+            const int64_t off = (int64_t)(j0 >> 5) * v.ldb * 64 + boff + q * bstep;
+            bh[q] = ((const GLB u32x4*)Bhi)[off];
+            bl[q] = ((const GLB u32x4*)Blo)[off];
         }
         if (tid < WIDE) rfi = v.fT[(int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + tid];
     };

@@ -142,15 +142,17 @@ __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, b
 }
 
 // Copies of a new state value for the next contraction: FP64 state-major (padded states), and
-// the cell-major BF16 hi/lo split when that precision is in use.
+// the BF16 hi/lo split in the contraction's fragment order (fb_index) when that precision is in
+// use.
 __device__ inline void publish(const UpdateArgs& a, int c, int k, double x) {
     const int kp = a.pad_index[k];
     a.fT[(int64_t)kp * a.ldc + c] = x;
     if (a.fhi) {
         uint16_t hi, lo;
         split_bf16(x, &hi, &lo);
-        a.fhi[(int64_t)c * a.ldb + kp] = hi;
-        a.flo[(int64_t)c * a.ldb + kp] = lo;
+        const int64_t q = fb_index(c, kp, a.ldb);
+        a.fhi[q] = hi;
+        a.flo[q] = lo;
     }
 }
 
@@ -851,7 +853,7 @@ hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const in
     a.N = N;
     a.C = C;
     a.ldc = ldc;
-    a.ldb = Np;
+    a.ldb = ldc / 16;
     a.pad_index = pad_index;
     a.fT = fT;
     a.fhi = fhi;

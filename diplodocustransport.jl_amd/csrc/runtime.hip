@@ -105,7 +105,7 @@ struct dtx_handle {
     int num_cu = 256;             // compute units of the device (work-item sizing)
     int wide = 0;                 // wide gain items (C > 64): k tiles per item (BF16x3 2 or 4,
                                   // FP64 2), 0 for 64-cell tiles
-    uint16_t* d_fhi = nullptr;    // BF16 split, cell-major: [ldc][Np]
+    uint16_t* d_fhi = nullptr;    // BF16 split in fragment order (fb_index), ldc x Np
     uint16_t* d_flo = nullptr;
     double *d_invap = nullptr, *d_dfinj = nullptr, *d_vol = nullptr, *d_finit = nullptr;
     double* d_eweight = nullptr;
@@ -608,7 +608,7 @@ static int build_work(dtx_handle* h) {
                 w.rows = c.No;
                 w.cells = C;
                 w.ldc = h->ldc;
-                w.ldb = h->Np;
+                w.ldb = h->ldc / 16;
                 w.N = h->N;
                 w.kind = c.moment >= 0 ? CH_MOMENT : c.kind;
                 w.tc = wide ? WIDE : h->tc;
@@ -872,7 +872,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.fhi = h->d_fhi;
     a.flo = h->d_flo;
     a.pad_index = h->d_pad_index;
-    a.ldb = h->Np;
+    a.ldb = h->ldc / 16;
     a.inv_ap = h->d_invap;
     a.df_inj = h->d_dfinj;
     a.vol = h->d_vol;
