@@ -34,7 +34,7 @@ struct BinWork {
     const void* A;            // packed A of the channel, k-tile 0: [ktile][chunk][...]
     const uint32_t* chunks;   // chunk table: i_pad | (j0_pad << 16)  (padded state indices)
     const double* fT;         // FP64 f, state-major over padded states: fT[kp * ldc + cell]
-    const void* B;            // B operand: FP64 / FP32 -> fT (FP64 f is read directly),
+    const void* B;            // B operand: FP64 -> fT; FP32 -> fT32 (FP32 f, state-major),
                               // BF16 -> hi bits in fragment order: B[fb_index(cell, kp, ldb)]
     int64_t b_lo;             // BF16: elements from the hi to the lo array
     double* part;             // partial sums of this slot: part[cell * N + row]
@@ -86,6 +86,7 @@ struct UpdateArgs {
     const double* f;        // current state (cell-major)
     double* f_out;          // next state
     double* fT;             // FP64 state-major copy of f_out (padded states) for the contraction
+    float* fT32;            // FP32 copy in the same layout (FP32 contraction), or null
     uint16_t* fhi;          // BF16 hi/lo split in fragment order (fb_index), or null
     uint16_t* flo;
     const int32_t* pad_index;  // padded index of every state (N)
@@ -193,7 +194,7 @@ hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* blo
                                     int count, int total_blocks, int max_N, hipStream_t stream);
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream);
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
-                                int Np, double* fT, uint16_t* fhi, uint16_t* flo,
+                                int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
                                 hipStream_t stream);
 
 // synthetic generator (synth.hip)

@@ -346,7 +346,7 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wk = wave % T::WK, wc = wave / T::WK;
     const GLB float* A = (const GLB float*)v.A + (int64_t)v.c_begin * KC * TK;
-    const GLB double* B = v.fT;
+    const GLB float* B = (const GLB float*)v.B;   // FP32 f_j; f_i from the FP64 fT
     // Operand order in 16-byte pieces: piece p of a chunk tile holds elements 4 p .. 4 p + 3 of
     // block p / 16 = ks RB + blk (A) or ks CB + blk (B), k step ks and row or cell block blk, at
     // lanes l0 = 4 (p % 16) .. l0 + 3: the same k, 4 ks + l0 / 16, and four consecutive rows or
@@ -370,8 +370,8 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
     }
     load_ctab(v, ctab);
 
-    f4 ra[APC];
-    d4 rb[BPC], rf[BPC];
+    f4 ra[APC], rb[BPC];
+    d4 rf[BPC];
     auto load_chunk = [&](int ch) {
         const uint32_t ce = ctab[ch];
         const GLB float* src = A + (int64_t)ch * KC * TK;
@@ -381,7 +381,7 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
 #pragma unroll
         for (int q = 0; q < BPC; ++q) {
             if (BPIECES % 256 == 0 || q * 256 + tid < BPIECES) {
-                rb[q] = *(const GLB d4*)(B + (int64_t)((int)(ce >> 16) + b_k[q]) * v.ldc + v.c0 +
+                rb[q] = *(const GLB f4*)(B + (int64_t)((int)(ce >> 16) + b_k[q]) * v.ldc + v.c0 +
                                          b_cell[q]);
                 rf[q] = *(const GLB d4*)(fi + b_cell[q]);
             }
@@ -393,7 +393,8 @@ __device__ __forceinline__ void f32_item(const BinWork& w, SmemF32<TC>& sm) {
 #pragma unroll
         for (int q = 0; q < BPC; ++q)
             if (BPIECES % 256 == 0 || q * 256 + tid < BPIECES) {
-                const d4 x = rf[q] * rb[q];
+                const d4 x = rf[q] * d4{(double)rb[q][0], (double)rb[q][1], (double)rb[q][2],
+                                        (double)rb[q][3]};
                 *(f4*)&Bs[buf][4 * (q * 256 + tid)] =
                     f4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
             }

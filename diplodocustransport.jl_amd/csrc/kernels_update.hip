@@ -147,6 +147,7 @@ __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, b
 __device__ inline void publish(const UpdateArgs& a, int c, int k, double x) {
     const int kp = a.pad_index[k];
     a.fT[(int64_t)kp * a.ldc + c] = x;
+    if (a.fT32) a.fT32[(int64_t)kp * a.ldc + c] = (float)x;
     if (a.fhi) {
         uint16_t hi, lo;
         split_bf16(x, &hi, &lo);
@@ -846,7 +847,7 @@ __global__ void publish_state_kernel(UpdateArgs a) {
 }
 
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
-                                int Np, double* fT, uint16_t* fhi, uint16_t* flo,
+                                int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
                                 hipStream_t stream) {
     UpdateArgs a{};
     a.f = f;
@@ -856,6 +857,7 @@ hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const in
     a.ldb = ldc / 16;
     a.pad_index = pad_index;
     a.fT = fT;
+    a.fT32 = fT32;
     a.fhi = fhi;
     a.flo = flo;
     const int64_t total = (int64_t)N * C;

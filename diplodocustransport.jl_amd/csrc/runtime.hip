@@ -102,6 +102,7 @@ struct dtx_handle {
     double* d_f[2] = {nullptr, nullptr};
     int cur = 0;
     double* d_fT = nullptr;       // FP64, state-major, padded states: [Np][ldc]
+    float* d_fT32 = nullptr;      // FP32 contraction: FP32 copy of d_fT
     int num_cu = 256;             // compute units of the device (work-item sizing)
     int wide = 0;                 // wide gain items (C > 64): k tiles per item (BF16x3 2 or 4,
                                   // FP64 2), 0 for 64-cell tiles
@@ -597,6 +598,7 @@ static int build_work(dtx_handle* h) {
                 w.chunks = c.chunks;
                 w.fT = h->d_fT;
                 if (prec == DTX_PREC_FP64) w.B = h->d_fT;
+                else if (prec == DTX_PREC_FP32) w.B = h->d_fT32;
                 else w.B = h->d_fhi;
                 w.b_lo = (bf16 && !loss) ? (int64_t)(h->d_flo - h->d_fhi) : 0;
                 w.a_ktile_stride = chunk_elems(prec) * c.nchunks;
@@ -772,7 +774,7 @@ static int finalize_binary(dtx_handle* h) {
 // Publish the current f into the contraction's B-operand copies (padded, per precision).
 static int refresh_fT(dtx_handle* h) {
     HIP_CHECK(h, launch_publish_state(h->d_f[h->cur], h->N, h->C, h->ldc, h->d_pad_index, h->Np,
-                                      h->d_fT, h->d_fhi, h->d_flo, h->stream));
+                                      h->d_fT, h->d_fT32, h->d_fhi, h->d_flo, h->stream));
     return DTX_OK;
 }
 
@@ -869,6 +871,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
     a.f = h->d_f[h->cur];
     a.f_out = h->d_f[h->cur ^ 1];
     a.fT = h->d_fT;
+    a.fT32 = h->d_fT32;
     a.fhi = h->d_fhi;
     a.flo = h->d_flo;
     a.pad_index = h->d_pad_index;
@@ -1357,6 +1360,7 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     rc |= dalloc(hp, &h->d_f[0], n);
     rc |= dalloc(hp, &h->d_f[1], n);
     rc |= dalloc(hp, &h->d_fT, (int64_t)h->ldc * h->Np);
+    if (h->precision == DTX_PREC_FP32) rc |= dalloc(hp, &h->d_fT32, (int64_t)h->ldc * h->Np);
     if (h->precision == DTX_PREC_BF16X3) {
         rc |= dalloc(hp, &h->d_fhi, (int64_t)h->ldc * h->Np);
         rc |= dalloc(hp, &h->d_flo, (int64_t)h->ldc * h->Np);
@@ -1400,6 +1404,8 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     if (!rc) rc = upload(hp, h->d_species_off, h->sp_off.data(), h->S + 1);
     if (!rc) {
         e = hipMemsetAsync(h->d_fT, 0, (size_t)h->ldc * h->Np * 8, h->stream);
+        if (e == hipSuccess && h->d_fT32)
+            e = hipMemsetAsync(h->d_fT32, 0, (size_t)h->ldc * h->Np * 4, h->stream);
         if (e == hipSuccess && h->d_fhi)
             e = hipMemsetAsync(h->d_fhi, 0, (size_t)h->ldc * h->Np * 2, h->stream);
         if (e == hipSuccess && h->d_flo)
