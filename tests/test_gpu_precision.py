@@ -19,9 +19,9 @@ STEPS = 60
 F_TOL = {"fp32": 1e-6, "bf16x3": 1e-4}
 
 
-def run(w, precision, correction=True, steps=STEPS):
+def run(w, precision, correction=True, steps=STEPS, method="euler"):
     m = workloads.make_struct(w, precision=precision, use_monolithic=False,
-                              moment_correction=correction)
+                              moment_correction=correction, method=method)
     t = 0.0
     for _ in range(steps):
         dt, _ = m(t, 1e300, w.dt, 0)
@@ -104,3 +104,14 @@ def test_correction_must_precede_first_step():
     rc = m.lib.dtx_set_moment_correction(m._h, 0)
     assert rc == -4  # DTX_ERR_STATE
     m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_symplectic_reduced_precision(precision):
+    """ForwardSymplecticEuler with the reduced-precision contraction follows its FP64 run to the
+    same bars (the momentum and space kernels take the same corrected binary sums)."""
+    w = workloads.tiny(p_num=8, u_num=4, x_num=3, monolithic=False, injection=True)
+    f64, n64, e64 = run(w, "fp64", method="symplectic", steps=30)
+    fc, nc, ec = run(w, precision, method="symplectic", steps=30)
+    assert rel_err_f(fc, f64) <= F_TOL[precision]
+    assert np.all(np.isfinite(nc)) and np.all(np.isfinite(ec))

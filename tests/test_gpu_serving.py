@@ -17,7 +17,7 @@ def small_mix():
     return specs
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "bf16x3"])
 def test_alone_equals_mixed_bitwise(precision):
     specs = small_mix()
     mixed, _, f_mixed = serving.run(specs, precision=precision, realtime=False, fetch=True)
