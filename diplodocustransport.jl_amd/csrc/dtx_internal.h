@@ -194,7 +194,7 @@ hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* blo
                                     int count, int total_blocks, int max_N, hipStream_t stream);
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream);
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
-                                int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
+                                double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
                                 hipStream_t stream);
 
 // synthetic generator (synth.hip)

@@ -847,7 +847,7 @@ __global__ void publish_state_kernel(UpdateArgs a) {
 }
 
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
-                                int Np, double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
+                                double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,
                                 hipStream_t stream) {
     UpdateArgs a{};
     a.f = f;

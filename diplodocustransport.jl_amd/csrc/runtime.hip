@@ -773,7 +773,7 @@ static int finalize_binary(dtx_handle* h) {
 
 // Publish the current f into the contraction's B-operand copies (padded, per precision).
 static int refresh_fT(dtx_handle* h) {
-    HIP_CHECK(h, launch_publish_state(h->d_f[h->cur], h->N, h->C, h->ldc, h->d_pad_index, h->Np,
+    HIP_CHECK(h, launch_publish_state(h->d_f[h->cur], h->N, h->C, h->ldc, h->d_pad_index,
                                       h->d_fT, h->d_fT32, h->d_fhi, h->d_flo, h->stream));
     return DTX_OK;
 }
