@@ -474,13 +474,18 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const int k = k0 + (tid >> 2), q = tid & 3;
     const bool mine = k < k1;
+    // the state's own loads and its flux row do not depend on the binary term: issue them first
+    const int64_t r = base + k;
+    double fk = 0.0, iap = 0.0, dinj = 0.0, dF = 0.0;
+    if (mine && q == 0) {
+        fk = a.f[r];
+        iap = a.inv_ap[r];
+        dinj = a.df_inj[r];
+        dF = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f) : 0.0;
+    }
     const double dB = a.binary ? binary_term(a, c, k, q, mine && active) : 0.0;
     if (mine && q == 0) {
-        const int64_t r = base + k;
-        const double fk = a.f[r];
         sumf = fk;
-        const double dF = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
-                                      : 0.0;
         double df = -dF;
         double dE = 0.0;
         if (a.emission) {
@@ -488,7 +493,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
             df = df + dE;
         }
         if (a.binary) df = df + dB;
-        df = df * (a.inv_ap[r] * a.s);
+        df = df * (iap * a.s);
         sumdf = df;
         if (want_cr) {
             cr = cr_min(cr, df / fk);
@@ -498,7 +503,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
                 crf = cr_min(crf, (-dF) / fk);
             }
         }
-        double x = fk + (df + a.df_inj[r] * a.s);
+        double x = fk + (df + dinj * a.s);
         x = strong_zero(x, x >= THR_1F_10);
         x = strong_zero(x, x != INFINITY);
         a.f_out[r] = x;
