@@ -96,8 +96,16 @@ struct UpdateArgs {
     const double* vol;
     const double* part;     // binary partial sums [slot][cell][N] (moment-rate corrected:
                             // the loss sums [cell][N])
-    const double* gsum;     // moment-rate corrected: gain sums [cell][N], else null
-    const double* coef;     // moment-rate corrected: (alpha, beta) [cell][S][2], else null
+    // moment-rate correction (null gsum: off): gain sums [cell][N], the moment partials of the
+    // gain sums [cell][S][zb][3], the moment-rate item sums [index][ldc] and their lists
+    const double* gsum;
+    const double* cpart;
+    const double* mpart;
+    const int32_t* mom_off;
+    const int32_t* mom_list;
+    int32_t zb;
+    double* coef;           // (alpha, beta) [cell][S][2] from correct_coef_kernel, or null: each
+                            // update workgroup derives its own (grids of at most one round)
     const int32_t* nslot;   // slots per state (N)
     const int32_t* nslot32; // leading slots of each state holding FP32 sums (N), or null
     int64_t part_stride;    // doubles between slots
@@ -144,25 +152,21 @@ enum BlockSlot { BP_SUMDF = 0, BP_SUMF, BP_CR, BP_CR_BIN, BP_CR_EMI, BP_CR_FLUX,
 // dBg = sum of a state's gain slots (gsum), dBl = sum of its loss slots (lsum); R_N and R_E are
 // the exact FP64 number and energy rates of the gain term (sums of the moment-rate item sums
 // listed for (s, moment) in mom_off / mom_list).  The update kernels take the corrected binary
-// sum dBg (1 + a + b E) + dBl, with (a, b) = coef[c][s] chosen so that sum dBg' = R_N and
-// sum E dBg' = R_E over the species.
+// sum dBg (1 + a + b E) + dBl, with (a, b) chosen so that sum dBg' = R_N and sum E dBg' = R_E
+// over the species (correction_coef).  CorrectArgs drives the launch that sums the slots.
 struct CorrectArgs {
     const double* part;        // partial slots [slot][cell][N]
     int64_t part_stride;
     double* gsum;              // gain sums [cell][N]
     double* lsum;              // loss sums [cell][N]
     double* cpart;             // moment partials of the gain sums [cell][S][zb][3]
-    double* coef;              // (alpha, beta) [cell][S][2]
     int32_t zb;                // workgroups per species in the sums launch: ceil(max ns / 64)
     const int32_t* species_off;  // (S+1)
     const int32_t* gslots;     // gain slots per species (S)
     const int32_t* nslots;     // gain + loss slots per species (S)
     int32_t gain_f32;          // the gain slots hold FP32 sums (wide split-BF16 items)
     const double* eweight;     // per-state energy weight (N) or null (number only)
-    const double* mpart;       // moment-rate item sums [index][ldc]
-    const int32_t* mom_off;    // (2S+1) offsets into mom_list for (s, moment)
-    const int32_t* mom_list;   // mpart indices, in summation order
-    int32_t N, C, S, ldc;
+    int32_t N, C, S;
 };
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------

@@ -93,15 +93,17 @@ __device__ inline double ell_row(const int32_t* col, const double* val, int widt
 // channels and the FP64 loss channels) times Vol.  The four threads q = 0..3 of the state sum
 // slots q, q+4, ...; the four sums are combined as (s0 + s1) + (s2 + s3) by a butterfly, so every
 // lane holds the same, order-fixed result.
-__device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, bool valid) {
+__device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, bool valid,
+                                     const double* coef) {
     constexpr int SB = 8;  // slot loads in flight per thread
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
-    if (a.coef) {
+    if (a.gsum) {
         // moment-rate corrected: the state's gain sum rescaled by (1 + alpha + beta E_k) of its
-        // (cell, species), plus its loss sum (thread q = 0; the others add zeros)
+        // species (coef: the workgroup's (alpha, beta) per species), plus its loss sum (thread
+        // q = 0; the others add zeros)
         if (valid && q == 0) {
-            const double* cf = a.coef + ((int64_t)c * a.S + a.state_species[k]) * 2;
+            const double* cf = coef + 2 * a.state_species[k];
             const double e = a.eweight ? a.eweight[k] : 0.0;
             g = a.gsum[idx] * (1.0 + (cf[0] + cf[1] * e)) + a.part[idx];
         }
@@ -198,6 +200,67 @@ __device__ inline double wave_combine(int t, bool is_sum, const double* red) {
     for (int w = 1; w < UB / 64; ++w)
         r = is_sum ? r + red[w * RED_VALUES + t] : cr_min(r, red[w * RED_VALUES + t]);
     return r;
+}
+
+// The moment-rate correction's (alpha, beta) of (cell c, species s) into out[0..1], by one wave:
+// the moment partials of the gain sums (workgroups of the sums launch, lane-strided in order)
+// and the moment-rate item sums (lane-strided in list order), one butterfly, then the 2x2
+// system.  Every update workgroup of the cell runs the same arithmetic, so they agree exactly.
+__device__ inline void correction_coef(const UpdateArgs& a, int c, int s, double* out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)c * a.S + s;
+    const int ns = a.species_off[s + 1] - a.species_off[s];
+    const int nz = (ns + UPDATE_STATES - 1) / UPDATE_STATES;
+    double v[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int z = lane; z < nz; z += 64)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] = v[i] + a.cpart[(p * a.zb + z) * 3 + i];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int t0 = a.mom_off[2 * s + m], t1 = a.mom_off[2 * s + m + 1];
+        for (int t = t0 + lane; t < t1; t += 64)
+            v[3 + m] = v[3 + m] + a.mpart[(int64_t)a.mom_list[t] * a.ldc + c];
+    }
+    wave_reduce_batch(v, 0xFFu);
+    if (lane != 0) return;
+    const double S0 = v[0], S1 = v[1], S2 = v[2], RN = v[3], RE = v[4];
+    double alpha = 0.0, beta = 0.0;
+    if (S0 != 0.0) {
+        const double dn = RN - S0, de = RE - S1;
+        const double det = S0 * S2 - S1 * S1;
+        if (a.eweight && fabs(det) > 1e-12 * fabs(S0 * S2)) {
+            alpha = (dn * S2 - de * S1) / det;
+            beta = (de * S0 - dn * S1) / det;
+        } else {
+            alpha = dn / S0;
+        }
+        if (!isfinite(alpha) || !isfinite(beta)) alpha = beta = 0.0;
+    }
+    out[0] = alpha;
+    out[1] = beta;
+}
+
+// The coefficients of cell c for binary_term (indexed 2 species): those of correct_coef_kernel
+// when it ran, else those of every species with states in [k0, k1), derived into the
+// workgroup's LDS array one wave per species (ends with a barrier).
+__device__ inline const double* load_correction(const UpdateArgs& a, int c, int k0, int k1,
+                                                double* lds) {
+    if (a.coef) return a.coef + (int64_t)c * a.S * 2;
+    if (k0 < k1) {
+        const int sa = a.state_species[k0], sb = a.state_species[k1 - 1];
+        for (int s = sa + (int)(threadIdx.x >> 6); s <= sb; s += UB / 64)
+            correction_coef(a, c, s, lds + 2 * s);
+    }
+    __syncthreads();
+    return lds;
+}
+
+// One wave per (cell, species): the coefficients for large update grids, where deriving them in
+// every update workgroup would repeat the work once per round of workgroups.
+__global__ __launch_bounds__(UB) void correct_coef_kernel(UpdateArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * (UB / 64) + (threadIdx.x >> 6);
+    if (p >= (int64_t)a.C * a.S) return;  // the whole wave
+    correction_coef(a, (int)(p / a.S), (int)(p % a.S), a.coef + p * 2);
 }
 
 // Per-species moment partials of this block's new states and the per-block partials (sums and
@@ -474,6 +537,8 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const int k = k0 + (tid >> 2), q = tid & 3;
     const bool mine = k < k1;
+    __shared__ double coef_lds[2 * DTX_MAX_SPECIES];
+    const double* coef = a.gsum ? load_correction(a, c, k0, k1, coef_lds) : nullptr;
     // the state's own loads and its flux row do not depend on the binary term: issue them first
     const int64_t r = base + k;
     double fk = 0.0, iap = 0.0, dinj = 0.0, dF = 0.0;
@@ -483,7 +548,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
         dinj = a.df_inj[r];
         dF = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f) : 0.0;
     }
-    const double dB = a.binary ? binary_term(a, c, k, q, mine && active) : 0.0;
+    const double dB = a.binary ? binary_term(a, c, k, q, mine && active, coef) : 0.0;
     if (mine && q == 0) {
         sumf = fk;
         double df = -dF;
@@ -514,10 +579,19 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
                  INFINITY, INFINITY);
 }
 
+// Grid (C, nseg).
+__global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
+    extern __shared__ double smem[];
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    __shared__ int ired[UB / 64];
+    euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
+}
+
 // Grid (C, nseg), or (C + 1, nseg) with the previous step's finalisation folded in: workgroup
 // (C, 0) runs the single-workgroup finalisation of `fin` (that step's arguments; its block
-// partials are in the other of two buffers) beside the update, and (C, seg > 0) exit.
-__global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a, UpdateArgs fin) {
+// partials are in the other of two buffers) beside the update, and (C, seg > 0) exit.  A kernel
+// of its own: the finalisation's registers would lower the occupancy of the plain update.
+__global__ __launch_bounds__(UB) void euler_update_fin_kernel(UpdateArgs a, UpdateArgs fin) {
     extern __shared__ double smem[];
     __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
@@ -647,7 +721,9 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
     const int k = k0 + (tid >> 2), q = tid & 3;
     const bool mine = k < k1;
-    const double dB = a.binary ? binary_term(a, c, k, q, mine && active) : 0.0;
+    __shared__ double coef_lds[2 * DTX_MAX_SPECIES];
+    const double* coef = a.gsum ? load_correction(a, c, k0, k1, coef_lds) : nullptr;
+    const double dB = a.binary ? binary_term(a, c, k, q, mine && active, coef) : 0.0;
     if (mine && q == 0) {
         const int64_t r = base + k;
         const double dP = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
@@ -723,10 +799,10 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
 // and R_E likewise, come from the moment-rate slots.  Per (cell c, output species s) the gain
 // sums of the species are rescaled per state by (1 + alpha + beta E_k) with (alpha, beta) such
 // that their number and energy sums equal R_N and R_E (number only when no energy weights are
-// set), and the loss sums are added unchanged.  Two launches: the slot sums of every state
-// with per-workgroup moment partials (many workgroups, the slot reads in flight together),
-// then one wave per (c, s) for the coefficients; the update kernels apply them
-// (binary_term).
+// set), and the loss sums are added unchanged.  One launch sums the slots of every state and
+// forms per-workgroup moment partials (many workgroups, the slot reads in flight together);
+// each update workgroup then derives the coefficients of its species (correction_coef) and
+// applies them (binary_term).
 
 // Sum of slots b + q, b + q + 4, ... below e of element idx (in order; FP32 slots if f32).
 // FP32 slots: past the last slot a load repeats slot s0 (valid, unused), so the loads of a
@@ -798,49 +874,8 @@ __global__ __launch_bounds__(UB) void correct_sums_kernel(CorrectArgs a) {
         a.cpart[(((int64_t)c * a.S + s) * a.zb + z) * 3 + tid] = wave_combine(tid, true, red);
 }
 
-// One wave per (cell c, species s): the moments of the gain sums (the workgroup partials in
-// order, lane-strided), the rates R_N and R_E (the listed moment-rate item sums, lane-strided
-// in list order), one butterfly, then (alpha, beta) into coef[c][s].
-__global__ __launch_bounds__(UB) void correct_coef_kernel(CorrectArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int64_t p = (int64_t)blockIdx.x * (UB / 64) + (threadIdx.x >> 6);
-    if (p >= (int64_t)a.C * a.S) return;  // the whole wave
-    const int c = (int)(p / a.S), s = (int)(p % a.S);
-    const int ns = a.species_off[s + 1] - a.species_off[s];
-    const int nz = (ns + UB / 4 - 1) / (UB / 4);
-    double v[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int z = lane; z < nz; z += 64)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) v[i] = v[i] + a.cpart[(p * a.zb + z) * 3 + i];
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int t0 = a.mom_off[2 * s + m], t1 = a.mom_off[2 * s + m + 1];
-        for (int t = t0 + lane; t < t1; t += 64)
-            v[3 + m] = v[3 + m] + a.mpart[(int64_t)a.mom_list[t] * a.ldc + c];
-    }
-    wave_reduce_batch(v, 0xFFu);
-    if (lane != 0) return;
-    const double S0 = v[0], S1 = v[1], S2 = v[2], RN = v[3], RE = v[4];
-    double alpha = 0.0, beta = 0.0;
-    if (S0 != 0.0) {
-        const double dn = RN - S0, de = RE - S1;
-        const double det = S0 * S2 - S1 * S1;
-        if (a.eweight && fabs(det) > 1e-12 * fabs(S0 * S2)) {
-            alpha = (dn * S2 - de * S1) / det;
-            beta = (de * S0 - dn * S1) / det;
-        } else {
-            alpha = dn / S0;
-        }
-        if (!isfinite(alpha) || !isfinite(beta)) alpha = beta = 0.0;
-    }
-    a.coef[p * 2] = alpha;
-    a.coef[p * 2 + 1] = beta;
-}
-
 hipError_t launch_gain_correct(const CorrectArgs& a, hipStream_t stream) {
     correct_sums_kernel<<<dim3(a.C, a.S, a.zb), UB, 0, stream>>>(a);
-    const int64_t pairs = (int64_t)a.C * a.S;
-    correct_coef_kernel<<<(unsigned)((pairs + UB / 64 - 1) / (UB / 64)), UB, 0, stream>>>(a);
     return hipGetLastError();
 }
 
@@ -902,21 +937,34 @@ hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream) {
     return hipGetLastError();
 }
 
+static hipError_t launch_coef(const UpdateArgs& a, hipStream_t stream) {
+    if (!a.gsum || !a.coef) return hipSuccess;
+    const int64_t pairs = (int64_t)a.C * a.S;
+    correct_coef_kernel<<<(unsigned)((pairs + UB / 64 - 1) / (UB / 64)), UB, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
-    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a, a);
+    hipError_t e0 = launch_coef(a, stream);
+    if (e0 != hipSuccess) return e0;
+    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize(a, stream);
 }
 
 hipError_t launch_euler_update_deferred(const UpdateArgs& a, const UpdateArgs* prev,
                                         hipStream_t stream) {
+    hipError_t e0 = launch_coef(a, stream);
+    if (e0 != hipSuccess) return e0;
     dim3 g = update_grid(a);
     if (prev) g.x += 1;
-    euler_update_kernel<<<g, UB, update_smem(a), stream>>>(a, prev ? *prev : a);
+    euler_update_fin_kernel<<<g, UB, update_smem(a), stream>>>(a, prev ? *prev : a);
     return hipGetLastError();
 }
 
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream) {
+    hipError_t e0 = launch_coef(a, stream);
+    if (e0 != hipSuccess) return e0;
     symplectic_momentum_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
     return hipGetLastError();
 }

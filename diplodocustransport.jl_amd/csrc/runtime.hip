@@ -78,7 +78,9 @@ struct dtx_handle {
     bool correct_active = false;
     std::vector<PackedChannel> mchans;
     std::vector<double> eweight_host;
-    // moment-rate correction: gain and loss sums per state, moment partials, coefficients
+    // moment-rate correction: gain and loss sums per state, moment partials, and the
+    // coefficients when a kernel of their own computes them (update grids of more than one
+    // round of workgroups; otherwise every update workgroup derives them)
     double *d_gsum = nullptr, *d_lsum = nullptr, *d_cpart = nullptr, *d_coef = nullptr;
     int32_t zb = 0;
     int32_t *d_ones = nullptr, *d_gslots = nullptr, *d_nslots_s = nullptr, *d_mom_off = nullptr,
@@ -710,7 +712,10 @@ static int build_work(dtx_handle* h) {
     rc = dalloc(h, &h->d_gsum, h->n);
     if (!rc) rc = dalloc(h, &h->d_lsum, h->n);
     if (!rc) rc = dalloc(h, &h->d_cpart, (int64_t)h->C * h->S * h->zb * 3);
-    if (!rc) rc = dalloc(h, &h->d_coef, (int64_t)h->C * h->S * 2);
+    // one round of update workgroups: about four per CU
+    const int64_t update_wgs = (int64_t)h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
+    if (!rc && update_wgs > 4 * (int64_t)h->num_cu)
+        rc = dalloc(h, &h->d_coef, (int64_t)h->C * h->S * 2);
     if (!rc) rc = dalloc(h, &h->d_ones, h->N);
     if (!rc) rc = dalloc(h, &h->d_gslots, h->S);
     if (!rc) rc = dalloc(h, &h->d_nslots_s, h->S);
@@ -884,6 +889,11 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
         a.nslot = h->d_ones;
         a.nslot32 = nullptr;
         a.gsum = h->d_gsum;
+        a.cpart = h->d_cpart;
+        a.mpart = h->d_mpart;
+        a.mom_off = h->d_mom_off;
+        a.mom_list = h->d_mom_list;
+        a.zb = h->zb;
         a.coef = h->d_coef;
     } else {
         a.part = h->d_part;
@@ -961,20 +971,15 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             ca.gsum = h->d_gsum;
             ca.lsum = h->d_lsum;
             ca.cpart = h->d_cpart;
-            ca.coef = h->d_coef;
             ca.zb = h->zb;
             ca.species_off = h->d_species_off;
             ca.gslots = h->d_gslots;
             ca.nslots = h->d_nslots_s;
             ca.gain_f32 = h->wide && h->precision == DTX_PREC_BF16X3;
             ca.eweight = h->d_eweight;
-            ca.mpart = h->d_mpart;
-            ca.mom_off = h->d_mom_off;
-            ca.mom_list = h->d_mom_list;
             ca.N = h->N;
             ca.C = h->C;
             ca.S = h->S;
-            ca.ldc = h->ldc;
             HIP_CHECK(h, launch_gain_correct(ca, h->stream));
         }
         rc = timed_end(h, &ev);
