@@ -103,7 +103,7 @@ struct UpdateArgs {
     const double* mpart;
     const int32_t* mom_off;
     const int32_t* mom_list;
-    int32_t zb;
+    int32_t zb, zspan;      // sums workgroups per species and the states each covers
     double* coef;           // (alpha, beta) [cell][S][2] from correct_coef_kernel, or null: each
                             // update workgroup derives its own (grids of at most one round)
     const int32_t* nslot;   // slots per state (N)
@@ -160,7 +160,8 @@ struct CorrectArgs {
     double* gsum;              // gain sums [cell][N]
     double* lsum;              // loss sums [cell][N]
     double* cpart;             // moment partials of the gain sums [cell][S][zb][3]
-    int32_t zb;                // workgroups per species in the sums launch: ceil(max ns / 64)
+    int32_t zb;                // workgroups per species in the sums launch: ceil(max ns / zspan)
+    int32_t tps;               // threads per state in the sums launch (4 or 16); zspan = 256 / tps
     const int32_t* species_off;  // (S+1)
     const int32_t* gslots;     // gain slots per species (S)
     const int32_t* nslots;     // gain + loss slots per species (S)

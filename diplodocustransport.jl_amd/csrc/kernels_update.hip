@@ -210,7 +210,7 @@ __device__ inline void correction_coef(const UpdateArgs& a, int c, int s, double
     const int lane = threadIdx.x & 63;
     const int64_t p = (int64_t)c * a.S + s;
     const int ns = a.species_off[s + 1] - a.species_off[s];
-    const int nz = (ns + UPDATE_STATES - 1) / UPDATE_STATES;
+    const int nz = (ns + a.zspan - 1) / a.zspan;
     double v[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int z = lane; z < nz; z += 64)
 #pragma unroll
@@ -804,59 +804,62 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
 // each update workgroup then derives the coefficients of its species (correction_coef) and
 // applies them (binary_term).
 
-// Sum of slots b + q, b + q + 4, ... below e of element idx (in order; FP32 slots if f32).
+// Sum of slots b + q, b + q + tps, ... below e of element idx (in order; FP32 slots if f32).
 // FP32 slots: past the last slot a load repeats slot s0 (valid, unused), so the loads of a
 // batch carry no branch and stay in flight together (a conditional load followed by its
 // conversion waits for that load alone).
 __device__ inline double slot_sum(const CorrectArgs& a, int64_t idx, int q, int b, int e,
                                   bool f32) {
     constexpr int LB = 8;
+    const int t = a.tps;
     double sum = 0.0;
-    for (int s0 = b + q; s0 < e; s0 += 4 * LB) {
+    for (int s0 = b + q; s0 < e; s0 += t * LB) {
         if (f32) {
             float v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
-                const int sl = s0 + 4 * u < e ? s0 + 4 * u : s0;
+                const int sl = s0 + t * u < e ? s0 + t * u : s0;
                 v[u] = ((const float*)(a.part + (int64_t)sl * a.part_stride))[idx];
             }
 #pragma unroll
             for (int u = 0; u < LB; ++u)
-                if (s0 + 4 * u < e) sum = sum + (double)v[u];
+                if (s0 + t * u < e) sum = sum + (double)v[u];
         } else {
             double v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u)
-                v[u] = s0 + 4 * u < e ? a.part[(int64_t)(s0 + 4 * u) * a.part_stride + idx] : 0.0;
+                v[u] = s0 + t * u < e ? a.part[(int64_t)(s0 + t * u) * a.part_stride + idx] : 0.0;
 #pragma unroll
             for (int u = 0; u < LB; ++u)
-                if (s0 + 4 * u < e) sum = sum + v[u];
+                if (s0 + t * u < e) sum = sum + v[u];
         }
     }
     return sum;
 }
 
-// Grid (C, S, ceil(max species states / 64)): four threads per state q = 0..3, each summing
-// slots q, q + 4, ... of the gain slots (FP32 sums for wide split-BF16 items) and of the loss
-// slots; the four sums are combined as (s0 + s1) + (s2 + s3).  Gain sums go to gsum, loss sums
+// Grid (C, S, ceil(max species states / (256 / tps))): tps threads per state q = 0 .. tps - 1
+// (4, or 16 for small launches), each summing slots q, q + tps, ... of the gain slots (FP32 sums
+// for wide split-BF16 items) and of the loss slots; the tps sums are combined by a butterfly in
+// a fixed order ((s0 + s1) + (s2 + s3) for four).  Gain sums go to gsum, loss sums
 // to lsum, and the workgroup's number and energy moments of its gain sums (sum g, sum E g,
 // sum E^2 g) to cpart[c][s][z].
 __global__ __launch_bounds__(UB) void correct_sums_kernel(CorrectArgs a) {
     __shared__ double red[(UB / 64) * RED_VALUES];
-    const int c = blockIdx.x, s = blockIdx.y, z = blockIdx.z, tid = threadIdx.x, q = tid & 3;
+    const int c = blockIdx.x, s = blockIdx.y, z = blockIdx.z, tid = threadIdx.x;
+    const int q = tid & (a.tps - 1);
     const int k0 = a.species_off[s], ns = a.species_off[s + 1] - k0;
-    const int kb = z * (UB / 4);
+    const int kb = z * (UB / a.tps);
     if (kb >= ns) return;  // the whole workgroup
-    const int kk = kb + (tid >> 2);
+    const int kk = kb + tid / a.tps;
     const bool valid = kk < ns;
     const int64_t idx = (int64_t)c * a.N + k0 + (valid ? kk : 0);
     const int gs = a.gslots[s], nsl = a.nslots[s];
     double g = slot_sum(a, idx, q, 0, gs, a.gain_f32 != 0);
     double l = slot_sum(a, idx, q, gs, nsl, false);
-    g = g + __shfl_xor(g, 1, 64);
-    g = g + __shfl_xor(g, 2, 64);
-    l = l + __shfl_xor(l, 1, 64);
-    l = l + __shfl_xor(l, 2, 64);
+    for (int m = 1; m < a.tps; m <<= 1) {
+        g = g + __shfl_xor(g, m, 64);
+        l = l + __shfl_xor(l, m, 64);
+    }
     const bool mine = valid && q == 0;
     if (mine) {
         a.gsum[idx] = g;

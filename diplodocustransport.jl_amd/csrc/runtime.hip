@@ -82,7 +82,7 @@ struct dtx_handle {
     // coefficients when a kernel of their own computes them (update grids of more than one
     // round of workgroups; otherwise every update workgroup derives them)
     double *d_gsum = nullptr, *d_lsum = nullptr, *d_cpart = nullptr, *d_coef = nullptr;
-    int32_t zb = 0;
+    int32_t zb = 0, zspan = UPDATE_STATES, tps = 4;
     int32_t *d_ones = nullptr, *d_gslots = nullptr, *d_nslots_s = nullptr, *d_mom_off = nullptr,
             *d_mom_list = nullptr;
     int32_t n_mom_list = 0, max_ns = 0;
@@ -708,7 +708,11 @@ static int build_work(dtx_handle* h) {
     h->max_ns = 0;
     for (int s = 0; s < h->S; ++s) h->max_ns = std::max(h->max_ns, h->sp_size[s]);
     std::vector<int32_t> nsl(next_slot.begin(), next_slot.end());
-    h->zb = (h->max_ns + UPDATE_STATES - 1) / UPDATE_STATES;
+    // threads per state in the slot sums: 16 when four would leave the launch with fewer than
+    // 65536 threads (single-cell boxes), so each thread has fewer slots to walk
+    h->tps = (int64_t)h->C * h->max_ns * 4 < 65536 ? 16 : 4;
+    h->zspan = UPDATE_THREADS / h->tps;
+    h->zb = (h->max_ns + h->zspan - 1) / h->zspan;
     rc = dalloc(h, &h->d_gsum, h->n);
     if (!rc) rc = dalloc(h, &h->d_lsum, h->n);
     if (!rc) rc = dalloc(h, &h->d_cpart, (int64_t)h->C * h->S * h->zb * 3);
@@ -894,6 +898,7 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
         a.mom_off = h->d_mom_off;
         a.mom_list = h->d_mom_list;
         a.zb = h->zb;
+        a.zspan = h->zspan;
         a.coef = h->d_coef;
     } else {
         a.part = h->d_part;
@@ -972,6 +977,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             ca.lsum = h->d_lsum;
             ca.cpart = h->d_cpart;
             ca.zb = h->zb;
+            ca.tps = h->tps;
             ca.species_off = h->d_species_off;
             ca.gslots = h->d_gslots;
             ca.nslots = h->d_nslots_s;
