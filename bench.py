@@ -216,6 +216,40 @@ def run_serving(count, serial_sample):
                 serial_fp64_sample=serial_sample)
 
 
+def run_flux_setup(name):
+    """Set-up time of the flux operators, Ap and Vol: the host path (flux_matrices + X + P as
+    CSC, ap_vol, then dtx_set_flux_csc's CSC -> ELL conversion and upload) against
+    dtx_synth_flux, which builds the same ELL rows on the device.  Wall-clock, one call each
+    after one untimed call of both (first-touch allocations)."""
+    import ctypes as C
+    import numpy as np
+    from dtx import _lib, synth, workloads
+    w = getattr(workloads, name)()
+    ps, spec, n = w.ps, w.flux_spec, w.ps.n
+    m = workloads.make_struct(w, device_flux=True)
+    lib, h = m.lib, m._h
+    fs, keep = _lib.make_flux_synth(ps, **spec)
+    times = {}
+    for rep in range(2):
+        t0 = time.perf_counter()
+        X, P = synth.flux_matrices(ps, **spec)
+        F = synth.csc_add(n, X, P)
+        Ap, Vol = synth.ap_vol(ps)
+        t1 = time.perf_counter()
+        cp, rv, nz = (np.ascontiguousarray(a) for a in F)
+        _lib.check(lib.dtx_set_flux_csc(h, _lib.FLUX_F, n, _lib.i64ptr(cp), _lib.i64ptr(rv),
+                                        _lib.dptr(nz)), h, "dtx_set_flux_csc")
+        t2 = time.perf_counter()
+        _lib.check(lib.dtx_synth_flux(h, C.byref(fs)), h, "dtx_synth_flux")
+        t3 = time.perf_counter()
+        times = {"host_assembly_s": round(t1 - t0, 4), "host_csc_to_ell_upload_s":
+                 round(t2 - t1, 4), "device_synth_s": round(t3 - t2, 5)}
+    host = times["host_assembly_s"] + times["host_csc_to_ell_upload_s"]
+    print(json.dumps({"flux_setup": dict(workload=name, n=n, nnz=int(F[0][-1] - 1), **times,
+                                         speedup=round(host / times["device_synth_s"], 1))}))
+    del keep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,7 +273,13 @@ def main():
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (5 times), the "
                          "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--flux-setup", action="store_true",
+                    help="only compare the flux set-up of the workload: host assembly + CSC "
+                         "upload against the stencils built on the device (dtx_synth_flux)")
     args = ap.parse_args()
+    if args.flux_setup:
+        run_flux_setup(args.workload)
+        return
     if args.copy_kernel_mib:
         import torch
         n = args.copy_kernel_mib * (1 << 20) // 8

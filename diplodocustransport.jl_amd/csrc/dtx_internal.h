@@ -217,6 +217,29 @@ hipError_t launch_synth_moment_packed(double* AN, double* AE, int64_t nchunks, i
                                       int kind, int No, int N1, int N2, const double* ew,
                                       uint64_t key, double scale, hipStream_t stream);
 
+// Synthetic Cartesian flux stencils built on the device (synth_flux_kernel): upwind advection
+// along x and / or y, synchrotron radiation reaction along p.
+constexpr int FLUX_MAX_ROW = 8;  // entries per row: diagonal, 2 per space axis, p +- 1
+struct FluxSynthArgs {
+    const double *xr, *yr, *zr;        // cell bounds (x_num + 1, ...)
+    const double *pr, *ur, *hr;        // momentum bounds of every species, concatenated
+    int32_t pr_off[DTX_MAX_SPECIES], ur_off[DTX_MAX_SPECIES], hr_off[DTX_MAX_SPECIES];
+    int32_t px[DTX_MAX_SPECIES], py[DTX_MAX_SPECIES];
+    int32_t sp_off[DTX_MAX_SPECIES + 1];
+    double mass[DTX_MAX_SPECIES], charge[DTX_MAX_SPECIES];
+    double dt0, sync_K;
+    int64_t n;
+    int32_t N, S, x_num, y_num, z_num, axes, periodic;
+    int32_t which;                     // DTX_FLUX_F (X + P), DTX_FLUX_X or DTX_FLUX_P
+    int32_t width;                     // 0: count pass, else the ELL width of the fill pass
+    int32_t* col;                      // fill pass: [width][n]
+    double* val;
+    unsigned long long* stats;         // count pass: longest row, entries
+    double* inv_ap;                    // count pass, when set: 1/Ap [n] and Vol [C]
+    double* vol;
+};
+hipError_t launch_synth_flux(const FluxSynthArgs& a, hipStream_t stream);
+
 uint64_t splitmix_key(uint64_t seed, uint64_t stream);
 
 // grouped stepping of several handles (runtime.hip)

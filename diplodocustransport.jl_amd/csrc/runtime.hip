@@ -128,6 +128,7 @@ struct dtx_handle {
     double *d_fmom = nullptr, *d_dfmom = nullptr, *d_dpflux = nullptr, *d_dbin = nullptr,
            *d_demi = nullptr;
     bool vectors_set = false;
+    bool geometry_set = false;      // 1/Ap and Vol written by dtx_synth_flux
     bool emission = false;
 
     int64_t step_counter = 0;
@@ -1472,6 +1473,103 @@ int dtx_set_flux_csc(dtx_handle* h, int32_t which, int64_t n, const int64_t* col
     return set_ell(h, h->ell[which], n, colptr, rowval, nzval);
 }
 
+int dtx_synth_flux(dtx_handle* h, const dtx_flux_synth* sp) {
+    if (!h) return DTX_ERR_INVALID;
+    if (!sp || !sp->xr || !sp->yr || !sp->zr || !sp->pr || !sp->ur || !sp->hr)
+        return fail(h, DTX_ERR_INVALID, "null bounds array");
+    if ((sp->axes & ~3) || !(sp->sync_K >= 0.0) || !std::isfinite(sp->sync_K))
+        return fail(h, DTX_ERR_INVALID, "invalid flux spec (axes %d, sync_K %g)", sp->axes,
+                    sp->sync_K);
+    const dtx_config& cfg = h->cfg;
+    FluxSynthArgs a{};
+    // one buffer of bounds: x, y, z, then p, u, phi of every species
+    std::vector<double> host;
+    host.insert(host.end(), sp->xr, sp->xr + cfg.x_num + 1);
+    host.insert(host.end(), sp->yr, sp->yr + cfg.y_num + 1);
+    host.insert(host.end(), sp->zr, sp->zr + cfg.z_num + 1);
+    int64_t np = 0, nu = 0, nh = 0;
+    for (int s = 0; s < h->S; ++s) {
+        a.pr_off[s] = (int32_t)np;
+        a.ur_off[s] = (int32_t)nu;
+        a.hr_off[s] = (int32_t)nh;
+        np += cfg.px_num[s] + 1;
+        nu += cfg.py_num[s] + 1;
+        nh += cfg.pz_num[s] + 1;
+        a.px[s] = cfg.px_num[s];
+        a.py[s] = cfg.py_num[s];
+        a.sp_off[s] = h->sp_off[s];
+        a.mass[s] = sp->mass[s];
+        a.charge[s] = sp->charge[s];
+    }
+    a.sp_off[h->S] = h->sp_off[h->S];
+    const int64_t o_p = (int64_t)host.size();
+    host.insert(host.end(), sp->pr, sp->pr + np);
+    const int64_t o_u = (int64_t)host.size();
+    host.insert(host.end(), sp->ur, sp->ur + nu);
+    const int64_t o_h = (int64_t)host.size();
+    host.insert(host.end(), sp->hr, sp->hr + nh);
+    double* d_bounds = nullptr;
+    unsigned long long* d_stats = nullptr;
+    const int64_t nb = (int64_t)host.size();
+    int rc = dalloc(h, &d_bounds, nb);
+    if (!rc) rc = dalloc(h, &d_stats, 2);
+    if (!rc) rc = upload(h, d_bounds, host.data(), nb);
+    a.xr = d_bounds;
+    a.yr = a.xr + cfg.x_num + 1;
+    a.zr = a.yr + cfg.y_num + 1;
+    a.pr = d_bounds + o_p;
+    a.ur = d_bounds + o_u;
+    a.hr = d_bounds + o_h;
+    a.dt0 = cfg.dt0;
+    a.sync_K = sp->sync_K;
+    a.n = h->n;
+    a.N = h->N;
+    a.S = h->S;
+    a.x_num = cfg.x_num;
+    a.y_num = cfg.y_num;
+    a.z_num = cfg.z_num;
+    a.axes = sp->axes;
+    a.periodic = sp->periodic ? 1 : 0;
+    a.stats = d_stats;
+    // the matrices the method reads; the first count pass also writes 1/Ap and Vol
+    const int which_list[2] = {DTX_FLUX_X, DTX_FLUX_P};
+    const bool euler = cfg.method == DTX_METHOD_FORWARD_EULER;
+    const int nwhich = euler ? 1 : 2;
+    for (int q = 0; q < nwhich && !rc; ++q) {
+        a.which = euler ? DTX_FLUX_F : which_list[q];
+        a.width = 0;
+        a.inv_ap = q == 0 ? h->d_invap : nullptr;
+        a.vol = q == 0 ? h->d_vol : nullptr;
+        unsigned long long st[2] = {0, 0};
+        rc = upload(h, d_stats, st, 2);
+        if (rc) break;
+        HIP_CHECK(h, launch_synth_flux(a, h->stream));
+        HIP_CHECK(h, hipMemcpyAsync(st, d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+        HIP_CHECK(h, stream_sync(h));
+        EllMatrix& E = h->ell[a.which];
+        dfree(h, &E.col, E.n * E.width);
+        dfree(h, &E.val, E.n * E.width);
+        E.n = h->n;
+        E.width = (int32_t)st[0];
+        E.nnz = (int64_t)st[1];
+        if (E.width == 0) continue;
+        rc = dalloc(h, &E.col, E.width * E.n);
+        if (!rc) rc = dalloc(h, &E.val, E.width * E.n);
+        if (rc) break;
+        a.width = E.width;
+        a.col = E.col;
+        a.val = E.val;
+        a.inv_ap = nullptr;
+        a.vol = nullptr;
+        HIP_CHECK(h, launch_synth_flux(a, h->stream));
+        HIP_CHECK(h, stream_sync(h));
+    }
+    dfree(h, &d_bounds, nb);
+    dfree(h, &d_stats, 2);
+    if (!rc) h->geometry_set = true;
+    return rc;
+}
+
 int dtx_set_binary_dense(dtx_handle* h, int64_t N, const double* M) {
     if (!h) return DTX_ERR_INVALID;
     if (N != h->N)
@@ -1632,12 +1730,17 @@ int dtx_set_emission_dense(dtx_handle* h, int64_t n, const double* M) {
 int dtx_set_vectors(dtx_handle* h, const double* Ap, const double* Vol, const double* f_init,
                     const double* df_inj) {
     if (!h) return DTX_ERR_INVALID;
-    if (!Ap || !Vol || !f_init || !df_inj) return fail(h, DTX_ERR_INVALID, "null vector");
+    const bool keep_geometry = !Ap && !Vol && h->geometry_set;
+    if ((!keep_geometry && (!Ap || !Vol)) || !f_init || !df_inj)
+        return fail(h, DTX_ERR_INVALID, "null vector");
     const int64_t n = h->n;
-    std::vector<double> inv((size_t)n);
-    for (int64_t i = 0; i < n; ++i) inv[(size_t)i] = 1.0 / Ap[i];  // 1 ./ Ap_Flux
-    int rc = upload(h, h->d_invap, inv.data(), n);
-    if (!rc) rc = upload(h, h->d_vol, Vol, h->C);
+    int rc = DTX_OK;
+    if (!keep_geometry) {
+        std::vector<double> inv((size_t)n);
+        for (int64_t i = 0; i < n; ++i) inv[(size_t)i] = 1.0 / Ap[i];  // 1 ./ Ap_Flux
+        rc = upload(h, h->d_invap, inv.data(), n);
+        if (!rc) rc = upload(h, h->d_vol, Vol, h->C);
+    }
     if (!rc) rc = upload(h, h->d_finit, f_init, n);
     if (!rc) rc = upload(h, h->d_dfinj, df_inj, n);
     if (rc) return rc;

@@ -163,4 +163,189 @@ hipError_t launch_synth_loss(double* loss, int N, int off1, int N1, int off2, in
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------- synthetic flux stencils -----
+// The rows of the space and momentum flux matrices of the synthetic Cartesian problems, one
+// thread per state row r = c N + k.  Each row's entries are formed in the order
+// BuildFluxMatrices pushes them (per advection axis: (r, r+), (r, r), (r, r), (r, r-); then the
+// synchrotron (r, r+1), (r, r), (r, r), (r, r-1)), duplicates summed from 0.0 in that order as
+// sparse() combines them (src/Fluxes/BuildFluxMatrices.jl:119-129), F = X + P summed X first
+// (src/SteppingStructs.jl:60), explicit zeros kept, and the row's columns sorted ascending as
+// a CSC -> ELL conversion leaves them.  Expressions keep the operation order of the host
+// generator (no contraction).
+
+struct FluxRow {
+    int64_t col[FLUX_MAX_ROW];
+    double val[FLUX_MAX_ROW];
+    int cnt;
+};
+
+__device__ inline void flux_push(FluxRow& R, int64_t col, double v) {
+    for (int e = 0; e < R.cnt; ++e)
+        if (R.col[e] == col) {
+            R.val[e] = R.val[e] + v;
+            return;
+        }
+    R.col[R.cnt] = col;
+    R.val[R.cnt] = 0.0 + v;
+    ++R.cnt;
+}
+
+// x sqrt(1 - x^2) + asin(x): twice the integral of sqrt(1 - u^2) (the perpendicular velocity)
+__device__ inline double perp_primitive(double u) { return u * sqrt(1.0 - u * u) + asin(u); }
+
+// g(u) = 3u - u^3 of the synchrotron force's angular integral
+// (src/Fluxes/FluxFunctionsEmissionForces.jl:46-70)
+__device__ inline double sync_angular(double u) { return 3.0 * u - pow(u, 3.0); }
+
+// The row r of the matrix a.which into R (and of X into X, P into P for the F merge).
+__device__ void flux_row(const FluxSynthArgs& a, int64_t r, FluxRow& R) {
+    const int c = (int)(r / a.N), k = (int)(r % a.N);
+    int s = 0;
+    while (s + 1 < a.S && k >= a.sp_off[s + 1]) ++s;
+    const int t = k - a.sp_off[s];
+    const int ip = t % a.px[s], iu = (t / a.px[s]) % a.py[s], ih = t / (a.px[s] * a.py[s]);
+    const double* pr = a.pr + a.pr_off[s];
+    const double* ur = a.ur + a.ur_off[s];
+    const double* hr = a.hr + a.hr_off[s];
+    const int yz = a.y_num * a.z_num;
+    const int cx = c / yz, cy = (c / a.z_num) % a.y_num, cz = c % a.z_num;
+    const double dx = fabs(a.xr[cx + 1] - a.xr[cx]), dy = fabs(a.yr[cy + 1] - a.yr[cy]),
+                 dz = fabs(a.zr[cz + 1] - a.zr[cz]);
+    const double m = a.mass[s];
+    const double p0 = pr[ip], p1 = pr[ip + 1], u0 = ur[iu], u1 = ur[iu + 1], h0 = hr[ih],
+                 h1 = hr[ih + 1];
+    const double du = u1 - u0, dh = h1 - h0;
+    FluxRow X, P;
+    X.cnt = 0;
+    P.cnt = 0;
+    if (a.which != DTX_FLUX_P) {
+        // upwind advection (the B / C stencils, src/Fluxes/BuildFluxMatrices.jl:708-974) with
+        // the bin-averaged velocity component (E1 - E0)/dp times the angular average
+        const double E0 = sqrt(m * m + p0 * p0), E1 = sqrt(m * m + p1 * p1);
+        const double vbar = (E1 - E0) / (p1 - p0);
+        for (int ax = 0; ax < 2; ++ax) {
+            const int num = ax == 0 ? a.x_num : a.y_num;
+            if (!((a.axes >> ax) & 1) || num == 1) continue;
+            double vel, trans;
+            int ci;
+            if (ax == 0) {
+                vel = vbar * (u1 * u1 - u0 * u0) / (2.0 * du);
+                trans = dy * dz;
+                ci = cx;
+            } else {
+                vel = vbar * (perp_primitive(u1) - perp_primitive(u0)) / (2.0 * du) *
+                      (cos(h0) - cos(h1)) / dh;
+                trans = dx * dz;
+                ci = cy;
+            }
+            int cp = ci + 1, cm = ci - 1;
+            if (a.periodic) {
+                if (cp >= num) cp = 0;
+                if (cm < 0) cm = num - 1;
+            } else {
+                cp = cp < num - 1 ? cp : num - 1;
+                cm = cm > 0 ? cm : 0;
+            }
+            const int cellp = ax == 0 ? cp * yz + cy * a.z_num + cz : cx * yz + cp * a.z_num + cz;
+            const int cellm = ax == 0 ? cm * yz + cy * a.z_num + cz : cx * yz + cm * a.z_num + cz;
+            const int64_t bp = (int64_t)cellp * a.N + k, bm = (int64_t)cellm * a.N + k;
+            const double Bp = a.dt0 * trans * vel, Bm = -Bp;
+            const double hpr = Bp > 0.0 ? 0.0 : 1.0, hpl = 1.0 - hpr;
+            const double hmr = Bm > 0.0 ? 1.0 : 0.0, hml = 1.0 - hmr;
+            if (bp != r) {
+                flux_push(X, bp, Bp * hpr);
+                flux_push(X, r, Bp * hpl);
+            }
+            if (bm != r) {
+                flux_push(X, r, Bm * hmr);
+                flux_push(X, bm, Bm * hml);
+            }
+        }
+    }
+    if (a.which != DTX_FLUX_X && a.sync_K > 0.0) {
+        // synchrotron radiation reaction along p (the I stencil, :275-413), closed p boundaries
+        const double ang = sync_angular(u1) - sync_angular(u0);
+        const double vol = a.dt0 * dx * dy * dz;
+        auto norm_of = [&](int q) { return (pr[q + 1] - pr[q]) * du * dh; };
+        double Ip = 0.0, Im = -0.0;
+        if (m != 0.0 && a.charge[s] != 0.0) {
+            auto force = [&](double pp) {
+                return -a.sync_K / (3.0 * m) * pp * sqrt(m * m + pp * pp) * ang * dh;
+            };
+            Ip = vol * force(p1);
+            Im = -(vol * force(p0));
+        }
+        const double hpr = Ip > 0.0 ? 0.0 : 1.0, hpl = 1.0 - hpr;
+        const double hmr = Im > 0.0 ? 1.0 : 0.0, hml = 1.0 - hmr;
+        const double norm = norm_of(ip);
+        if (ip < a.px[s] - 1) {
+            flux_push(P, r + 1, Ip * hpr / norm_of(ip + 1));
+            flux_push(P, r, Ip * hpl / norm);
+        }
+        if (ip > 0) {
+            flux_push(P, r, Im * hmr / norm);
+            flux_push(P, r - 1, Im * hml / norm_of(ip - 1));
+        }
+    }
+    R.cnt = 0;
+    for (int e = 0; e < X.cnt; ++e) flux_push(R, X.col[e], X.val[e]);
+    for (int e = 0; e < P.cnt; ++e) flux_push(R, P.col[e], P.val[e]);
+    for (int e = 1; e < R.cnt; ++e)  // insertion sort by column
+        for (int q = e; q > 0 && R.col[q - 1] > R.col[q]; --q) {
+            const int64_t tc = R.col[q];
+            R.col[q] = R.col[q - 1];
+            R.col[q - 1] = tc;
+            const double tv = R.val[q];
+            R.val[q] = R.val[q - 1];
+            R.val[q - 1] = tv;
+        }
+}
+
+// Count pass (a.width == 0): the longest row into stats[0] and the entry count into stats[1];
+// with a.inv_ap set, also 1/Ap = 1/(dx dy dz) of every row and Vol = dt0 dx dy dz of every
+// cell.  Fill pass: the rows into the [width][n] ELL arrays, padded with column -1 and 0.0.
+__global__ __launch_bounds__(256) void synth_flux_kernel(FluxSynthArgs a) {
+    unsigned long long longest = 0, entries = 0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        FluxRow R;
+        flux_row(a, r, R);
+        if (a.width == 0) {
+            longest = longest > (unsigned long long)R.cnt ? longest : (unsigned long long)R.cnt;
+            entries += (unsigned long long)R.cnt;
+        } else {
+            for (int e = 0; e < a.width; ++e) {
+                a.col[(int64_t)e * a.n + r] = e < R.cnt ? (int32_t)R.col[e] : -1;
+                a.val[(int64_t)e * a.n + r] = e < R.cnt ? R.val[e] : 0.0;
+            }
+        }
+        if (a.inv_ap) {
+            const int c = (int)(r / a.N);
+            const int yz = a.y_num * a.z_num;
+            const int cx = c / yz, cy = (c / a.z_num) % a.y_num, cz = c % a.z_num;
+            const double v = fabs(a.xr[cx + 1] - a.xr[cx]) * fabs(a.yr[cy + 1] - a.yr[cy]) *
+                             fabs(a.zr[cz + 1] - a.zr[cz]);
+            a.inv_ap[r] = 1.0 / v;
+            if (r % a.N == 0) a.vol[c] = a.dt0 * v;
+        }
+    }
+    if (a.width == 0) {
+        for (int m = 1; m < 64; m <<= 1) {
+            const unsigned long long o = __shfl_xor(longest, m, 64);
+            longest = longest > o ? longest : o;
+            entries += __shfl_xor(entries, m, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(&a.stats[0], longest);
+            atomicAdd(&a.stats[1], entries);
+        }
+    }
+}
+
+hipError_t launch_synth_flux(const FluxSynthArgs& a, hipStream_t stream) {
+    const int blocks = (int)std::min<int64_t>((a.n + 255) / 256, 8192);
+    synth_flux_kernel<<<blocks, 256, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
 }  // namespace dtx

@@ -70,6 +70,12 @@ class RequestResult(C.Structure):
                 ("steps_done", I64)]
 
 
+class FluxSynth(C.Structure):
+    _fields_ = [("xr", PD), ("yr", PD), ("zr", PD), ("pr", PD), ("ur", PD), ("hr", PD),
+                ("mass", D * MAX_SPECIES), ("charge", D * MAX_SPECIES), ("sync_K", D),
+                ("axes", I32), ("periodic", I32)]
+
+
 # (name, restype, argtypes) for every entry point of include/dtx.h
 SIGNATURES = [
     ("dtx_create", I32, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
@@ -86,6 +92,7 @@ SIGNATURES = [
     ("dtx_set_moment_weights", I32, [C.c_void_p, PD]),
     ("dtx_set_moment_correction", I32, [C.c_void_p, I32]),
     ("dtx_synth_binary", I32, [C.c_void_p, U64, D]),
+    ("dtx_synth_flux", I32, [C.c_void_p, C.POINTER(FluxSynth)]),
     ("dtx_step", I32, [C.c_void_p, D, D, D, I32, PD, PI32, C.POINTER(StepStats)]),
     ("dtx_solve", I32, [C.c_void_p, D, I64, PD, I32, PD, PD, PI64]),
     ("dtx_reset", I32, [C.c_void_p]),
@@ -165,3 +172,22 @@ def make_config(ps, precision="fp64", method=METHOD_FORWARD_EULER, device=0) -> 
     cfg.device = device
     cfg.dt0 = ps.dt0
     return cfg
+
+
+def make_flux_synth(ps, sync_K=0.0, axes="x", periodic=True):
+    """dtx_flux_synth for a PhaseSpace; returns (struct, arrays it points into)."""
+    import numpy as np
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (
+        ps.xr, ps.yr, ps.zr,
+        np.concatenate([s.pr for s in ps.species]),
+        np.concatenate([s.ur for s in ps.species]),
+        np.concatenate([s.hr for s in ps.species]))]
+    fs = FluxSynth()
+    fs.xr, fs.yr, fs.zr, fs.pr, fs.ur, fs.hr = (dptr(a) for a in keep)
+    for i, s in enumerate(ps.species):
+        fs.mass[i] = s.mass
+        fs.charge[i] = s.charge
+    fs.sync_K = float(sync_K)
+    fs.axes = (1 if "x" in axes else 0) | (2 if "y" in axes else 0)
+    fs.periodic = 1 if periodic else 0
+    return fs, keep

@@ -66,11 +66,15 @@ class EmissionMatricesStruct:
 
 @dataclass
 class FluxMatricesStruct:
-    """Ap_Flux (n), X_Flux / P_Flux as (colptr, rowval, nzval) 1-based CSC tuples, Vol (C)."""
-    Ap_Flux: np.ndarray
+    """Ap_Flux (n), X_Flux / P_Flux as (colptr, rowval, nzval) 1-based CSC tuples, Vol (C).
+    With `synth` = dict(sync_K=..., axes="x"|"y"|"xy", periodic=...), the synthetic Cartesian
+    stencils (dtx/synth.py flux_matrices, ap_vol) are built on the device instead
+    (dtx_synth_flux); the matrices and Ap / Vol given here are then ignored and may be None."""
+    Ap_Flux: Optional[np.ndarray]
     X_Flux: Optional[tuple]
     P_Flux: Optional[tuple]
-    Vol: np.ndarray
+    Vol: Optional[np.ndarray]
+    synth: Optional[dict] = None
 
 
 @dataclass
@@ -120,7 +124,11 @@ class _SteppingStruct:
         _lib.check(self.lib.dtx_create(C.byref(cfg), C.byref(self._h)), None, "dtx_create")
         h = self._h
         # fluxes
-        if self.METHOD == _lib.METHOD_FORWARD_EULER:
+        if FluxM.synth is not None:
+            fs, keep = _lib.make_flux_synth(PhaseSpace, **FluxM.synth)
+            _lib.check(self.lib.dtx_synth_flux(h, C.byref(fs)), h, "dtx_synth_flux")
+            del keep
+        elif self.METHOD == _lib.METHOD_FORWARD_EULER:
             F = self._flux_sum(n, FluxM)
             if F is not None:
                 cp, rv, nz = _csc_args(F)
@@ -173,9 +181,13 @@ class _SteppingStruct:
                 M = np.asfortranarray(EmiM.M_Emi, dtype=np.float64).reshape(-1, order="F")
                 _lib.check(self.lib.dtx_set_emission_dense(h, n, _lib.dptr(M)), h,
                            "dtx_set_emission_dense")
-        ap, vol = _f64(FluxM.Ap_Flux), _f64(FluxM.Vol)
+        if FluxM.synth is not None:  # 1/Ap and Vol are already on the device
+            ap = vol = None
+        else:
+            ap, vol = _f64(FluxM.Ap_Flux), _f64(FluxM.Vol)
         inj = _f64(Injection)
-        _lib.check(self.lib.dtx_set_vectors(h, _lib.dptr(ap), _lib.dptr(vol),
+        _lib.check(self.lib.dtx_set_vectors(h, None if ap is None else _lib.dptr(ap),
+                                            None if vol is None else _lib.dptr(vol),
                                             _lib.dptr(self.f_init), _lib.dptr(inj)), h,
                    "dtx_set_vectors")
         w = _f64(PhaseSpace.energy_weights())

@@ -49,6 +49,7 @@ class Workload:
     synth_seed: Optional[int] = None
     emission_blocks: Optional[np.ndarray] = None
     domain: Optional[list] = None
+    flux_spec: Optional[dict] = None  # synth.flux_matrices arguments of X and P, for the device
 
     @property
     def F(self):
@@ -138,7 +139,7 @@ def transport_1d(p_num=32, u_num=8, h_num=1, x_num=128, steps=2000, seed=None,
     X, P = synth.flux_matrices(ps, sync_K=K, axes="x")
     M = synth.assemble_M_Bin(ps, gains, losses) if monolithic else None
     return Workload("transport_1d", ps, f0, np.zeros(ps.n), Ap, Vol, X, P, dt, steps, seed,
-                    gains, losses, M)
+                    gains, losses, M, flux_spec=dict(sync_K=K, axes="x", periodic=True))
 
 
 def precision_box(p_num=64, u_num=8, h_num=2, num_species=3, steps=1000, seed=None):
@@ -163,7 +164,7 @@ def transport_2d(p_num=32, u_num=8, h_num=2, x_num=32, y_num=32, steps=500, seed
     K = _sync_K(ps, dt)
     X, P = synth.flux_matrices(ps, sync_K=K, axes="xy")
     return Workload("transport_2d", ps, f0, np.zeros(ps.n), Ap, Vol, X, P, dt, steps, seed,
-                    synth_seed=seed)
+                    synth_seed=seed, flux_spec=dict(sync_K=K, axes="xy", periodic=True))
 
 
 def tiny(p_num=4, u_num=2, h_num=1, x_num=3, seed=7, steps=20, monolithic=True,
@@ -180,10 +181,11 @@ def tiny(p_num=4, u_num=2, h_num=1, x_num=3, seed=7, steps=20, monolithic=True,
     Ap, Vol = synth.ap_vol(ps)
     gains, losses = synth.binary_channels(ps, seed)
     dt = _dt_from_losses(ps, losses, f0, Ap, Vol)
-    X = P = None
+    X = P = spec = None
     if with_flux and x_num > 1:
         K = _sync_K(ps, dt, cooling_steps=20.0)
-        X, P = synth.flux_matrices(ps, sync_K=K, axes="x")
+        spec = dict(sync_K=K, axes="x", periodic=True)
+        X, P = synth.flux_matrices(ps, **spec)
     M = synth.assemble_M_Bin(ps, gains, losses) if monolithic else None
     inj = np.zeros(ps.n)
     if injection:
@@ -196,7 +198,7 @@ def tiny(p_num=4, u_num=2, h_num=1, x_num=3, seed=7, steps=20, monolithic=True,
         blocks = (raw.reshape(ps.n_space, N, N) - 0.5) * (0.2 / dt / N)
         blocks = np.ascontiguousarray(blocks.transpose(0, 2, 1))  # column-major per cell
     return Workload("tiny", ps, f0, inj, Ap, Vol, X, P, dt, steps, seed, gains, losses, M,
-                    emission_blocks=blocks, domain=domain)
+                    emission_blocks=blocks, domain=domain, flux_spec=spec)
 
 
 def emission_dense(w: Workload) -> Optional[np.ndarray]:
@@ -212,8 +214,9 @@ def emission_dense(w: Workload) -> Optional[np.ndarray]:
 
 
 def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=None,
-                moment_correction=True):
-    """ForwardEulerStruct / ForwardSymplecticEulerStruct on the HIP backend for a workload."""
+                moment_correction=True, device_flux=False):
+    """ForwardEulerStruct / ForwardSymplecticEulerStruct on the HIP backend for a workload;
+    device_flux: the flux stencils, Ap and Vol built on the device from w.flux_spec."""
     from .api import (BinaryMatricesStruct, EmissionMatricesStruct, FluxMatricesStruct,
                       ForwardEulerStruct, ForwardSymplecticEulerStruct)
     mono = (w.M_Bin is not None) if use_monolithic is None else use_monolithic
@@ -227,7 +230,12 @@ def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=No
                                     losses=w.losses)
     EmiM = EmissionMatricesStruct(Emission_list=["synthetic"] if w.emission_blocks is not None
                                   else [], blocks=w.emission_blocks)
-    FluxM = FluxMatricesStruct(w.Ap, w.X, w.P, w.Vol)
+    if device_flux:
+        if w.flux_spec is None:
+            raise ValueError(f"workload {w.name} has no flux stencil to build on the device")
+        FluxM = FluxMatricesStruct(None, None, None, None, synth=w.flux_spec)
+    else:
+        FluxM = FluxMatricesStruct(w.Ap, w.X, w.P, w.Vol)
     cls = ForwardEulerStruct if method == "euler" else ForwardSymplecticEulerStruct
     return cls(w.ps, w.f0, w.inj, BinM, EmiM, FluxM, Precision=precision,
                MomentCorrection=moment_correction)

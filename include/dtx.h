@@ -141,7 +141,8 @@ int dtx_set_emission_dense(dtx_handle* h, int64_t n, const double* M_Emi);
 int dtx_set_emission_blocks(dtx_handle* h, const double* blocks);
 
 /* FluxM.Ap_Flux (n; the library forms 1 ./ Ap as src/SteppingStructs.jl:55 does), FluxM.Vol
- * (C), Initial (n, f_init) and Injection (n, df_Inj).  f is set to f_init. */
+ * (C), Initial (n, f_init) and Injection (n, df_Inj).  f is set to f_init.  Ap and Vol may both
+ * be NULL after dtx_synth_flux, which has written them on the device. */
 int dtx_set_vectors(dtx_handle* h, const double* Ap, const double* Vol, const double* f_init,
                     const double* df_inj);
 
@@ -164,6 +165,32 @@ int dtx_set_moment_correction(dtx_handle* h, int32_t enable);
  * (sum_k G[k,i,j] = L[i,j]).  No counterpart in the reference, which reads DiplodocusCollisions
  * files; used for the benchmark workloads whose tensors are too large to build on the host. */
 int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale);
+
+/* Synthetic Cartesian flux operators built on the device, in place of the CSC that
+ * BuildFluxMatrices assembles on the host (src/Fluxes/BuildFluxMatrices.jl:10-184) and
+ * dtx_set_flux_csc uploads: upwind advection along x and / or y with the bin-averaged velocity
+ * component (the B / C stencils, :708-974; Cartesian face fluxes,
+ * src/Fluxes/FluxFunctionsCoordinate.jl:264-267) and synchrotron radiation reaction along p
+ * (the I stencil, :275-413, with src/Fluxes/FluxFunctionsEmissionForces.jl:46-70), closed p
+ * boundaries.  Builds the matrices the method reads (ForwardEuler: F = X + P; symplectic: X and
+ * P) with the entries, explicit zeros, duplicate sums and column order that dtx_set_flux_csc
+ * gives for the host-assembled CSC, and writes 1 ./ Ap (Ap = dx dy dz per cell, :145-154) and
+ * Vol = dt0 dx dy dz (src/Fluxes/FluxFunctionsCoordinate.jl:313-315), so dtx_set_vectors may
+ * then take NULL Ap and Vol.  The bounds arrays are read during the call only. */
+typedef struct {
+    const double* xr;  /* x_num + 1 cell bounds; likewise yr (y_num + 1) and zr (z_num + 1)  */
+    const double* yr;
+    const double* zr;
+    const double* pr;  /* momentum bounds of every species in order: px_num[s] + 1 each      */
+    const double* ur;  /* u bounds: py_num[s] + 1 each                                       */
+    const double* hr;  /* phi bounds: pz_num[s] + 1 each                                     */
+    double mass[DTX_MAX_SPECIES];
+    double charge[DTX_MAX_SPECIES];
+    double sync_K;     /* synchrotron coefficient; 0: no momentum flux (P empty)             */
+    int32_t axes;      /* bit 0: advection along x, bit 1: along y (axes of one cell skipped) */
+    int32_t periodic;  /* 1: periodic space boundaries, 0: closed                            */
+} dtx_flux_synth;
+int dtx_synth_flux(dtx_handle* h, const dtx_flux_synth* spec);
 
 /* ---- stepping ---------------------------------------------------------------------------- */
 
