@@ -11,7 +11,8 @@
 // Requests are synthetic workloads built from (shape, seed): species 0 electron, 1 photon,
 // 2.. positron; log p grid on [1e-3, 1e3], uniform u, phi on [0, 2 pi]; Maxwell-Juttner f0 at
 // theta with 1% seeded noise; lognormal gain tensors generated on the device; x advection plus
-// synchrotron cooling when with_flux and x_num > 1; dt = 0.1 / (fastest expected loss rate).
+// synchrotron cooling when with_flux and x_num > 1, built on the device; dt = 0.1 / (fastest
+// expected loss rate).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -90,54 +91,11 @@ std::vector<double> maxwell_juttner(const SpeciesGrid& g, double theta) {
     return f;
 }
 
-struct Coo {
-    std::vector<int64_t> r, c;
-    std::vector<double> v;
-    void push(int64_t rr, int64_t cc, double vv) {
-        r.push_back(rr);
-        c.push_back(cc);
-        v.push_back(vv);
-    }
-};
-
-// 1-based CSC with duplicates summed and explicit zeros kept.
-void coo_to_csc(int64_t n, const Coo& a, std::vector<int64_t>& colptr, std::vector<int64_t>& rowval,
-                std::vector<double>& nzval) {
-    std::vector<size_t> ord(a.r.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
-    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
-        return a.c[x] != a.c[y] ? a.c[x] < a.c[y] : a.r[x] < a.r[y];
-    });
-    colptr.assign((size_t)n + 1, 0);
-    rowval.clear();
-    nzval.clear();
-    for (size_t t = 0; t < ord.size(); ++t) {
-        const size_t i = ord[t];
-        if (!rowval.empty() && t > 0 && a.c[ord[t - 1]] == a.c[i] && a.r[ord[t - 1]] == a.r[i]) {
-            nzval.back() += a.v[i];
-            continue;
-        }
-        rowval.push_back(a.r[i] + 1);
-        nzval.push_back(a.v[i]);
-        colptr[(size_t)a.c[i] + 1]++;
-    }
-    colptr[0] = 1;
-    for (int64_t j = 0; j < n; ++j) colptr[(size_t)j + 1] += colptr[(size_t)j];
-}
-
-// x advection (periodic, upwind, bin-averaged velocity along u) and synchrotron cooling in p
-// (closed boundaries, massive charged species), as one F = X + P.
-void build_flux(const dtx_config& c, const std::vector<SpeciesGrid>& sg, const std::vector<int>& off,
-                int N, double dt, std::vector<int64_t>& colptr, std::vector<int64_t>& rowval,
-                std::vector<double>& nzval) {
-    const int X = c.x_num;
-    const int64_t n = (int64_t)N * X;
-    const double dx = 1.0 / X;
-    Coo a;
-    // cooling rate prefactor: fastest outflow coefficient (K = 1) sets cooling time ~100 steps
+// Synchrotron coefficient K that gives the fastest-cooling bin a cooling time of about 100
+// steps: the largest outflow coefficient I(p0) / (dp du) of the massive charged species at K = 1.
+double sync_coefficient(const std::vector<SpeciesGrid>& sg, double dt) {
     double cmax = 0.0;
-    for (size_t s = 0; s < sg.size(); ++s) {
-        const SpeciesGrid& g = sg[s];
+    for (const SpeciesGrid& g : sg) {
         if (g.mass == 0.0 || g.charge == 0.0) continue;
         for (int u = 0; u < g.u; ++u)
             for (int p = 1; p < g.p; ++p) {
@@ -148,52 +106,34 @@ void build_flux(const dtx_config& c, const std::vector<SpeciesGrid>& sg, const s
                 cmax = std::max(cmax, I / ((g.pr[p + 1] - g.pr[p]) * (u1 - u0)));
             }
     }
-    const double K = cmax > 0.0 ? 1.0 / (100.0 * dt * cmax) : 0.0;
-    for (int x = 0; x < X; ++x)
-        for (size_t s = 0; s < sg.size(); ++s) {
-            const SpeciesGrid& g = sg[s];
-            for (int hh = 0; hh < g.h; ++hh)
-                for (int u = 0; u < g.u; ++u)
-                    for (int p = 0; p < g.p; ++p) {
-                        const int local = p + g.p * (u + g.u * hh);
-                        const int64_t aidx = (int64_t)x * N + off[s] + local;
-                        const double E0 = std::sqrt(g.mass * g.mass + g.pr[p] * g.pr[p]);
-                        const double E1 = std::sqrt(g.mass * g.mass + g.pr[p + 1] * g.pr[p + 1]);
-                        const double u0 = g.ur[u], u1 = g.ur[u + 1];
-                        const double v = (E1 - E0) / (g.pr[p + 1] - g.pr[p]) *
-                                         (u1 * u1 - u0 * u0) / (2 * (u1 - u0)) * c.dt0;
-                        const int xp = (x + 1) % X, xm = (x + X - 1) % X;
-                        const int64_t bp = (int64_t)xp * N + off[s] + local;
-                        const int64_t bm = (int64_t)xm * N + off[s] + local;
-                        // upwind: plus face v, minus face -v (src/Fluxes/BuildFluxMatrices.jl:764-833)
-                        if (v > 0) {
-                            a.push(aidx, bp, 0.0);
-                            a.push(aidx, aidx, v);
-                            a.push(aidx, aidx, 0.0);
-                            a.push(aidx, bm, -v);
-                        } else {
-                            a.push(aidx, bp, v);
-                            a.push(aidx, aidx, 0.0);
-                            a.push(aidx, aidx, -v);
-                            a.push(aidx, bm, 0.0);
-                        }
-                        if (K > 0.0) {
-                            const double ang = (3 * u1 - u1 * u1 * u1) - (3 * u0 - u0 * u0 * u0);
-                            const double norm = (g.pr[p + 1] - g.pr[p]) * (u1 - u0);
-                            auto Iface = [&](double pp) {
-                                if (g.mass == 0.0 || g.charge == 0.0) return 0.0;
-                                return -K / (3 * g.mass) * pp * std::sqrt(g.mass * g.mass + pp * pp) *
-                                       ang * c.dt0 * dx;
-                            };
-                            if (p + 1 < g.p) {
-                                const double normp = (g.pr[p + 2] - g.pr[p + 1]) * (u1 - u0);
-                                a.push(aidx, aidx + 1, Iface(g.pr[p + 1]) / normp);
-                            }
-                            if (p > 0) a.push(aidx, aidx, -Iface(g.pr[p]) / norm);
-                        }
-                    }
-        }
-    coo_to_csc(n, a, colptr, rowval, nzval);
+    return cmax > 0.0 ? 1.0 / (100.0 * dt * cmax) : 0.0;
+}
+
+// Periodic upwind x advection and synchrotron cooling in p (closed boundaries), F = X + P, with
+// 1/Ap and Vol, built on the device (dtx_synth_flux): x cells of width 1 / x_num on [0, 1].
+int set_flux(dtx_handle* h, const dtx_config& c, const std::vector<SpeciesGrid>& sg, double dt) {
+    dtx_flux_synth fs{};
+    std::vector<double> xr(c.x_num + 1), yr(c.y_num + 1), zr(c.z_num + 1), pr, ur, hr;
+    for (int i = 0; i <= c.x_num; ++i) xr[i] = (double)i / c.x_num;
+    for (int i = 0; i <= c.y_num; ++i) yr[i] = (double)i / c.y_num;
+    for (int i = 0; i <= c.z_num; ++i) zr[i] = (double)i / c.z_num;
+    for (size_t s = 0; s < sg.size(); ++s) {
+        pr.insert(pr.end(), sg[s].pr.begin(), sg[s].pr.end());
+        ur.insert(ur.end(), sg[s].ur.begin(), sg[s].ur.end());
+        hr.insert(hr.end(), sg[s].hr.begin(), sg[s].hr.end());
+        fs.mass[s] = sg[s].mass;
+        fs.charge[s] = sg[s].charge;
+    }
+    fs.xr = xr.data();
+    fs.yr = yr.data();
+    fs.zr = zr.data();
+    fs.pr = pr.data();
+    fs.ur = ur.data();
+    fs.hr = hr.data();
+    fs.sync_K = sync_coefficient(sg, dt);
+    fs.axes = 1;
+    fs.periodic = 1;
+    return dtx_synth_flux(h, &fs);
 }
 
 struct Active {
@@ -327,16 +267,18 @@ static int build_request(dtx_server* sv, const dtx_request& rq, dtx_handle** out
         }
     }
     const double dt = 0.1 / worst;
-    if (rq.with_flux && c.x_num > 1) {
-        std::vector<int64_t> colptr, rowval;
-        std::vector<double> nzval;
-        build_flux(c, sg, off, N, dt, colptr, rowval, nzval);
-        rc = dtx_set_flux_csc(h, DTX_FLUX_F, n, colptr.data(), rowval.data(), nzval.data());
+    {
+        const bool flux = rq.with_flux && c.x_num > 1;
+        if (flux) {
+            rc = set_flux(h, c, sg, dt);
+            if (rc) goto fail;
+        }
+        rc = dtx_synth_binary(h, rq.seed, 1.0);
         if (rc) goto fail;
+        // with the flux built on the device, 1/Ap and Vol are already there
+        rc = dtx_set_vectors(h, flux ? nullptr : ap.data(), flux ? nullptr : vol.data(),
+                             f0.data(), inj.data());
     }
-    rc = dtx_synth_binary(h, rq.seed, 1.0);
-    if (rc) goto fail;
-    rc = dtx_set_vectors(h, ap.data(), vol.data(), f0.data(), inj.data());
     if (rc) goto fail;
     {
         std::vector<double> w((size_t)N);
