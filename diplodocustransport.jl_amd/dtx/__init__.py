@@ -8,4 +8,4 @@ from . import grids, synth  # noqa: F401
 from ._lib import DtxError, load  # noqa: F401
 from .api import (BinaryMatricesStruct, CPUBackend, EmissionMatricesStruct,  # noqa: F401
                   FluxMatricesStruct, ForwardEulerStruct, ForwardSymplecticEulerStruct,
-                  HIPBackend, OutputStruct, Solve)
+                  HIPBackend, OutputStruct, Solve, SolutionFileLoad, SolutionFileSave)

@@ -308,11 +308,48 @@ class ForwardSymplecticEulerStruct(_SteppingStruct):
 def Solve(method: _SteppingStruct, dt_initial: float, t_save, progress: bool = False,
           fileName=None, fileLocation=None, Verbose: int = 1) -> OutputStruct:
     """Solve (src/Solver.jl:1-64): the time loop runs inside libdtx (dtx_solve) with the same
-    dt clipping and carry-over; the JLD2 write (:53-59) is not part of this package."""
-    if fileName is not None:
-        raise NotImplementedError("SolutionFileSave (JLD2) is not provided by this package")
+    dt clipping and carry-over; with fileName, the output and the PhaseSpace are written by
+    SolutionFileSave (:53-59)."""
     out, _ = method.solve_native(dt_initial, t_save, Verbose)
+    if fileName is not None:
+        print("Saving Simulation output")
+        SolutionFileSave(out, method.PhaseSpace, "." if fileLocation is None else fileLocation,
+                         fileName)
     return out
+
+
+def SolutionFileSave(sol: OutputStruct, PhaseSpace: PhaseSpace, fileLocation: str,
+                     fileName: str) -> None:
+    """SolutionFileSave (src/DataReading.jl:31-41): the entries "PhaseSpace" and "sol" in one
+    file at fileLocation/fileName, overwriting an existing file.  The container is a NumPy .npz
+    without pickled objects (JLD2 is a Julia format): "sol.f" (n_save x n), "sol.t" (n_save)
+    and "PhaseSpace" (the grid as JSON text)."""
+    import json
+    import os
+    from dataclasses import asdict
+    path = os.path.join(fileLocation, fileName)
+    f = np.stack([np.asarray(x, dtype=np.float64) for x in sol.f]) if len(sol.f) else \
+        np.zeros((0, PhaseSpace.n))
+    with open(path, "wb") as fh:  # an explicit handle: np.savez would append ".npz"
+        np.savez(fh, **{"sol.f": f, "sol.t": np.asarray(sol.t, dtype=np.float64),
+                        "PhaseSpace": np.array(json.dumps(asdict(PhaseSpace)))})
+
+
+def SolutionFileLoad(fileLocation: str, fileName: str):
+    """SolutionFileLoad (src/DataReading.jl:6-28): (PhaseSpace, sol) from a file written by
+    SolutionFileSave, with the reference's error for a missing file.  The path is joined with
+    the platform's separator (the reference joins with a literal backslash, :8)."""
+    import json
+    import os
+    from .grids import Species
+    path = os.path.join(fileLocation, fileName)
+    if not os.path.isfile(path):
+        raise RuntimeError(f"no file with name {fileName} found at location {fileLocation}")
+    with np.load(path, allow_pickle=False) as z:
+        ps = json.loads(str(z["PhaseSpace"]))
+        f, t = z["sol.f"], z["sol.t"]
+    ps["species"] = [Species(**sp) for sp in ps["species"]]
+    return PhaseSpace(**ps), OutputStruct(f=[row.copy() for row in f], t=t.copy())
 
 
 def FourFlow_N0(state: np.ndarray, ps: PhaseSpace, species: int, x_idx=1, y_idx=1, z_idx=1):

@@ -217,3 +217,19 @@ def test_status_of_unsynchronised_steps():
         np.testing.assert_array_equal(u, v)
     a.close()
     b.close()
+
+
+def test_solve_writes_solution_file(tmp_path):
+    """Solve with fileName saves the output and the PhaseSpace (src/Solver.jl:53-59); loading
+    the file gives back the same states and times."""
+    from dtx import SolutionFileLoad
+    w = workloads.tiny(steps=4)
+    m = workloads.make_struct(w)
+    out = Solve(m, w.dt, w.t_save(3), Verbose=0, fileName="tiny.jld2",
+                fileLocation=str(tmp_path))
+    ps, sol = SolutionFileLoad(str(tmp_path), "tiny.jld2")
+    assert ps == w.ps
+    assert len(sol.f) == len(out.f) == 3
+    for a, b in zip(sol.f, out.f):
+        assert np.array_equal(a, b)
+    assert np.array_equal(sol.t, np.asarray(out.t))
