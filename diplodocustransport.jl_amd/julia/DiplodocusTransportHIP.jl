@@ -256,3 +256,43 @@ function dtx_moments(method)
                        h.ptr, num, en))
     return num, en
 end
+
+# dtx_flux_synth of include/dtx.h (field order and types as in C)
+struct DtxFluxSynth
+    xr::Ptr{Float64}
+    yr::Ptr{Float64}
+    zr::Ptr{Float64}
+    pr::Ptr{Float64}
+    ur::Ptr{Float64}
+    hr::Ptr{Float64}
+    mass::NTuple{DTX_MAX_SPECIES,Float64}
+    charge::NTuple{DTX_MAX_SPECIES,Float64}
+    sync_K::Float64
+    axes::Int32
+    periodic::Int32
+end
+
+"""
+    dtx_synth_flux!(method, xr, yr, zr, pr_list, ur_list, hr_list, mass, charge;
+                    sync_K=0.0, axes=(:x,), periodic=true)
+
+Builds the synthetic Cartesian flux operators (upwind advection along the given axes,
+synchrotron cooling along p when sync_K > 0), 1 ./ Ap and Vol on the device, in place of the
+X_Flux / P_Flux CSC upload (src/Fluxes/BuildFluxMatrices.jl:10-184).  `pr_list[s]`,
+`ur_list[s]`, `hr_list[s]` are the bounds of species s; `mass` and `charge` one value per species.
+"""
+function dtx_synth_flux!(method, xr, yr, zr, pr_list, ur_list, hr_list, mass, charge;
+                         sync_K=0.0, axes=(:x,), periodic=true)
+    h = method.HIP
+    pad(v) = ntuple(i -> i <= length(v) ? Float64(v[i]) : 0.0, DTX_MAX_SPECIES)
+    bufs = map(v -> Vector{Float64}(v),
+               (xr, yr, zr, reduce(vcat, pr_list), reduce(vcat, ur_list), reduce(vcat, hr_list)))
+    GC.@preserve bufs begin
+        spec = DtxFluxSynth(map(pointer, bufs)..., pad(mass), pad(charge), Float64(sync_K),
+                            Int32((:x in axes ? 1 : 0) | (:y in axes ? 2 : 0)),
+                            Int32(periodic ? 1 : 0))
+        dtx_check(h, ccall((:dtx_synth_flux, libdtx), Cint, (Ptr{Cvoid}, Ref{DtxFluxSynth}),
+                           h.ptr, spec))
+    end
+    return nothing
+end
