@@ -309,7 +309,11 @@ def Solve(method: _SteppingStruct, dt_initial: float, t_save, progress: bool = F
           fileName=None, fileLocation=None, Verbose: int = 1) -> OutputStruct:
     """Solve (src/Solver.jl:1-64): the time loop runs inside libdtx (dtx_solve) with the same
     dt clipping and carry-over; with fileName, the output and the PhaseSpace are written by
-    SolutionFileSave (:53-59)."""
+    SolutionFileSave (:53-59).  A missing fileLocation is created before stepping, as the
+    reference's mkpath (:3-5) does."""
+    import os
+    if fileLocation is not None and not os.path.isdir(fileLocation):
+        os.makedirs(fileLocation, exist_ok=True)
     out, _ = method.solve_native(dt_initial, t_save, Verbose)
     if fileName is not None:
         print("Saving Simulation output")

@@ -11,9 +11,16 @@ from the reference's lines:
 from __future__ import annotations
 
 import math
+import struct
 
-THR_1F_10 = 1.00000001335143196e-10  # Float64(1f-10)
-THR_1F_20 = 9.99999968e-21           # Float64(1f-20)
+
+def _float32(x: float) -> float:
+    """The Float32 literal x promoted exactly to Float64 (round to nearest binary32)."""
+    return struct.unpack("<f", struct.pack("<f", x))[0]
+
+
+THR_1F_10 = _float32(1e-10)  # Float64(1f-10) = 1.00000001335143196e-10
+THR_1F_20 = _float32(1e-20)  # Float64(1f-20) = 9.999999682655225e-21
 
 
 def bool_mul(x: float, cond: bool) -> float:
@@ -79,14 +86,29 @@ def binary_term_triple(M3, f, N, C, Vol):
     return dB
 
 
+def div(a: float, b: float) -> float:
+    """IEEE a / b (Python raises on a zero divisor)."""
+    try:
+        return a / b
+    except ZeroDivisionError:
+        if a == 0.0 or math.isnan(a):
+            return math.nan
+        return math.copysign(math.inf, a) * math.copysign(1.0, b)
+
+
+def seq_sum(xs) -> float:
+    """Left-to-right sum from 0.0 (Python's sum() starts from the integer 0)."""
+    t = 0.0
+    for x in xs:
+        t = t + x
+    return t
+
+
 def neg_min_ratio(num, den):
+    """-mapreduce(x -> isnan(x) ? Inf : x, min, num ./ den), with Julia's min(-0.0, 0.0) = -0.0."""
     m = math.inf
     for a, b in zip(num, den):
-        try:
-            r = a / b
-        except ZeroDivisionError:
-            r = math.nan if a == 0.0 or math.isnan(a) else math.copysign(math.inf, a) * \
-                math.copysign(1.0, b)
+        r = div(a, b)
         if math.isnan(r):
             r = math.inf
         if r < m or (r == m and math.copysign(1.0, r) < 0):
@@ -118,22 +140,28 @@ def forward_euler(state, t_start, t_stop, dt, verbose):
         df = [a + b for a, b in zip(df, state["df_Bin"])]
     invAp = state["invAp"]
     df = [a * (ia * s) for a, ia in zip(df, invAp)]
-    stats = {"finite": math.isfinite(sum(df)), "cr": 0.0, "cr_evaluated": False}
-    if verbose in (1, 2, 3) and sum(f) != 0.0:
+    # src/SteppingMethods.jl:48-94: the stats start at 0.0 and are only set when sum(f) != 0
+    stats = {"finite": math.isfinite(seq_sum(df)), "cr": 0.0, "cr_bin": 0.0, "cr_emi": 0.0,
+             "cr_flux": 0.0, "cr_evaluated": False}
+    if verbose in (1, 2, 3) and seq_sum(f) != 0.0:
         stats["cr_evaluated"] = True
-        stats["cr"] = neg_min_ratio(df, f)
         if verbose == 3:
             if state.get("M_Bin") is not None:
                 stats["cr_bin"] = neg_min_ratio(state["df_Bin"], f)
+            if dE is not None:
+                stats["cr_emi"] = neg_min_ratio(dE, f)
             stats["cr_flux"] = neg_min_ratio([-v for v in dF], f)
+        stats["cr"] = neg_min_ratio(df, f)
     inj = state["df_inj"]
-    out = []
+    out, pre = [], []
     for i in range(n):
         x = f[i] + (df[i] + inj[i] * s)
+        pre.append(x)
         x = bool_mul(x, x >= THR_1F_10)
         x = bool_mul(x, x != math.inf)
         out.append(x)
     state["f"] = out
+    state["pre"] = pre
     return dt, save, stats
 
 
@@ -148,6 +176,7 @@ def forward_symplectic_euler(state, t_start, t_stop, dt, verbose):
     n = len(f)
     dP = csc_matvec(state["P"], f) if state.get("P") is not None else [0.0] * n
     dm = [-v for v in dP]
+    dE = None
     if state.get("M_Emi") is not None:
         dE = dense_matvec(state["M_Emi"], f, n, n)
         dm = [a + b for a, b in zip(dm, dE)]
@@ -168,17 +197,34 @@ def forward_symplectic_euler(state, t_start, t_stop, dt, verbose):
     for i in range(n):
         x = fm[i] + dS[i]
         fs.append(bool_mul(x, x >= THR_1F_20))
-    stats = {"cr": 0.0}
-    if verbose in (1, 2, 3) and sum(f) != 0.0:
+    # src/SteppingMethods.jl:175-229
+    stats = {"finite": True, "cr": 0.0, "cr_bin": 0.0, "cr_emi": 0.0, "cr_flux": 0.0,
+             "cr_momentum": 0.0, "cr_space": 0.0, "cr_evaluated": False}
+    if verbose in (1, 2, 3) and seq_sum(f) != 0.0:
+        stats["cr_evaluated"] = True
+        if verbose == 3:
+            if state.get("M_Bin") is not None:
+                stats["cr_bin"] = neg_min_ratio(
+                    [(b * ia) * s for b, ia in zip(state["df_Bin"], invAp)], f)
+            if dE is not None:
+                stats["cr_emi"] = neg_min_ratio([(e * ia) * s for e, ia in zip(dE, invAp)], f)
+            # Cr_PFlux: the ratio's NaNs are replaced by 0.0 before the reduction
+            ratio = [div(((-p) * ia) * s, x) for p, ia, x in zip(dP, invAp, f)]
+            ratio = [0.0 if math.isnan(r) else r for r in ratio]
+            stats["cr_flux"] = neg_min_ratio(ratio, [1.0] * n)
+            stats["cr_momentum"] = neg_min_ratio(dm, f)
+            stats["cr_space"] = neg_min_ratio(dS, fm)
         stats["cr"] = neg_min_ratio([a + b for a, b in zip(dS, dm)], f)
     inj = state["df_inj"]
-    out = []
+    out, pre = [], []
     for i in range(n):
         x = fs[i] + inj[i] * s
+        pre.append(x)
         x = bool_mul(x, x >= THR_1F_20) * jl_sign(x)
         x = bool_mul(x, x != math.inf)
         out.append(x)
     state["f"] = out
+    state["pre"] = pre
     return dt, save, stats
 
 

@@ -48,7 +48,16 @@ struct dtxo_ws {
     double* df_xflux;
     double* f_mom;
     double* f_space;
+    double* pre;       /* the state before the final clamps of the last step */
+    /* channel form evaluated for all cells at once: operator-row blocks x cells */
+    double* Tb;        /* rows x CELL_BATCH of the current channel          */
+    double* Fb;        /* f of the batch's cells, state-major: [N][CELL_BATCH] */
+    int64_t Tb_rows;
 };
+
+/* Cells evaluated together in the batched channel form: each operator row is read once per
+ * batch of this many cells. */
+#define CELL_BATCH 64
 
 int dtxo_num_threads(void) {
 #ifdef _OPENMP
@@ -84,6 +93,17 @@ dtxo_ws* dtxo_ws_create(const dtxo_problem* p) {
     w->df_xflux = zalloc(n);
     w->f_mom = zalloc(n);
     w->f_space = zalloc(n);
+    w->pre = zalloc(n);
+    if (p->batched && p->M_Bin == NULL) {
+        int64_t rows = 0;
+        for (int32_t g = 0; g < p->n_gain; ++g) {
+            int64_t r = p->species_size[p->gain[g].s_out] * p->species_size[p->gain[g].s1];
+            if (r > rows) rows = r;
+        }
+        w->Tb_rows = rows;
+        w->Tb = zalloc(rows * CELL_BATCH);
+        w->Fb = zalloc(p->N * CELL_BATCH);
+    }
     return w;
 }
 
@@ -91,12 +111,13 @@ void dtxo_ws_destroy(dtxo_ws* w) {
     if (!w) return;
     free(w->f); free(w->df); free(w->df_flux); free(w->df_emi); free(w->df_bin);
     free(w->df_tmp); free(w->T); free(w->df_mom); free(w->df_space); free(w->df_xflux);
-    free(w->f_mom); free(w->f_space);
+    free(w->f_mom); free(w->f_space); free(w->pre); free(w->Tb); free(w->Fb);
     free(w);
 }
 
 double* dtxo_ws_f(dtxo_ws* w) { return w->f; }
 const double* dtxo_ws_df_bin(dtxo_ws* w) { return w->df_bin; }
+const double* dtxo_ws_pre_clamp(dtxo_ws* w) { return w->pre; }
 
 /* Julia's Bool strong-zero product  x * (cond). */
 static inline double bool_mul(double x, int cond) { return cond ? x : copysign(0.0, x); }
@@ -184,9 +205,103 @@ static void channel_cell(const dtxo_problem* p, const double* fc, double* dB, do
     }
 }
 
+/* channel_cell for a batch of cells (cells[0..nc), nc <= CELL_BATCH) with the same arithmetic
+ * per (row, cell): T[r] = sum_j G[r, j] f2_j from 0.0 in ascending j, then
+ * out[k] = out[k] + (sum_i T[k + No i] f1_i from 0.0 in ascending i), channels in list order,
+ * then the losses.  Only the loop nest differs: each operator row block is read once for the
+ * batch instead of once per cell. */
+static void channel_batch(const dtxo_problem* p, dtxo_ws* w, const int64_t* cells, int64_t nc) {
+    const int64_t N = p->N;
+    double* Fb = w->Fb;  /* Fb[k * CELL_BATCH + c] = f of cell c, state k */
+    for (int64_t c = 0; c < nc; ++c)
+        for (int64_t k = 0; k < N; ++k) {
+            Fb[k * CELL_BATCH + c] = w->f[cells[c] * N + k];
+            w->df_bin[cells[c] * N + k] = 0.0;
+        }
+    for (int32_t g = 0; g < p->n_gain; ++g) {
+        const dtxo_gain* ch = &p->gain[g];
+        const int64_t No = p->species_size[ch->s_out], N1 = p->species_size[ch->s1],
+                      N2 = p->species_size[ch->s2];
+        const int64_t o1 = p->species_offset[ch->s1], o2 = p->species_offset[ch->s2],
+                      oo = p->species_offset[ch->s_out];
+        const int64_t rows = No * N1, RB = 256;
+        double* Tb = w->Tb;
+#pragma omp parallel for schedule(static)
+        for (int64_t b = 0; b < (rows + RB - 1) / RB; ++b) {
+            const int64_t r0 = b * RB, r1 = r0 + RB < rows ? r0 + RB : rows;
+            for (int64_t r = r0; r < r1; ++r)
+                for (int64_t c = 0; c < nc; ++c) Tb[r * CELL_BATCH + c] = 0.0;
+            for (int64_t j = 0; j < N2; ++j) {
+                const double* Gj = ch->G + rows * j;
+                const double* fj = Fb + (o2 + j) * CELL_BATCH;
+                for (int64_t r = r0; r < r1; ++r) {
+                    const double a = Gj[r];
+                    double* t = Tb + r * CELL_BATCH;
+                    for (int64_t c = 0; c < nc; ++c) t[c] = t[c] + a * fj[c];
+                }
+            }
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < No; ++k) {
+            double acc[CELL_BATCH];
+            for (int64_t c = 0; c < nc; ++c) acc[c] = 0.0;
+            for (int64_t i = 0; i < N1; ++i) {
+                const double* t = Tb + (k + No * i) * CELL_BATCH;
+                const double* fi = Fb + (o1 + i) * CELL_BATCH;
+                for (int64_t c = 0; c < nc; ++c) acc[c] = acc[c] + t[c] * fi[c];
+            }
+            for (int64_t c = 0; c < nc; ++c) {
+                double* out = w->df_bin + cells[c] * N + oo + k;
+                *out = *out + acc[c];
+            }
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nc; ++c)
+        for (int32_t l = 0; l < p->n_loss; ++l) {
+            const dtxo_loss* ch = &p->loss[l];
+            const int64_t N1 = p->species_size[ch->s1], N2 = p->species_size[ch->s2];
+            const int64_t o1 = p->species_offset[ch->s1], o2 = p->species_offset[ch->s2];
+            double* out = w->df_bin + cells[c] * N + o1;
+            for (int64_t a = 0; a < N1; ++a) {
+                double acc = 0.0;
+                for (int64_t q = 0; q < N2; ++q)
+                    acc = acc + ch->L[a + N1 * q] * Fb[(o2 + q) * CELL_BATCH + c];
+                out[a] = out[a] - Fb[(o1 + a) * CELL_BATCH + c] * acc;
+            }
+        }
+}
+
 /* update_Big_Bin!  src/SteppingMethods.jl:332-397 */
 static void update_big_bin(const dtxo_problem* p, dtxo_ws* w) {
     int64_t N = p->N;
+    if (p->batched && p->M_Bin == NULL) {
+        int64_t cells[CELL_BATCH], nc = 0;
+        for (int64_t off = 0; off < p->C; ++off) {
+            const double* fv = w->f + off * N;
+            int all_zero = 1;
+            for (int64_t k = 0; k < N; ++k)
+                if (fv[k] != 0.0) { all_zero = 0; break; }
+            if (!(p->n_domain < 0 || in_domain(p, off) || all_zero)) continue;  /* :369 */
+            cells[nc++] = off;
+            if (nc == CELL_BATCH || off == p->C - 1) {
+                channel_batch(p, w, cells, nc);
+                nc = 0;
+            }
+        }
+        if (nc) channel_batch(p, w, cells, nc);
+        for (int64_t off = 0; off < p->C; ++off) {
+            const double* fv = w->f + off * N;
+            int all_zero = 1;
+            for (int64_t k = 0; k < N; ++k)
+                if (fv[k] != 0.0) { all_zero = 0; break; }
+            if (!(p->n_domain < 0 || in_domain(p, off) || all_zero)) continue;
+            double* dB = w->df_bin + off * N;
+            const double vol = p->Vol[off];
+            for (int64_t k = 0; k < N; ++k) dB[k] = dB[k] * vol;  /* :380 */
+        }
+        return;
+    }
     for (int64_t off = 0; off < p->C; ++off) {          /* x, y, z loop, z fastest (:360-362) */
         const double* fv = w->f + off * N;              /* fView (:364-367) */
         int all_zero = 1;
@@ -295,6 +410,7 @@ int dtxo_forward_euler(const dtxo_problem* p, dtxo_ws* w, double t_start, double
     }
     for (int64_t i = 0; i < n; ++i) {                            /* :97-101 */
         double x = w->f[i] + (w->df[i] + p->df_inj[i] * s);
+        w->pre[i] = x;
         x = bool_mul(x, x >= THR_1F_10);
         x = bool_mul(x, x != INFINITY);
         w->f[i] = x;
@@ -362,14 +478,20 @@ int dtxo_forward_symplectic_euler(const dtxo_problem* p, dtxo_ws* w, double t_st
                     double r = (((-w->df_flux[i]) * p->invAp[i]) * s) / w->f[i];
                     w->df_tmp[i] = isnan(r) ? 0.0 : r;     /* replace!(df_tmp, NaN=>0.0) */
                 }
+                /* after replace!(NaN => 0.0) the isnan -> Inf map of the reduction finds no
+                 * NaN left, so it is a plain minimum */
                 {
                     double m = INFINITY;
                     for (int64_t i = 0; i < n; ++i) {
                         double r = w->df_tmp[i];
-                        if (r < m) m = r;
+                        if (r < m || (r == m && signbit(r))) m = r;
                     }
                     local.cr_flux = -m;
                 }
+                /* Cr_Momentum = -min(df_Momentum ./ f)                          (:218-219) */
+                local.cr_momentum = neg_min_ratio(w->df_mom, w->f, n);
+                /* Cr_Space = -min(df_Space ./ f_Momentum)                        (:222-223) */
+                local.cr_space = neg_min_ratio(w->df_space, w->f_mom, n);
             }
             for (int64_t i = 0; i < n; ++i) w->df_tmp[i] = w->df_space[i] + w->df_mom[i];
             local.cr = neg_min_ratio(w->df_tmp, w->f, n);
@@ -378,6 +500,7 @@ int dtxo_forward_symplectic_euler(const dtxo_problem* p, dtxo_ws* w, double t_st
     }
     for (int64_t i = 0; i < n; ++i) {                            /* :233-237 */
         double x = w->f_space[i] + p->df_inj[i] * s;
+        w->pre[i] = x;
         x = bool_mul(x, x >= THR_1F_20) * jl_sign(x);
         x = bool_mul(x, x != INFINITY);
         w->f[i] = x;

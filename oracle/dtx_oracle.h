@@ -60,15 +60,22 @@ typedef struct {
     const double* invAp;  /* n   (1 ./ Ap_Flux)   */
     const double* Vol;    /* C                    */
     const double* df_inj; /* n                    */
+    /* Channel form only: 1 = evaluate the channels for all cells at once (each operator row
+     * read once per step, every (row, cell) sum in the same order as the per-cell loop, so the
+     * result is bitwise the same), 0 = cell by cell as update_Big_Bin! loops. */
+    int32_t batched;
 } dtxo_problem;
 
 typedef struct {
-    int    finite;        /* isfinite(sum(df))                                  */
+    int    finite;        /* isfinite(sum(df))  (ForwardEuler; 1 for the symplectic method)  */
     int    cr_evaluated;  /* Verbose in 1..3 and sum(f) != 0                    */
-    double cr, cr_bin, cr_emi, cr_flux;
+    double cr, cr_bin, cr_emi;
+    double cr_flux;       /* Cr_Flux (ForwardEuler) or Cr_PFlux (symplectic)    */
     double dt;            /* dt actually used (after the clip)                  */
     double t_start;
     int    save;
+    double cr_momentum;   /* Cr_Momentum (symplectic, Verbose 3)                */
+    double cr_space;      /* Cr_Space    (symplectic, Verbose 3)                */
 } dtxo_stats;
 
 typedef struct dtxo_ws dtxo_ws;
@@ -79,6 +86,10 @@ void     dtxo_ws_destroy(dtxo_ws* w);
 double*  dtxo_ws_f(dtxo_ws* w);
 /* binary term of the last step (length n), as kept in method.df_Bin */
 const double* dtxo_ws_df_bin(dtxo_ws* w);
+/* The last step's state before its clamps (length n): f + (df + df_Inj s) for ForwardEuler,
+ * f_Space + df_Inj s for the symplectic method.  A NaN there is clamped to a zero whose sign
+ * is the NaN's, which depends on the hardware that produced the NaN. */
+const double* dtxo_ws_pre_clamp(dtxo_ws* w);
 
 /* One ForwardEuler functor call: returns (dt, save) through dt_out/save_out. */
 int dtxo_forward_euler(const dtxo_problem* p, dtxo_ws* w, double t_start, double t_stop,

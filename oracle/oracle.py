@@ -38,12 +38,13 @@ class Problem(C.Structure):
                 ("F", Csc), ("X", Csc), ("P", Csc), ("M_Emi", PD), ("M_Bin", PD),
                 ("n_gain", I32), ("n_loss", I32), ("gain", C.POINTER(Gain)),
                 ("loss", C.POINTER(Loss)), ("domain", PI64), ("n_domain", I64),
-                ("invAp", PD), ("Vol", PD), ("df_inj", PD)]
+                ("invAp", PD), ("Vol", PD), ("df_inj", PD), ("batched", I32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("finite", C.c_int), ("cr_evaluated", C.c_int), ("cr", D), ("cr_bin", D),
-                ("cr_emi", D), ("cr_flux", D), ("dt", D), ("t_start", D), ("save", C.c_int)]
+                ("cr_emi", D), ("cr_flux", D), ("dt", D), ("t_start", D), ("save", C.c_int),
+                ("cr_momentum", D), ("cr_space", D)]
 
 
 _lib = None
@@ -64,6 +65,10 @@ def load():
         lib.dtxo_ws_destroy.argtypes = [C.c_void_p]
         lib.dtxo_ws_f.restype = PD
         lib.dtxo_ws_f.argtypes = [C.c_void_p]
+        lib.dtxo_ws_pre_clamp.restype = PD
+        lib.dtxo_ws_pre_clamp.argtypes = [C.c_void_p]
+        lib.dtxo_ws_df_bin.restype = PD
+        lib.dtxo_ws_df_bin.argtypes = [C.c_void_p]
         lib.dtxo_forward_euler.argtypes = [C.POINTER(Problem), C.c_void_p, D, D, D, C.c_int,
                                            PD, C.POINTER(C.c_int), C.POINTER(Stats)]
         lib.dtxo_forward_symplectic_euler.argtypes = lib.dtxo_forward_euler.argtypes
@@ -97,7 +102,7 @@ class OracleProblem:
     form).  F / X / P: (colptr, rowval, nzval) 1-based CSC.  M_Emi: dense n x n."""
 
     def __init__(self, ps, Ap, Vol, df_inj, F=None, X=None, P=None, M_Emi=None, M_Bin=None,
-                 gains=(), losses=(), domain=None):
+                 gains=(), losses=(), domain=None, batched=False):
         self.ps = ps
         self.keep = []
         p = Problem()
@@ -140,6 +145,7 @@ class OracleProblem:
         inj = np.ascontiguousarray(df_inj, dtype=np.float64)
         self.keep += [invAp, vol, inj]
         p.invAp, p.Vol, p.df_inj = _dp(invAp), _dp(vol), _dp(inj)
+        p.batched = 1 if batched else 0
         self.p = p
 
     def binary_term(self, f):
@@ -185,6 +191,17 @@ class Stepper:
         n = self.problem.ps.n
         return np.ctypeslib.as_array(self.lib.dtxo_ws_f(self.ws), shape=(n,)).copy()
 
+    @property
+    def pre_clamp(self):
+        """The last step's state before its final clamps."""
+        n = self.problem.ps.n
+        return np.ctypeslib.as_array(self.lib.dtxo_ws_pre_clamp(self.ws), shape=(n,)).copy()
+
+    @property
+    def df_bin(self):
+        n = self.problem.ps.n
+        return np.ctypeslib.as_array(self.lib.dtxo_ws_df_bin(self.ws), shape=(n,)).copy()
+
     def __call__(self, t_start, t_stop, dt, verbose):
         dt_out = C.c_double()
         save = C.c_int()
@@ -221,11 +238,12 @@ def set_threads(n: int):
     os.environ["OMP_NUM_THREADS"] = str(n)
 
 
-def problem_from_workload(w, use_monolithic=None):
-    """The oracle problem for a dtx.workloads.Workload (same arrays the HIP path receives)."""
+def problem_from_workload(w, use_monolithic=None, batched=False):
+    """The oracle problem for a dtx.workloads.Workload (same arrays the HIP path receives).
+    batched: channel form evaluated for many cells at once (bitwise the same result)."""
     from dtx.workloads import emission_dense
     mono = (w.M_Bin is not None) if use_monolithic is None else use_monolithic
-    kw = {}
+    kw = {"batched": batched}
     if mono:
         kw["M_Bin"] = w.M_Bin
     else:

@@ -29,3 +29,26 @@ def test_save_load_round_trip(tmp_path):
 def test_load_missing_file(tmp_path):
     with pytest.raises(RuntimeError, match="no file with name nope.jld2 found at location"):
         SolutionFileLoad(str(tmp_path), "nope.jld2")
+
+
+def test_solve_creates_missing_file_location(tmp_path):
+    """Solve creates a missing fileLocation before stepping (src/Solver.jl:3-5), so the save at
+    the end cannot fail on the directory; a stand-in method records that the directory was
+    already there when the time loop started."""
+    from dtx import Solve
+    ps = PhaseSpace(species_pair(2, 1, 1), x_num=1)
+    where = tmp_path / "a" / "b"
+
+    class Method:
+        PhaseSpace = ps
+        seen = None
+
+        def solve_native(self, dt, t_save, Verbose):
+            Method.seen = where.is_dir()
+            return OutputStruct(f=[np.zeros(ps.n) for _ in t_save], t=np.asarray(t_save)), 1
+
+    out = Solve(Method(), 0.1, [0.0, 0.1], fileName="x.jld2", fileLocation=str(where),
+                Verbose=0)
+    assert Method.seen is True
+    ps2, sol = SolutionFileLoad(str(where), "x.jld2")
+    assert ps2 == ps and len(sol.f) == len(out.f) == 2
