@@ -14,7 +14,8 @@ time step and the step count.
   transport_2d     -- 32 x 32 periodic x-y cells, 2 species x (32 p x 8 u x 2 phi), advection in
                       x and y, cooling in p, device-generated gain tensors shared by all cells,
                       500 steps.
-The time step is 0.1 / (fastest loss rate of the initial state).
+The time step is 0.1 / (fastest loss rate of the initial state), rounded down to 12 significant
+bits so that save times are exact multiples of it.
 """
 from __future__ import annotations
 
@@ -64,8 +65,8 @@ class Workload:
 
     def t_save(self, n_save: int = 2):
         """Save times after 0, steps/(n_save-1), ..., steps steps of size dt, accumulated as
-        Solve accumulates t_start (src/Solver.jl:36), so every interval ends on a step
-        boundary and the clipped dt stays equal to dt up to rounding."""
+        Solve accumulates t_start (src/Solver.jl:36).  With the workloads' 12-bit dt every sum
+        is exact: each interval ends on a step and the clipped dt equals dt."""
         marks = np.linspace(0, self.steps, n_save).round().astype(int)
         out, t, k = [0.0], 0.0, 0
         for m in marks[1:]:
@@ -81,13 +82,23 @@ def species_pair(p_num=32, u_num=8, h_num=1, p_low=-3.0, p_up=3.0):
             Species("Pho", 0.0, 0.0, p_num, u_num, h_num, p_low, p_up)]
 
 
+def _exact_dt(dt: float) -> float:
+    """dt rounded down to 12 significant bits.  Every save time k*dt (k < 2^41) and every
+    Solve accumulation t_start + dt is then exact, so each save interval ends exactly on a step
+    and the clipped dt that Solve carries into the next interval (src/Solver.jl:35) equals dt;
+    with an unrounded dt the carried value drifts by an ulp per interval until an interval ends
+    a rounding error after a step."""
+    m, e = math.frexp(dt)
+    return math.ldexp(math.floor(m * 4096.0) / 4096.0, e)
+
+
 def _dt_from_losses(ps, losses, f0, Ap, Vol):
     N = ps.n_momentum
     worst = 0.0
     for c in range(ps.n_space):
         worst = max(worst, synth.max_loss_rate(ps, losses, f0[c * N:(c + 1) * N], Vol[c],
                                                Ap[c * N]))
-    return 0.1 / worst
+    return _exact_dt(0.1 / worst)
 
 
 def _dt_expected(ps, f0, Ap, Vol):
@@ -106,7 +117,7 @@ def _dt_expected(ps, f0, Ap, Vol):
                 elif a == y:
                     rate += sa.size * E_HALF * tot[x]
             worst = max(worst, rate * Vol[c] / Ap[c * N])
-    return 0.1 / worst
+    return _exact_dt(0.1 / worst)
 
 
 def _sync_K(ps, dt, cooling_steps=100.0):
@@ -211,6 +222,26 @@ def emission_dense(w: Workload) -> Optional[np.ndarray]:
         # blocks are stored column-major: element (i, j) at [c, j, i] of the C-order array
         M[c * N:(c + 1) * N, c * N:(c + 1) * N] = w.emission_blocks[c].T
     return M
+
+
+def emission_csc(w: Workload):
+    """The per-cell emission blocks as the reference's sparse M_Emi (SparseMatrixCSC, 1-based
+    Int64, every block entry stored): (colptr, rowval, nzval), or None."""
+    if w.emission_blocks is None:
+        return None
+    n, N, C = w.ps.n, w.ps.n_momentum, w.ps.n_space
+    colptr = np.empty(n + 1, np.int64)
+    colptr[0] = 1
+    rowval = np.empty(n * N, np.int64)
+    nzval = np.empty(n * N)
+    for c in range(C):
+        for j in range(N):
+            col = c * N + j
+            k0 = col * N
+            rowval[k0:k0 + N] = np.arange(c * N, (c + 1) * N) + 1
+            nzval[k0:k0 + N] = w.emission_blocks[c, j, :]  # column j of the block (rows i)
+            colptr[col + 1] = k0 + N + 1
+    return colptr, rowval, nzval
 
 
 def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=None,

@@ -165,6 +165,14 @@ static int in_domain(const dtxo_problem* p, int64_t off) {
     return 0;
 }
 
+static int has_emission(const dtxo_problem* p) { return p->M_Emi != NULL || p->Emi.ncols > 0; }
+
+/* mul!(df_Emi, M_Emi, f) with the dense or the sparse M_Emi */
+static void emission_mul(const dtxo_problem* p, int64_t n, const double* f, double* y) {
+    if (p->Emi.ncols > 0) csc_mul(&p->Emi, n, f, y);
+    else dense_mul(p->M_Emi, n, n, f, y);
+}
+
 static int has_binary(const dtxo_problem* p) {
     return p->M_Bin != NULL || p->n_gain > 0 || p->n_loss > 0;
 }
@@ -376,10 +384,10 @@ int dtxo_forward_euler(const dtxo_problem* p, dtxo_ws* w, double t_start, double
 
     csc_mul(&p->F, n, w->f, w->df_flux);                         /* :31 */
     for (int64_t i = 0; i < n; ++i) w->df[i] = -w->df_flux[i];   /* :32 */
-    int emission = p->M_Emi != NULL;
+    int emission = has_emission(p);
     int binary = has_binary(p);
     if (emission) {                                              /* :35-38 */
-        dense_mul(p->M_Emi, n, n, w->f, w->df_emi);
+        emission_mul(p, n, w->f, w->df_emi);
         for (int64_t i = 0; i < n; ++i) w->df[i] = w->df[i] + w->df_emi[i];
     }
     if (binary) {                                                /* :41-44 */
@@ -429,13 +437,13 @@ int dtxo_forward_symplectic_euler(const dtxo_problem* p, dtxo_ws* w, double t_st
     int save;
     clip_dt(t_start, t_stop, &dt, &save);                        /* :126-131 */
     double s = dt / p->dt0;                                      /* :135 */
-    int emission = p->M_Emi != NULL;
+    int emission = has_emission(p);
     int binary = has_binary(p);
 
     csc_mul(&p->P, n, w->f, w->df_flux);                         /* :140 df_PFlux */
     for (int64_t i = 0; i < n; ++i) w->df_mom[i] = -w->df_flux[i];
     if (emission) {
-        dense_mul(p->M_Emi, n, n, w->f, w->df_emi);
+        emission_mul(p, n, w->f, w->df_emi);
         for (int64_t i = 0; i < n; ++i) w->df_mom[i] = w->df_mom[i] + w->df_emi[i];
     }
     if (binary) {

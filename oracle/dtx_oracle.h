@@ -64,6 +64,11 @@ typedef struct {
      * read once per step, every (row, cell) sum in the same order as the per-cell loop, so the
      * result is bitwise the same), 0 = cell by cell as update_Big_Bin! loops. */
     int32_t batched;
+    /* M_Emi as SparseMatrixCSC (Emi_sparse, src/Collisions/BuildEmissionMatrices.jl:28-39):
+     * used instead of the dense M_Emi when Emi.ncols > 0.  For finite states both give the same
+     * df_Emi; a non-finite f reaches every row of the dense product through its stored zeros
+     * (0 * Inf = NaN) but only the stored entries' rows of the sparse one. */
+    dtxo_csc Emi;
 } dtxo_problem;
 
 typedef struct {

@@ -38,7 +38,7 @@ class Problem(C.Structure):
                 ("F", Csc), ("X", Csc), ("P", Csc), ("M_Emi", PD), ("M_Bin", PD),
                 ("n_gain", I32), ("n_loss", I32), ("gain", C.POINTER(Gain)),
                 ("loss", C.POINTER(Loss)), ("domain", PI64), ("n_domain", I64),
-                ("invAp", PD), ("Vol", PD), ("df_inj", PD), ("batched", I32)]
+                ("invAp", PD), ("Vol", PD), ("df_inj", PD), ("batched", I32), ("Emi", Csc)]
 
 
 class Stats(C.Structure):
@@ -102,7 +102,7 @@ class OracleProblem:
     form).  F / X / P: (colptr, rowval, nzval) 1-based CSC.  M_Emi: dense n x n."""
 
     def __init__(self, ps, Ap, Vol, df_inj, F=None, X=None, P=None, M_Emi=None, M_Bin=None,
-                 gains=(), losses=(), domain=None, batched=False):
+                 gains=(), losses=(), domain=None, batched=False, M_Emi_csc=None):
         self.ps = ps
         self.keep = []
         p = Problem()
@@ -146,6 +146,7 @@ class OracleProblem:
         self.keep += [invAp, vol, inj]
         p.invAp, p.Vol, p.df_inj = _dp(invAp), _dp(vol), _dp(inj)
         p.batched = 1 if batched else 0
+        p.Emi = _csc(self.keep, M_Emi_csc)
         self.p = p
 
     def binary_term(self, f):
@@ -238,15 +239,19 @@ def set_threads(n: int):
     os.environ["OMP_NUM_THREADS"] = str(n)
 
 
-def problem_from_workload(w, use_monolithic=None, batched=False):
+def problem_from_workload(w, use_monolithic=None, batched=False, sparse_emission=False):
     """The oracle problem for a dtx.workloads.Workload (same arrays the HIP path receives).
-    batched: channel form evaluated for many cells at once (bitwise the same result)."""
-    from dtx.workloads import emission_dense
+    batched: channel form evaluated for many cells at once (bitwise the same result).
+    sparse_emission: M_Emi as the CSC of its per-cell blocks instead of the dense n x n."""
+    from dtx.workloads import emission_csc, emission_dense
     mono = (w.M_Bin is not None) if use_monolithic is None else use_monolithic
     kw = {"batched": batched}
+    if sparse_emission:
+        kw["M_Emi_csc"] = emission_csc(w)
     if mono:
         kw["M_Bin"] = w.M_Bin
     else:
         kw["gains"], kw["losses"] = w.gains, w.losses
     return OracleProblem(w.ps, w.Ap, w.Vol, w.inj, F=w.F, X=w.X, P=w.P,
-                         M_Emi=emission_dense(w), domain=w.domain, **kw)
+                         M_Emi=None if sparse_emission else emission_dense(w), domain=w.domain,
+                         **kw)

@@ -18,7 +18,7 @@ from dtx import grids
 
 INF, NAN = math.inf, math.nan
 THR10 = float(np.float32(1e-10))   # Float64(1f-10) = 1.00000001335143196e-10
-THR20 = float(np.float32(1e-20))   # Float64(1f-20) = 9.99999968e-21
+THR20 = float(np.float32(1e-20))   # Float64(1f-20) = 9.999999682655225e-21
 
 
 def csc(dense):
@@ -141,7 +141,26 @@ SYMPLECTIC_CLAMP = HandCase(
     expect=dict(cr_evaluated=0, cr=0.0),
 )
 
-CASES = [EULER_BINARY, EULER_NONFINITE, SYMPLECTIC, SYMPLECTIC_CLAMP]
+# Forward Euler clamps f * (f >= 1f-10) then f * (f != Inf) (src/SteppingMethods.jl:97-101) on the
+# state itself: with no operators df = -(+0.0) = -0.0 and f + (df + 0.0) = f (except -0.0 + 0.0).
+EULER_CLAMP_F = [-1.0, 5e-11, THR10, float(np.nextafter(THR10, 0.0)),
+                 float(np.nextafter(THR10, 1.0)), INF, 2.0, NAN, -INF, -0.0, 1e-300]
+EULER_CLAMP = HandCase(
+    name="euler_clamp", method=0, f=EULER_CLAMP_F,
+    #  -1 -> -1 * false = -0.0 (kept by the Inf clamp);  5e-11 -> +0.0;  1f-10 kept;
+    #  1f-10 - 1 ulp -> +0.0;  1f-10 + 1 ulp kept;  Inf -> kept, then Inf * false = +0.0;
+    #  2 kept;  NaN -> NaN * false = +-0 (the NaN's sign);  -Inf -> -0.0;
+    #  -0.0 -> -0.0 + (-0.0 + 0.0) = +0.0 -> +0.0;  1e-300 -> +0.0
+    # sum(f) = NaN != 0, so Cr is evaluated: df / f = -0.0 / f has no value below -0.0
+    # (NaN -> Inf), so Cr = -(-0.0) = +0.0; sum(df) = -0.0 + ... is finite
+    expect_f=[-0.0, 0.0, THR10, 0.0, float(np.nextafter(THR10, 1.0)), 0.0, 2.0, 0.0, -0.0, 0.0,
+              0.0],
+    nan_sign_free=[7],
+    expect=dict(finite=1, cr_evaluated=1, cr=0.0, cr_bin=0.0, cr_emi=0.0, cr_flux=0.0),
+    expect_v1=dict(finite=1, cr_evaluated=1, cr=0.0),
+)
+
+CASES = [EULER_BINARY, EULER_NONFINITE, EULER_CLAMP, SYMPLECTIC, SYMPLECTIC_CLAMP]
 
 
 def same_bits(a: float, b: float) -> bool:

@@ -125,3 +125,25 @@ def test_batched_channels_equal_cell_loop(kw):
     fa = a.solve(w.f0, w.dt, w.t_save(3))[0]
     fb = b.solve(w.f0, w.dt, w.t_save(3))[0]
     np.testing.assert_array_equal(fa, fb)
+
+
+def test_sparse_and_dense_emission():
+    """The oracle's sparse M_Emi (Emi_sparse, src/Collisions/BuildEmissionMatrices.jl:28-39)
+    gives the dense product's df_Emi bit for bit on finite states; with a NaN in one cell the
+    dense n x n product spreads it to every cell through the stored zeros of the other blocks
+    (0 * NaN = NaN), the sparse one only to its own cell."""
+    w = workloads.tiny(steps=3, emission=True, x_num=3, with_flux=False)
+    dense = O.problem_from_workload(w)
+    sparse = O.problem_from_workload(w, sparse_emission=True)
+    fa = dense.solve(w.f0, w.dt, w.t_save(2))[0]
+    fb = sparse.solve(w.f0, w.dt, w.t_save(2))[0]
+    np.testing.assert_array_equal(fa, fb)
+    f0 = np.array(w.f0)
+    N = w.ps.n_momentum
+    f0[2 * N + 1] = math.nan
+    a, b = O.Stepper(dense, f0), O.Stepper(sparse, f0)
+    a(0.0, 1.0, w.dt, 0)
+    b(0.0, 1.0, w.dt, 0)
+    assert np.all(np.isnan(a.pre_clamp))
+    assert not np.any(np.isnan(b.pre_clamp[:2 * N]))   # cells 0, 1 untouched by the sparse one
+    assert np.all(np.isnan(b.pre_clamp[2 * N:]))
