@@ -172,6 +172,13 @@ struct CorrectArgs {
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------
 hipError_t launch_gain_correct(const CorrectArgs& a, hipStream_t stream);
+// Grouped forms (serving): member g of the table owns blocks [start[g], start[g+1]) of the
+// launch, which runs the member's own grid with its own arithmetic.  The coefficient launch
+// covers the members whose update grid is too large to derive them per workgroup (coef set).
+hipError_t launch_gain_correct_group(const CorrectArgs* table, const int32_t* start, int count,
+                                     int total_blocks, hipStream_t stream);
+hipError_t launch_coef_group(const UpdateArgs* table, const int32_t* start, int count,
+                             int total_blocks, hipStream_t stream);
 hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems,
                          hipStream_t stream);
 // Items of the narrow 64-row shapes (tc 1, 16 and 64, per item) in one launch: grouped steps
@@ -247,8 +254,16 @@ struct Group;
 Group* group_create(int device);
 void group_destroy(Group* g);
 const char* group_error(const Group* g);
+hipStream_t group_stream(const Group* g);
+// Packs a handle's operators and builds its work tables (from any thread; the handle must not be
+// in use elsewhere); required before a handle joins a group.
+int group_prepare(dtx_handle* h);
+// Members must be prepared; the tables are enqueued on the group stream (no device wait).
 int group_set(Group* g, dtx_handle* const* hs, int count);
 int group_step(Group* g, const double* dt, int verbose);
 int group_sync(Group* g);
+// read-only views of a handle for the server
+const dtx_config* handle_config(const dtx_handle* h);
+bool handle_has_vectors(const dtx_handle* h);
 
 }  // namespace dtx

@@ -672,6 +672,17 @@ __global__ __launch_bounds__(UB) void finalize_single_kernel(UpdateArgs a, int n
     finalize_single(a, nseg, red);
 }
 
+// Member of a grouped launch that owns block b: the last g with start[g] <= b.
+__device__ inline int group_member(const int32_t* start, int count, int b) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Grouped forms for several simulations in one launch (serving): block b of the update grid
 // belongs to the simulation g with block_start[g] <= b < block_start[g+1]; the finalize grid has
 // one block per simulation.
@@ -682,12 +693,7 @@ __global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs
     __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
     const int b = blockIdx.x;
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {  // last g with block_start[g] <= b
-        const int mid = (lo + hi + 1) >> 1;
-        if (block_start[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
+    const int lo = group_member(block_start, count, b);
     const UpdateArgs a = table[lo];
     const int nseg = (a.N + UPDATE_STATES - 1) / UPDATE_STATES;
     const int local = b - block_start[lo];
@@ -843,9 +849,8 @@ __device__ inline double slot_sum(const CorrectArgs& a, int64_t idx, int q, int 
 // a fixed order ((s0 + s1) + (s2 + s3) for four).  Gain sums go to gsum, loss sums
 // to lsum, and the workgroup's number and energy moments of its gain sums (sum g, sum E g,
 // sum E^2 g) to cpart[c][s][z].
-__global__ __launch_bounds__(UB) void correct_sums_kernel(CorrectArgs a) {
-    __shared__ double red[(UB / 64) * RED_VALUES];
-    const int c = blockIdx.x, s = blockIdx.y, z = blockIdx.z, tid = threadIdx.x;
+__device__ inline void correct_sums_body(const CorrectArgs& a, int c, int s, int z, double* red) {
+    const int tid = threadIdx.x;
     const int q = tid & (a.tps - 1);
     const int k0 = a.species_off[s], ns = a.species_off[s + 1] - k0;
     const int kb = z * (UB / a.tps);
@@ -877,8 +882,51 @@ __global__ __launch_bounds__(UB) void correct_sums_kernel(CorrectArgs a) {
         a.cpart[(((int64_t)c * a.S + s) * a.zb + z) * 3 + tid] = wave_combine(tid, true, red);
 }
 
+__global__ __launch_bounds__(UB) void correct_sums_kernel(CorrectArgs a) {
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    correct_sums_body(a, blockIdx.x, blockIdx.y, blockIdx.z, red);
+}
+
 hipError_t launch_gain_correct(const CorrectArgs& a, hipStream_t stream) {
     correct_sums_kernel<<<dim3(a.C, a.S, a.zb), UB, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+// Grouped slot sums (serving): member g owns blocks [start[g], start[g+1]), one per
+// (cell, species, z) of its own grid in the order of correct_sums_kernel's grid.
+__global__ __launch_bounds__(UB) void correct_sums_group_kernel(const CorrectArgs* table,
+                                                                const int32_t* start, int count) {
+    __shared__ double red[(UB / 64) * RED_VALUES];
+    const int g = group_member(start, count, blockIdx.x);
+    const CorrectArgs a = table[g];
+    const int local = blockIdx.x - start[g];
+    const int z = local % a.zb, s = (local / a.zb) % a.S, c = local / (a.zb * a.S);
+    correct_sums_body(a, c, s, z, red);
+}
+
+hipError_t launch_gain_correct_group(const CorrectArgs* table, const int32_t* start, int count,
+                                     int total_blocks, hipStream_t stream) {
+    if (count <= 0 || total_blocks <= 0) return hipSuccess;
+    correct_sums_group_kernel<<<total_blocks, UB, 0, stream>>>(table, start, count);
+    return hipGetLastError();
+}
+
+// Grouped coefficient launch: member g owns blocks [start[g], start[g+1]) of UB / 64 waves,
+// one wave per (cell, species) as in correct_coef_kernel (members without a coefficient array
+// own no blocks).
+__global__ __launch_bounds__(UB) void correct_coef_group_kernel(const UpdateArgs* table,
+                                                                const int32_t* start, int count) {
+    const int g = group_member(start, count, blockIdx.x);
+    const UpdateArgs& a = table[g];
+    const int64_t p = (int64_t)(blockIdx.x - start[g]) * (UB / 64) + (threadIdx.x >> 6);
+    if (!a.coef || p >= (int64_t)a.C * a.S) return;  // the whole wave
+    correction_coef(a, (int)(p / a.S), (int)(p % a.S), a.coef + p * 2);
+}
+
+hipError_t launch_coef_group(const UpdateArgs* table, const int32_t* start, int count,
+                             int total_blocks, hipStream_t stream) {
+    if (count <= 0 || total_blocks <= 0) return hipSuccess;
+    correct_coef_group_kernel<<<total_blocks, UB, 0, stream>>>(table, start, count);
     return hipGetLastError();
 }
 

@@ -959,6 +959,27 @@ static int timed_end(dtx_handle* h, TimedEvent* ev) {
     return DTX_OK;
 }
 
+// Arguments of the moment-rate correction's slot-sum launch (correct_active handles).
+static CorrectArgs make_correct_args(const dtx_handle* h) {
+    CorrectArgs ca{};
+    ca.part = h->d_part;
+    ca.part_stride = h->part_stride;
+    ca.gsum = h->d_gsum;
+    ca.lsum = h->d_lsum;
+    ca.cpart = h->d_cpart;
+    ca.zb = h->zb;
+    ca.tps = h->tps;
+    ca.species_off = h->d_species_off;
+    ca.gslots = h->d_gslots;
+    ca.nslots = h->d_nslots_s;
+    ca.gain_f32 = h->wide && h->precision == DTX_PREC_BF16X3;
+    ca.eweight = h->d_eweight;
+    ca.N = h->N;
+    ca.C = h->C;
+    ca.S = h->S;
+    return ca;
+}
+
 // Enqueue one functor call with an already clipped dt.
 static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save) {
     const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
@@ -970,25 +991,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
         if (h->wide)
             HIP_CHECK(h, launch_binary_wide(h->precision, h->wide, h->d_items, h->nitems, h->stream));
         else HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
-        if (h->correct_active) {
-            CorrectArgs ca{};
-            ca.part = h->d_part;
-            ca.part_stride = h->part_stride;
-            ca.gsum = h->d_gsum;
-            ca.lsum = h->d_lsum;
-            ca.cpart = h->d_cpart;
-            ca.zb = h->zb;
-            ca.tps = h->tps;
-            ca.species_off = h->d_species_off;
-            ca.gslots = h->d_gslots;
-            ca.nslots = h->d_nslots_s;
-            ca.gain_f32 = h->wide && h->precision == DTX_PREC_BF16X3;
-            ca.eweight = h->d_eweight;
-            ca.N = h->N;
-            ca.C = h->C;
-            ca.S = h->S;
-            HIP_CHECK(h, launch_gain_correct(ca, h->stream));
-        }
+        if (h->correct_active) HIP_CHECK(h, launch_gain_correct(make_correct_args(h), h->stream));
         rc = timed_end(h, &ev);
         if (rc) return rc;
     }
@@ -1072,42 +1075,54 @@ static int set_ell(dtx_handle* h, EllMatrix& E, int64_t n, const int64_t* colptr
 
 // ------------------------------------------------------------------- grouped stepping -------
 // Several simulations advanced together: one contraction launch per (kernel precision, cell-tile
-// width) over the members' concatenated work tables, one update launch over a table of the
-// members' update arguments.  Each member's work items and reduction order are the ones it uses
-// alone, so its results do not depend on the group.
+// width) over the members' concatenated work tables, one slot-sum and one coefficient launch
+// for the members with the FP64 moment-rate correction, and one update launch over a table of
+// the members' update arguments.  Each member's work items, reduction order and correction
+// arithmetic are the ones it uses alone, so its results do not depend on the group.
+//
+// Membership changes never synchronise the device: the combined tables live in grow-only
+// device buffers and are rewritten by copies on the group stream, which run after every launch
+// already enqueued there that reads the previous tables.  The host stages each copy in a
+// pinned buffer of a small ring whose reuse waits on the event recorded after its copy.
 
 namespace dtx {
 
 constexpr int GROUP_RING = 8;
 
+struct StagedUpload {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool used = false;
+};
+
 struct Group {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<dtx_handle*> members;
-    // narrow items of both shapes (list 0; list 1 stays empty), wide items of 2 / 4 k tiles
+    int precision = 0;
+    // static tables of the membership, one device buffer:
+    // narrow items (list 0), wide items of 2 / 4 k tiles (lists 2, 3), update block starts,
+    // slot-sum argument table and block starts, coefficient block starts
+    void* d_static = nullptr;
+    size_t static_cap = 0;
+    std::vector<void*> retired;        // replaced device buffers, freed with the group
     BinWork* d_items[4] = {nullptr, nullptr, nullptr, nullptr};
     int n_items[4] = {0, 0, 0, 0};
-    int precision = 0;
     int32_t* d_block_start = nullptr;
     int total_blocks = 0, max_N = 0;
+    CorrectArgs* d_ctable = nullptr;
+    int32_t* d_cstart = nullptr;
+    int n_corr = 0, corr_blocks = 0;
+    int32_t* d_coef_start = nullptr;
+    int coef_blocks = 0;
+    // per-step update argument table
     UpdateArgs* d_table = nullptr;
-    int table_cap = 0;
-    UpdateArgs* h_stage[GROUP_RING] = {};
-    hipEvent_t ev[GROUP_RING] = {};
-    bool ev_used[GROUP_RING] = {};
+    size_t table_cap = 0;
+    StagedUpload ring[GROUP_RING];
     int ring_pos = 0;
     std::string err;
 };
-
-static void group_free_tables(Group* g) {
-    for (int q = 0; q < 4; ++q) {
-        if (g->d_items[q]) (void)hipFree(g->d_items[q]);
-        g->d_items[q] = nullptr;
-        g->n_items[q] = 0;
-    }
-    if (g->d_block_start) (void)hipFree(g->d_block_start);
-    g->d_block_start = nullptr;
-}
 
 Group* group_create(int device) {
     Group* g = new Group();
@@ -1117,24 +1132,32 @@ Group* group_create(int device) {
         delete g;
         return nullptr;
     }
-    for (int q = 0; q < GROUP_RING; ++q) (void)hipEventCreateWithFlags(&g->ev[q], hipEventDisableTiming);
+    for (int q = 0; q < GROUP_RING; ++q)
+        (void)hipEventCreateWithFlags(&g->ring[q].ev, hipEventDisableTiming);
     return g;
 }
 
 void group_destroy(Group* g) {
     if (!g) return;
     (void)hipStreamSynchronize(g->stream);
-    group_free_tables(g);
+    if (g->d_static) (void)hipFree(g->d_static);
     if (g->d_table) (void)hipFree(g->d_table);
+    for (void* p : g->retired) (void)hipFree(p);
     for (int q = 0; q < GROUP_RING; ++q) {
-        if (g->h_stage[q]) (void)hipHostFree(g->h_stage[q]);
-        (void)hipEventDestroy(g->ev[q]);
+        if (g->ring[q].host) (void)hipHostFree(g->ring[q].host);
+        (void)hipEventDestroy(g->ring[q].ev);
     }
     (void)hipStreamDestroy(g->stream);
     delete g;
 }
 
 const char* group_error(const Group* g) { return g->err.c_str(); }
+
+const dtx_config* handle_config(const dtx_handle* h) { return &h->cfg; }
+
+bool handle_has_vectors(const dtx_handle* h) { return h->vectors_set; }
+
+hipStream_t group_stream(const Group* g) { return g->stream; }
 
 int group_sync(Group* g) {
     hipError_t e = hipStreamSynchronize(g->stream);
@@ -1145,16 +1168,75 @@ int group_sync(Group* g) {
     return DTX_OK;
 }
 
-// (Re)build the combined work tables for a member set.  Members must be idle.
-int group_set(Group* g, dtx_handle* const* hs, int count) {
-    int rc = group_sync(g);
+// A device buffer of at least `bytes`, grown geometrically; the buffer it replaces may still be
+// read by launches on the group stream, so it is kept until the group is destroyed.
+static int grow_device(Group* g, void** buf, size_t* cap, size_t bytes) {
+    if (bytes <= *cap && *buf) return DTX_OK;
+    const size_t want = std::max(bytes, 2 * *cap) + 4096;
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        g->err = "group table allocation failed";
+        return DTX_ERR_OOM;
+    }
+    if (*buf) g->retired.push_back(*buf);
+    *buf = p;
+    *cap = want;
+    return DTX_OK;
+}
+
+// Copy `bytes` from host memory to dst on the group stream through the next pinned staging
+// buffer of the ring (waiting for that buffer's previous copy to have run).
+static int staged_upload(Group* g, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return DTX_OK;
+    StagedUpload& u = g->ring[g->ring_pos++ % GROUP_RING];
+    if (u.used && hipEventSynchronize(u.ev) != hipSuccess) {
+        g->err = "staging event failed";
+        return DTX_ERR_HIP;
+    }
+    if (bytes > u.cap) {
+        if (u.host) (void)hipHostFree(u.host);
+        u.host = nullptr;
+        u.cap = 0;
+        const size_t want = std::max(bytes, (size_t)1 << 16);
+        if (hipHostMalloc(&u.host, want, hipHostMallocDefault) != hipSuccess) {
+            g->err = "pinned staging allocation failed";
+            return DTX_ERR_OOM;
+        }
+        u.cap = want;
+    }
+    memcpy(u.host, src, bytes);
+    hipError_t e = hipMemcpyAsync(dst, u.host, bytes, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) e = hipEventRecord(u.ev, g->stream);
+    u.used = true;
+    if (e != hipSuccess) {
+        g->err = std::string("group table upload: ") + hipGetErrorString(e);
+        return DTX_ERR_HIP;
+    }
+    return DTX_OK;
+}
+
+// Make a handle ready for grouped steps from any thread: operators packed and work tables
+// built, nothing pending on its own stream.
+int group_prepare(dtx_handle* h) {
+    if (h->method != DTX_METHOD_FORWARD_EULER)
+        return fail(h, DTX_ERR_UNSUPPORTED, "grouped stepping supports ForwardEuler only");
+    int rc = ensure_ready(h);
     if (rc) return rc;
-    group_free_tables(g);
+    HIP_CHECK(h, stream_sync(h));
+    return DTX_OK;
+}
+
+// Set the members (each prepared by group_prepare, or already a member).  Enqueues the new
+// tables on the group stream; does not wait for the device.
+int group_set(Group* g, dtx_handle* const* hs, int count) {
     g->members.assign(hs, hs + count);
     std::vector<BinWork> lists[4];  // cell tiles of 16, of 64, wide items of 2 and 4 k tiles
-    std::vector<int32_t> starts;
+    std::vector<int32_t> starts, cstart, coef_start;
+    std::vector<CorrectArgs> ctable;
     g->total_blocks = 0;
     g->max_N = 0;
+    g->corr_blocks = 0;
+    g->coef_blocks = 0;
     for (int m = 0; m < count; ++m) {
         dtx_handle* h = hs[m];
         if (h->method != DTX_METHOD_FORWARD_EULER) {
@@ -1166,102 +1248,92 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
             g->err = "grouped members must share one contraction precision";
             return DTX_ERR_UNSUPPORTED;
         }
-        if (h->moment_correction && h->precision != DTX_PREC_FP64) {
-            g->err = "grouped members run without the moment-rate correction";
-            return DTX_ERR_UNSUPPORTED;
-        }
-        rc = ensure_ready(h);
-        if (rc) {
-            g->err = h->err;
-            return rc;
-        }
-        if (stream_sync(h) != hipSuccess) {
-            g->err = "member stream failed";
-            return DTX_ERR_HIP;
+        if (!h->vectors_set || (h->binary && h->binary_dirty)) {
+            g->err = "grouped member is not prepared";
+            return DTX_ERR_STATE;
         }
         const int q = h->wide ? 1 + h->wide / 2 : (h->tc <= 16 ? 0 : 1);
         if (h->binary) lists[q].insert(lists[q].end(), h->items_host.begin(), h->items_host.end());
         starts.push_back(g->total_blocks);
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
         g->max_N = std::max(g->max_N, h->N);
+        if (h->binary && h->nitems > 0 && h->correct_active) {
+            ctable.push_back(make_correct_args(h));
+            cstart.push_back(g->corr_blocks);
+            g->corr_blocks += h->C * h->S * h->zb;
+        }
+        coef_start.push_back(g->coef_blocks);
+        if (h->binary && h->nitems > 0 && h->correct_active && h->d_coef)
+            g->coef_blocks += (int)(((int64_t)h->C * h->S + UPDATE_THREADS / 64 - 1) /
+                                    (UPDATE_THREADS / 64));
     }
+    g->n_corr = (int)ctable.size();
     // The narrow shapes go into one launch (list 0): GEMV items of single-cell requests and
     // 16-cell items stream their operator (HBM-bound) while 64-cell items are MFMA-bound, so the
-    // two lists are interleaved in proportion to run side by side.  Each member's items keep their order.
+    // two lists are interleaved in proportion to run side by side.  Each member's items keep
+    // their order.
     {
         std::vector<BinWork> mixed;
         mixed.reserve(lists[0].size() + lists[1].size());
         const size_t n0 = lists[0].size(), n1 = lists[1].size();
         size_t i0 = 0, i1 = 0;
         while (i0 < n0 || i1 < n1) {
-            // take from the list that is behind its share
             if (i1 >= n1 || (i0 < n0 && i0 * n1 <= i1 * n0)) mixed.push_back(lists[0][i0++]);
             else mixed.push_back(lists[1][i1++]);
         }
         lists[0].swap(mixed);
         lists[1].clear();
     }
+    // one host image of the static tables, 256-byte aligned parts
+    std::vector<uint8_t> img;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t off = (img.size() + 255) / 256 * 256;
+        img.resize(off + bytes);
+        if (bytes) memcpy(img.data() + off, src, bytes);
+        return off;
+    };
+    size_t off_items[4], off_start, off_ctable, off_cstart, off_coef;
     for (int q = 0; q < 4; ++q) {
         g->n_items[q] = (int)lists[q].size();
-        if (!g->n_items[q]) continue;
-        if (hipMalloc((void**)&g->d_items[q], lists[q].size() * sizeof(BinWork)) != hipSuccess ||
-            hipMemcpy(g->d_items[q], lists[q].data(), lists[q].size() * sizeof(BinWork),
-                      hipMemcpyHostToDevice) != hipSuccess) {
-            g->err = "work table upload failed";
-            return DTX_ERR_OOM;
-        }
+        off_items[q] = put(lists[q].data(), lists[q].size() * sizeof(BinWork));
     }
-    if (count > 0) {
-        if (hipMalloc((void**)&g->d_block_start, starts.size() * 4) != hipSuccess ||
-            hipMemcpy(g->d_block_start, starts.data(), starts.size() * 4,
-                      hipMemcpyHostToDevice) != hipSuccess) {
-            g->err = "block table upload failed";
-            return DTX_ERR_OOM;
-        }
-    }
-    if (count > g->table_cap) {
-        if (g->d_table) (void)hipFree(g->d_table);
-        for (int q = 0; q < GROUP_RING; ++q) {
-            if (g->h_stage[q]) (void)hipHostFree(g->h_stage[q]);
-            g->h_stage[q] = nullptr;
-            g->ev_used[q] = false;
-        }
-        g->table_cap = std::max(count, 2 * g->table_cap);
-        if (hipMalloc((void**)&g->d_table, (size_t)g->table_cap * sizeof(UpdateArgs)) !=
-            hipSuccess) {
-            g->err = "update table allocation failed";
-            return DTX_ERR_OOM;
-        }
-        for (int q = 0; q < GROUP_RING; ++q)
-            if (hipHostMalloc((void**)&g->h_stage[q], (size_t)g->table_cap * sizeof(UpdateArgs),
-                              hipHostMallocDefault) != hipSuccess) {
-                g->err = "pinned staging allocation failed";
-                return DTX_ERR_OOM;
-            }
-    }
-    return DTX_OK;
+    off_start = put(starts.data(), starts.size() * 4);
+    off_ctable = put(ctable.data(), ctable.size() * sizeof(CorrectArgs));
+    off_cstart = put(cstart.data(), cstart.size() * 4);
+    off_coef = put(coef_start.data(), coef_start.size() * 4);
+    int rc = grow_device(g, &g->d_static, &g->static_cap, img.size());
+    if (rc) return rc;
+    uint8_t* base = (uint8_t*)g->d_static;
+    for (int q = 0; q < 4; ++q) g->d_items[q] = (BinWork*)(base + off_items[q]);
+    g->d_block_start = (int32_t*)(base + off_start);
+    g->d_ctable = (CorrectArgs*)(base + off_ctable);
+    g->d_cstart = (int32_t*)(base + off_cstart);
+    g->d_coef_start = (int32_t*)(base + off_coef);
+    rc = staged_upload(g, g->d_static, img.data(), img.size());
+    if (rc) return rc;
+    return grow_device(g, (void**)&g->d_table, &g->table_cap,
+                       (size_t)std::max(count, 1) * sizeof(UpdateArgs));
 }
 
 // Enqueue one ForwardEuler step (dt[m], no save clipping) of every member.
 int group_step(Group* g, const double* dt, int verbose) {
     const int count = (int)g->members.size();
     if (count == 0) return DTX_OK;
-    const int slot = g->ring_pos++ % GROUP_RING;
-    if (g->ev_used[slot] && hipEventSynchronize(g->ev[slot]) != hipSuccess) {
-        g->err = "staging event failed";
-        return DTX_ERR_HIP;
-    }
+    std::vector<UpdateArgs> table((size_t)count);
     for (int m = 0; m < count; ++m) {
         dtx_handle* h = g->members[m];
-        g->h_stage[slot][m] = make_args(h, dt[m] / h->cfg.dt0, verbose, 0.0, dt[m], 0, DTX_FLUX_F);
+        table[(size_t)m] = make_args(h, dt[m] / h->cfg.dt0, verbose, 0.0, dt[m], 0, DTX_FLUX_F);
     }
-    hipError_t e = hipMemcpyAsync(g->d_table, g->h_stage[slot], (size_t)count * sizeof(UpdateArgs),
-                                  hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess) e = hipEventRecord(g->ev[slot], g->stream);
-    g->ev_used[slot] = true;
-    if (e == hipSuccess) e = launch_binary_mixed(g->precision, g->d_items[0], g->n_items[0], g->stream);
+    int rc = staged_upload(g, g->d_table, table.data(), table.size() * sizeof(UpdateArgs));
+    if (rc) return rc;
+    hipError_t e = launch_binary_mixed(g->precision, g->d_items[0], g->n_items[0], g->stream);
     for (int q = 2; q < 4 && e == hipSuccess; ++q)
         e = launch_binary_wide(g->precision, 2 * (q - 1), g->d_items[q], g->n_items[q], g->stream);
+    if (e == hipSuccess)
+        e = launch_gain_correct_group(g->d_ctable, g->d_cstart, g->n_corr, g->corr_blocks,
+                                      g->stream);
+    if (e == hipSuccess)
+        e = launch_coef_group(g->d_table, g->d_coef_start, count, g->coef_blocks, g->stream);
     if (e == hipSuccess)
         e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
                                       g->max_N, g->stream);

@@ -1,18 +1,25 @@
 // server.hip -- request loop with continuous batching (dtx_server_* of include/dtx.h).
 //
-// A scheduler thread owns the device work.  Requests wait in a FIFO; the head is admitted when
-// a slot is free and its HBM footprint fits the budget next to the running requests.  All
-// running requests advance together, one ForwardEuler step each per iteration, through the
-// grouped contraction and update launches (runtime.hip); a request that reaches its step count
-// is retired at once (final f and moments copied out, memory released) and the next waiting
-// requests take its place in the following iteration.  Every request keeps its own work
-// decomposition, so its result is bit-identical whether it runs alone or in a mix.
+// Three threads share the work of a server:
+//  - the builder takes requests FIFO: the head is admitted when a slot is free and its HBM
+//    footprint fits the budget next to the requests already admitted, then its simulation is
+//    built (synthetic requests) or prepared (caller-built handles) on its own stream;
+//  - the stepping thread owns the device work: all admitted, built requests advance together,
+//    one ForwardEuler step each per iteration, through the grouped contraction, moment-rate
+//    correction and update launches (runtime.hip).  A request that reaches its step count
+//    leaves the group at once; the event recorded after its last step is handed on, and the
+//    next built requests take its place in the following iteration.  This thread never waits
+//    for the device: membership changes enqueue their tables on the group stream;
+//  - the reaper waits for a finished request's event, copies its final f and moments out and
+//    releases its memory (hipFree synchronises the device, so it stays off the stepping thread).
+// Every request keeps its own work decomposition and arithmetic, so its result is bit-identical
+// alone, in any mix, and stepped through dtx_step on the handle dtx_request_create builds.
 //
-// Requests are synthetic workloads built from (shape, seed): species 0 electron, 1 photon,
-// 2.. positron; log p grid on [1e-3, 1e3], uniform u, phi on [0, 2 pi]; Maxwell-Juttner f0 at
-// theta with 1% seeded noise; lognormal gain tensors generated on the device; x advection plus
-// synchrotron cooling when with_flux and x_num > 1, built on the device; dt = 0.1 / (fastest
-// expected loss rate).
+// Synthetic requests are built from (shape, seed): species 0 electron, 1 photon, 2.. positron;
+// log p grid on [1e-3, 1e3], uniform u, phi on [0, 2 pi]; Maxwell-Juttner f0 at theta with 1%
+// seeded noise; lognormal gain tensors generated on the device; x advection plus synchrotron
+// cooling when with_flux and x_num > 1, built on the device; dt = 0.1 / (fastest expected loss
+// rate), rounded down to 12 significant bits.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -142,10 +149,13 @@ struct Active {
     int64_t steps = 0, done = 0;
     double dt = 0.0;
     int64_t bytes = 0;
+    hipEvent_t finished = nullptr;  // recorded on the group stream after the last step
 };
 
 struct Record {
     dtx_request req;
+    dtx_handle* handle = nullptr;   // caller-built simulation (dtx_server_submit_handle)
+    double dt = 0.0;                // its step
     dtx_request_result res;
     std::vector<double> f;
 };
@@ -156,15 +166,17 @@ struct dtx_server {
     dtx_server_config cfg;
     Group* group = nullptr;
     std::mutex mu;
-    std::condition_variable cv;
-    std::deque<int64_t> pending;
+    std::condition_variable cv;     // any state change (submit, built, retired, stop)
+    std::deque<int64_t> pending;    // submitted, not admitted
+    std::deque<Active> built;       // admitted and built, not yet stepping
+    std::deque<Active> retiring;    // finished, waiting for the reaper
     std::map<int64_t, Record> records;
-    std::vector<Active> active;
     int64_t next_id = 0;
-    int64_t used_bytes = 0;
+    int64_t used_bytes = 0;         // admitted requests (building, stepping, retiring)
+    int n_admitted = 0;             // admitted and not yet retired
     bool stop = false;
     int64_t unfinished = 0;
-    std::thread worker;
+    std::thread builder, stepper, reaper;
     std::string err;
 };
 
@@ -197,27 +209,33 @@ static int64_t estimate_bytes(const dtx_config& c, int precision) {
         }
     int64_t loss = 0;
     for (int s = 0; s < S; ++s) loss += div_up(ns[s], TK) * (Np / KC) * KC * TK * 8;
+    // moment-rate matrices of the reduced-precision modes: two FP64 N1 x N2 per gain channel
+    int64_t moment = 0;
+    if (precision != DTX_PREC_FP64)
+        for (int a = 0; a < S; ++a)
+            for (int b = a; b < S; ++b) moment += 2 * 2 * div_up(ns[a], TK) * TK * div_up(ns[b], KC) * KC * 8;
     const int64_t part = 160 * C * N * 8;
-    const int64_t state = C * N * 8 * 8 + Np * div_up(C, 16) * 16 * 12;
-    return gain + loss + part + state + ((int64_t)N * N * 8);
+    const int64_t state = C * N * 8 * 12 + Np * div_up(C, 16) * 16 * 12;
+    return gain + loss + moment + part + state + ((int64_t)N * N * 8);
 }
 
-static int build_request(dtx_server* sv, const dtx_request& rq, dtx_handle** out, double* dt_out) {
+// The simulation of a synthetic request as a handle ready for grouped steps, and its dt.
+static int build_request(const dtx_server_config& sc, const dtx_request& rq, dtx_handle** out,
+                         double* dt_out, std::string* err) {
     dtx_config c = rq.cfg;
-    c.precision = sv->cfg.precision;
+    c.precision = sc.precision;
     c.method = DTX_METHOD_FORWARD_EULER;
-    c.device = sv->cfg.device;
+    c.device = sc.device;
     if (!(c.dt0 > 0.0)) c.dt0 = 1.0;
     dtx_handle* h = nullptr;
     int rc = dtx_create(&c, &h);
     if (rc) {
-        sv->err = dtx_last_error(nullptr);
+        *err = dtx_last_error(nullptr);
         return rc;
     }
-    // grouped steps add the contraction's slots directly (no per-request correction launch)
-    rc = dtx_set_moment_correction(h, 0);
+    rc = dtx_set_moment_correction(h, sc.no_moment_correction ? 0 : 1);
     if (rc) {
-        sv->err = dtx_last_error(h);
+        *err = dtx_last_error(h);
         dtx_destroy(h);
         return rc;
     }
@@ -266,7 +284,12 @@ static int build_request(dtx_server* sv, const dtx_request& rq, dtx_handle** out
             worst = std::max(worst, rate * vol[(size_t)cc] / ap[(size_t)cc * N]);
         }
     }
-    const double dt = 0.1 / worst;
+    double dt = 0.1 / worst;
+    {  // 12 significant bits: k dt is exact, as for the benchmark workloads
+        int e2 = 0;
+        const double m2 = std::frexp(dt, &e2);
+        dt = std::ldexp(std::floor(m2 * 4096.0) / 4096.0, e2);
+    }
     {
         const bool flux = rq.with_flux && c.x_num > 1;
         if (flux) {
@@ -297,132 +320,233 @@ static int build_request(dtx_server* sv, const dtx_request& rq, dtx_handle** out
         rc = dtx_set_moment_weights(h, w.data());
         if (rc) goto fail;
     }
+    rc = group_prepare(h);
+    if (rc) goto fail;
     *out = h;
     *dt_out = dt;
     return DTX_OK;
 fail:
-    sv->err = dtx_last_error(h);
+    *err = dtx_last_error(h);
     dtx_destroy(h);
     return rc;
 }
 
-static void retire(dtx_server* sv, Active& a, int status) {
-    Record rec;
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        rec = sv->records[a.id];
-    }
-    Record& r = rec;
-    r.res.steps_done = a.done;
-    if (status == 2 && a.h) {
-        const int64_t n = (int64_t)r.req.cfg.x_num * r.req.cfg.y_num * r.req.cfg.z_num;
-        int64_t N = 0;
-        for (int s = 0; s < r.req.cfg.num_species; ++s)
-            N += (int64_t)r.req.cfg.px_num[s] * r.req.cfg.py_num[s] * r.req.cfg.pz_num[s];
-        r.f.assign((size_t)(n * N), 0.0);
-        std::vector<double> num((size_t)(n * r.req.cfg.num_species)), en(num.size());
-        if (dtx_get_f(a.h, r.f.data()) || dtx_moments(a.h, num.data(), en.data())) status = -3;
-        double tn = 0, te = 0;
-        for (size_t i = 0; i < num.size(); ++i) {
-            tn += num[i];
-            te += en[i];
+// Builder thread: FIFO admission by slot and HBM budget, then build / prepare.
+static void builder(dtx_server* sv) {
+    (void)hipSetDevice(sv->cfg.device);
+    while (true) {
+        int64_t id = -1;
+        Record rec;
+        int64_t est = 0;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            while (true) {
+                if (sv->stop) return;
+                if (!sv->pending.empty() && sv->n_admitted < sv->cfg.max_active) {
+                    const Record& r = sv->records[sv->pending.front()];
+                    est = r.handle ? dtx_device_bytes(r.handle)
+                                   : estimate_bytes(r.req.cfg, sv->cfg.precision);
+                    // the head waits for the budget unless nothing else holds memory
+                    if (sv->used_bytes + est <= sv->cfg.hbm_budget || sv->n_admitted == 0) break;
+                }
+                sv->cv.wait(lk);
+            }
+            id = sv->pending.front();
+            sv->pending.pop_front();
+            sv->used_bytes += est;
+            sv->n_admitted++;
+            Record& r = sv->records[id];
+            r.res.status = 1;
+            r.res.start_ms = now_ms();
+            rec.req = r.req;
+            rec.handle = r.handle;
+            rec.dt = r.dt;
         }
-        r.res.number_total = tn;
-        r.res.energy_total = te;
+        Active a;
+        a.id = id;
+        a.steps = rec.req.steps;
+        a.bytes = est;
+        std::string err;
+        int rc;
+        if (rec.handle) {
+            a.h = rec.handle;
+            a.dt = rec.dt;
+            rc = group_prepare(a.h);
+            if (rc) err = dtx_last_error(a.h);
+        } else {
+            rc = build_request(sv->cfg, rec.req, &a.h, &a.dt, &err);
+            if (rc) a.h = nullptr;
+        }
+        if (!rc && hipEventCreateWithFlags(&a.finished, hipEventDisableTiming) != hipSuccess) {
+            rc = DTX_ERR_HIP;
+            err = "event creation failed";
+        }
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (rc) {
+            sv->err = err;
+            if (a.h) dtx_destroy(a.h);
+            Record& r = sv->records[id];
+            r.handle = nullptr;
+            r.res.status = -1;
+            r.res.finish_ms = now_ms();
+            sv->used_bytes -= est;
+            sv->n_admitted--;
+            sv->unfinished--;
+        } else {
+            sv->built.push_back(a);
+        }
+        sv->cv.notify_all();
     }
-    if (a.h) dtx_destroy(a.h);
-    r.res.status = status;
-    r.res.finish_ms = now_ms();
-    std::lock_guard<std::mutex> lk(sv->mu);
-    sv->records[a.id] = std::move(r);
-    sv->used_bytes -= a.bytes;
-    sv->unfinished--;
-    sv->cv.notify_all();
 }
 
-static void scheduler(dtx_server* sv) {
+// Reaper thread: final f and moments of finished requests, then their memory.
+static void reaper(dtx_server* sv) {
     (void)hipSetDevice(sv->cfg.device);
+    while (true) {
+        Active a;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv.wait(lk, [&] { return sv->stop || !sv->retiring.empty(); });
+            if (sv->retiring.empty()) return;  // stop with nothing left
+            a = sv->retiring.front();
+            sv->retiring.pop_front();
+        }
+        int status = a.done >= a.steps ? 2 : -2;
+        std::vector<double> f;
+        double tn = 0, te = 0;
+        if (a.h && hipEventSynchronize(a.finished) != hipSuccess) status = -3;
+        if (status == 2) {
+            const dtx_config& c = *handle_config(a.h);
+            int64_t N = 0;
+            for (int q = 0; q < c.num_species; ++q)
+                N += (int64_t)c.px_num[q] * c.py_num[q] * c.pz_num[q];
+            const int64_t C = (int64_t)c.x_num * c.y_num * c.z_num;
+            const int64_t n = N * C, cs = C * c.num_species;
+            f.assign((size_t)n, 0.0);
+            std::vector<double> num((size_t)cs), en((size_t)cs);
+            if (dtx_get_f(a.h, f.data()) || dtx_moments(a.h, num.data(), en.data())) status = -3;
+            for (size_t i = 0; i < num.size(); ++i) {
+                tn += num[i];
+                te += en[i];
+            }
+        }
+        if (a.finished) (void)hipEventDestroy(a.finished);
+        if (a.h) dtx_destroy(a.h);
+        std::lock_guard<std::mutex> lk(sv->mu);
+        Record& r = sv->records[a.id];
+        r.handle = nullptr;
+        r.f = std::move(f);
+        r.res.steps_done = a.done;
+        r.res.number_total = tn;
+        r.res.energy_total = te;
+        r.res.status = status;
+        r.res.finish_ms = now_ms();
+        sv->used_bytes -= a.bytes;
+        sv->n_admitted--;
+        sv->unfinished--;
+        sv->cv.notify_all();
+    }
+}
+
+// Stepping thread: grouped steps of every built request; never waits for the device.
+static void stepper(dtx_server* sv) {
+    (void)hipSetDevice(sv->cfg.device);
+    std::vector<Active> active;
     std::vector<double> dts;
     std::vector<dtx_handle*> hs;
-    bool changed = true;
-    while (true) {
-        // admission: FIFO head, while a slot and the HBM budget allow
-        while (true) {
-            int64_t id = -1;
-            dtx_request rq;
-            {
-                std::unique_lock<std::mutex> lk(sv->mu);
-                if (sv->stop) return;
-                if (sv->pending.empty() || (int)sv->active.size() >= sv->cfg.max_active) {
-                    if (sv->active.empty())
-                        sv->cv.wait_for(lk, std::chrono::milliseconds(2));
-                    break;
-                }
-                id = sv->pending.front();
-                rq = sv->records[id].req;
-                const int64_t est = estimate_bytes(rq.cfg, sv->cfg.precision);
-                if (sv->used_bytes + est > sv->cfg.hbm_budget && !sv->active.empty()) break;
-                sv->pending.pop_front();
-                sv->used_bytes += est;
-                sv->records[id].res.status = 1;
-                sv->records[id].res.start_ms = now_ms();
-            }
-            Active a;
-            a.id = id;
-            a.steps = rq.steps;
-            a.bytes = estimate_bytes(rq.cfg, sv->cfg.precision);
-            if (build_request(sv, rq, &a.h, &a.dt) != DTX_OK) {
-                a.h = nullptr;
-                retire(sv, a, -1);
-                continue;
-            }
-            sv->active.push_back(a);
-            changed = true;
+    bool changed = false;
+    auto retire_all = [&](int status_steps_ok) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (auto& a : active) {
+            if (!status_steps_ok) a.done = -1;
+            (void)hipEventRecord(a.finished, group_stream(sv->group));
+            sv->retiring.push_back(a);
         }
-        if (sv->active.empty()) continue;
+        active.clear();
+        sv->cv.notify_all();
+    };
+    while (true) {
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            if (active.empty())
+                sv->cv.wait(lk, [&] { return sv->stop || !sv->built.empty(); });
+            if (sv->stop) break;
+            while (!sv->built.empty()) {
+                active.push_back(sv->built.front());
+                sv->built.pop_front();
+                changed = true;
+            }
+        }
         if (changed) {
             hs.clear();
             dts.clear();
-            for (auto& a : sv->active) {
+            for (auto& a : active) {
                 hs.push_back(a.h);
                 dts.push_back(a.dt);
             }
             if (group_set(sv->group, hs.data(), (int)hs.size()) != DTX_OK) {
-                sv->err = group_error(sv->group);
-                for (auto& a : sv->active) retire(sv, a, -2);
-                sv->active.clear();
+                {
+                    std::lock_guard<std::mutex> lk(sv->mu);
+                    sv->err = group_error(sv->group);
+                }
+                retire_all(0);
                 continue;
             }
             changed = false;
         }
         if (group_step(sv->group, dts.data(), 0) != DTX_OK) {
-            sv->err = group_error(sv->group);
-            for (auto& a : sv->active) retire(sv, a, -2);
-            sv->active.clear();
-            changed = true;
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                sv->err = group_error(sv->group);
+            }
+            retire_all(0);
             continue;
         }
         bool finished = false;
-        for (auto& a : sv->active) finished |= (++a.done >= a.steps);
+        for (auto& a : active) finished |= (++a.done >= a.steps);
         if (finished) {
-            (void)group_sync(sv->group);
             std::vector<Active> keep;
-            for (auto& a : sv->active) {
-                if (a.done >= a.steps) retire(sv, a, 2);
-                else keep.push_back(a);
+            std::lock_guard<std::mutex> lk(sv->mu);
+            for (auto& a : active) {
+                if (a.done >= a.steps) {
+                    (void)hipEventRecord(a.finished, group_stream(sv->group));
+                    sv->retiring.push_back(a);
+                } else {
+                    keep.push_back(a);
+                }
             }
-            sv->active.swap(keep);
+            active.swap(keep);
             changed = true;
+            sv->cv.notify_all();
         }
     }
+    // stopped: what is still stepping goes to the reaper as failed
+    for (auto& a : active) a.done = -1;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (auto& a : active) {
+        (void)hipEventRecord(a.finished, group_stream(sv->group));
+        sv->retiring.push_back(a);
+    }
+    sv->cv.notify_all();
 }
 
 extern "C" {
+
+int dtx_request_create(const dtx_request* req, const dtx_server_config* cfg, dtx_handle** out,
+                       double* dt) {
+    if (!req || !cfg || !out || !dt) return DTX_ERR_INVALID;
+    *out = nullptr;
+    if (req->cfg.num_species < 1 || req->cfg.num_species > DTX_MAX_SPECIES)
+        return DTX_ERR_INVALID;
+    std::string err;
+    return build_request(*cfg, *req, out, dt, &err);
+}
 
 int dtx_server_create(const dtx_server_config* cfg, dtx_server** out) {
     if (!cfg || !out) return DTX_ERR_INVALID;
     *out = nullptr;
     if (cfg->max_active < 1 || cfg->hbm_budget <= 0) return DTX_ERR_INVALID;
+    if (cfg->precision < DTX_PREC_FP64 || cfg->precision > DTX_PREC_BF16X3) return DTX_ERR_INVALID;
     dtx_server* sv = new dtx_server();
     sv->cfg = *cfg;
     sv->group = group_create(cfg->device);
@@ -430,28 +554,52 @@ int dtx_server_create(const dtx_server_config* cfg, dtx_server** out) {
         delete sv;
         return DTX_ERR_HIP;
     }
-    sv->worker = std::thread(scheduler, sv);
+    sv->builder = std::thread(builder, sv);
+    sv->stepper = std::thread(stepper, sv);
+    sv->reaper = std::thread(reaper, sv);
     *out = sv;
     return DTX_OK;
+}
+
+static int64_t enqueue_record(dtx_server* sv, Record&& r) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    const int64_t rid = sv->next_id++;
+    memset(&r.res, 0, sizeof r.res);
+    r.res.id = rid;
+    r.res.status = 0;
+    r.res.submit_ms = now_ms();
+    sv->records[rid] = std::move(r);
+    sv->pending.push_back(rid);
+    sv->unfinished++;
+    sv->cv.notify_all();
+    return rid;
 }
 
 int dtx_server_submit(dtx_server* sv, const dtx_request* req, int64_t* id) {
     if (!sv || !req || !id) return DTX_ERR_INVALID;
     if (req->cfg.num_species < 1 || req->cfg.num_species > DTX_MAX_SPECIES || req->steps < 1)
         return DTX_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(sv->mu);
-    const int64_t rid = sv->next_id++;
     Record r;
     r.req = *req;
-    memset(&r.res, 0, sizeof r.res);
-    r.res.id = rid;
-    r.res.status = 0;
-    r.res.submit_ms = now_ms();
-    sv->records[rid] = r;
-    sv->pending.push_back(rid);
-    sv->unfinished++;
-    sv->cv.notify_all();
-    *id = rid;
+    *id = enqueue_record(sv, std::move(r));
+    return DTX_OK;
+}
+
+int dtx_server_submit_handle(dtx_server* sv, dtx_handle* h, int64_t steps, double dt,
+                             int64_t* id) {
+    if (!sv || !h || !id || steps < 1 || !(dt > 0.0) || !std::isfinite(dt))
+        return DTX_ERR_INVALID;
+    const dtx_config& hc = *handle_config(h);
+    if (hc.precision != sv->cfg.precision || hc.method != DTX_METHOD_FORWARD_EULER ||
+        !handle_has_vectors(h))
+        return DTX_ERR_INVALID;
+    Record r;
+    memset(&r.req, 0, sizeof r.req);
+    r.req.cfg = hc;
+    r.req.steps = steps;
+    r.handle = h;
+    r.dt = dt;
+    *id = enqueue_record(sv, std::move(r));
     return DTX_OK;
 }
 
@@ -488,10 +636,18 @@ void dtx_server_destroy(dtx_server* sv) {
         sv->stop = true;
         sv->cv.notify_all();
     }
-    if (sv->worker.joinable()) sv->worker.join();
+    if (sv->builder.joinable()) sv->builder.join();
+    if (sv->stepper.joinable()) sv->stepper.join();
+    if (sv->reaper.joinable()) sv->reaper.join();
     (void)group_sync(sv->group);
-    for (auto& a : sv->active)
-        if (a.h) dtx_destroy(a.h);
+    // whatever the builder left built or the reaper left waiting
+    for (auto* q : {&sv->built, &sv->retiring})
+        for (auto& a : *q) {
+            if (a.finished) (void)hipEventDestroy(a.finished);
+            if (a.h) dtx_destroy(a.h);
+        }
+    for (auto& kv : sv->records)
+        if (kv.second.handle && kv.second.res.status == 0) dtx_destroy(kv.second.handle);
     group_destroy(sv->group);
     delete sv;
 }

@@ -56,7 +56,8 @@ class KernelTimes(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("device", I32), ("precision", I32), ("hbm_budget", I64), ("max_active", I32)]
+    _fields_ = [("device", I32), ("precision", I32), ("hbm_budget", I64), ("max_active", I32),
+                ("no_moment_correction", I32)]
 
 
 class Request(C.Structure):
@@ -106,11 +107,14 @@ SIGNATURES = [
     ("dtx_kernel_timing", I32, [C.c_void_p, I32, I32, C.POINTER(KernelTimes)]),
     ("dtx_server_create", I32, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     ("dtx_server_submit", I32, [C.c_void_p, C.POINTER(Request), PI64]),
+    ("dtx_server_submit_handle", I32, [C.c_void_p, C.c_void_p, I64, D, PI64]),
     ("dtx_server_poll", I32, [C.c_void_p, I64, C.POINTER(RequestResult)]),
     ("dtx_server_fetch", I32, [C.c_void_p, I64, PD, I64]),
     ("dtx_server_drain", I32, [C.c_void_p]),
     ("dtx_server_destroy", None, [C.c_void_p]),
     ("dtx_server_last_error", C.c_char_p, [C.c_void_p]),
+    ("dtx_request_create", I32, [C.POINTER(Request), C.POINTER(ServerConfig),
+                                 C.POINTER(C.c_void_p), PD]),
 ]
 
 _lib = None

@@ -285,6 +285,12 @@ class _SteppingStruct:
                                       _lib.dptr(tout), C.byref(steps)), self._h, "dtx_solve")
         return OutputStruct([fout[i] for i in range(t_save.size)], tout), steps.value
 
+    def release(self):
+        """Give up ownership of the library handle (e.g. to dtx_server_submit_handle); this
+        object is closed afterwards."""
+        h, self._h = self._h, C.c_void_p()
+        return h
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             self.lib.dtx_destroy(self._h)

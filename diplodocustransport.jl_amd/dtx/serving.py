@@ -76,15 +76,72 @@ def request_mix(count: int = 256, rate_per_s: float = 1.0, seed: int | None = No
     return out
 
 
+def server_config(precision="fp32", hbm_budget_gb: float = 200.0, max_active: int = 256,
+                  device: int = 0, moment_correction: bool = True) -> _lib.ServerConfig:
+    cfg = _lib.ServerConfig()
+    cfg.device = device
+    cfg.precision = _lib.PRECISIONS[precision]
+    cfg.hbm_budget = int(hbm_budget_gb * 1e9)
+    cfg.max_active = max_active
+    cfg.no_moment_correction = 0 if moment_correction else 1
+    return cfg
+
+
+class RequestHandle:
+    """The simulation the server builds for a synthetic request (dtx_request_create), as a
+    stand-alone handle: step it with `step` (dtx_step, t_stop never reached) to get the served
+    result; or give it to Server.submit_handle."""
+
+    def __init__(self, spec: "RequestSpec", precision="fp32", moment_correction=True, device=0):
+        self.lib = _lib.load()
+        self.spec = spec
+        cfg = server_config(precision, device=device, moment_correction=moment_correction)
+        req = spec.to_c()
+        self._h = C.c_void_p()
+        dt = C.c_double()
+        rc = self.lib.dtx_request_create(C.byref(req), C.byref(cfg), C.byref(self._h),
+                                         C.byref(dt))
+        if rc != 0:
+            raise _lib.DtxError(f"dtx_request_create failed ({rc}): "
+                                f"{self.lib.dtx_last_error(None).decode()}")
+        self.dt = dt.value
+
+    def step(self, steps: int):
+        out_dt, save = C.c_double(), C.c_int32()
+        t = 0.0
+        for _ in range(steps):
+            _lib.check(self.lib.dtx_step(self._h, t, 1e300, self.dt, 0, C.byref(out_dt),
+                                         C.byref(save), None), self._h, "dtx_step")
+            t += self.dt
+
+    @property
+    def f(self) -> np.ndarray:
+        out = np.empty(self.spec.n)
+        _lib.check(self.lib.dtx_get_f(self._h, _lib.dptr(out)), self._h, "dtx_get_f")
+        return out
+
+    def release(self):
+        """Give up ownership (after Server.submit_handle)."""
+        h, self._h = self._h, C.c_void_p()
+        return h
+
+    def close(self):
+        if self._h and self._h.value:
+            self.lib.dtx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     def __init__(self, precision="fp32", hbm_budget_gb: float = 200.0, max_active: int = 256,
-                 device: int = 0):
+                 device: int = 0, moment_correction: bool = True):
         self.lib = _lib.load()
-        cfg = _lib.ServerConfig()
-        cfg.device = device
-        cfg.precision = _lib.PRECISIONS[precision]
-        cfg.hbm_budget = int(hbm_budget_gb * 1e9)
-        cfg.max_active = max_active
+        cfg = server_config(precision, hbm_budget_gb, max_active, device, moment_correction)
         self._s = C.c_void_p()
         rc = self.lib.dtx_server_create(C.byref(cfg), C.byref(self._s))
         if rc != 0:
@@ -96,6 +153,16 @@ class Server:
         rc = self.lib.dtx_server_submit(self._s, C.byref(req), C.byref(rid))
         if rc != 0:
             raise _lib.DtxError(f"dtx_server_submit failed ({rc})")
+        return rid.value
+
+    def submit_handle(self, handle, steps: int, dt: float) -> int:
+        """A caller-built ForwardEuler handle (the server takes ownership): a RequestHandle or
+        a ForwardEulerStruct built with this server's precision."""
+        h = handle.release() if hasattr(handle, "release") else handle
+        rid = C.c_int64()
+        rc = self.lib.dtx_server_submit_handle(self._s, h, int(steps), float(dt), C.byref(rid))
+        if rc != 0:
+            raise _lib.DtxError(f"dtx_server_submit_handle failed ({rc})")
         return rid.value
 
     def poll(self, rid: int) -> _lib.RequestResult:
@@ -133,10 +200,10 @@ class Server:
 
 
 def run(specs, precision="fp32", hbm_budget_gb=200.0, max_active=256, realtime=True,
-        fetch=False):
+        fetch=False, moment_correction=True):
     """Submit the requests at their arrival times (or all at once) and wait for all of them.
     Returns (per-request results, wall seconds, final states if fetched)."""
-    srv = Server(precision, hbm_budget_gb, max_active)
+    srv = Server(precision, hbm_budget_gb, max_active, moment_correction=moment_correction)
     t0 = time.perf_counter()
     ids = []
     for s in specs:

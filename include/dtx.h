@@ -254,7 +254,9 @@ typedef struct {
     int32_t device;
     int32_t precision;       /* enum dtx_precision used for every request            */
     int64_t hbm_budget;      /* bytes of device memory the admitted requests may hold  */
-    int32_t max_active;      /* request slots stepped together                        */
+    int32_t max_active;      /* requests admitted at once (building, stepping, retiring) */
+    int32_t no_moment_correction;  /* 1: FP32 / BF16x3 requests without the FP64 moment-rate
+                                      correction (dtx_set_moment_correction); 0: with it    */
 } dtx_server_config;
 
 typedef struct {
@@ -276,6 +278,14 @@ typedef struct {
 int dtx_server_create(const dtx_server_config* cfg, dtx_server** out);
 /* Enqueue a request; returns its id through *id.  Thread-safe. */
 int dtx_server_submit(dtx_server* s, const dtx_request* req, int64_t* id);
+/* Enqueue a simulation the caller built (dtx_create, operators and dtx_set_vectors, in the
+ * server's precision, ForwardEuler) to be stepped `steps` times with step dt (no clipping) in
+ * the grouped launches; the server takes ownership of the handle and destroys it when the
+ * request finishes (its final f and moments stay available through poll / fetch).  The caller
+ * must not use the handle after this call.  Thread-safe.  No counterpart in the reference:
+ * this is how arriving user simulations, with their own operators and f0, join the loop. */
+int dtx_server_submit_handle(dtx_server* s, dtx_handle* h, int64_t steps, double dt,
+                             int64_t* id);
 /* Result of a request (status 2 when done).  Thread-safe. */
 int dtx_server_poll(dtx_server* s, int64_t id, dtx_request_result* out);
 /* Final f of a finished request (n doubles). */
@@ -284,6 +294,13 @@ int dtx_server_fetch(dtx_server* s, int64_t id, double* f, int64_t n);
 int dtx_server_drain(dtx_server* s);
 void dtx_server_destroy(dtx_server* s);
 const char* dtx_server_last_error(const dtx_server* s);
+
+/* The simulation a server builds for a synthetic request, as a stand-alone handle with the
+ * server config's device, precision and moment-rate correction, and its step *dt: stepped by
+ * dtx_step with that dt (and a t_stop it never reaches) it gives the served result bit for
+ * bit.  No counterpart in the reference. */
+int dtx_request_create(const dtx_request* req, const dtx_server_config* cfg, dtx_handle** out,
+                       double* dt);
 
 #ifdef __cplusplus
 }
