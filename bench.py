@@ -7,11 +7,13 @@ unknowns), upwind x advection + synchrotron cooling, per-cell binary collisions 
 gain tensors, plain FP64.  A step is one ForwardEuler functor call over the whole state; inputs
 are resident in HBM before the timed region.
 
-Prints ONE JSON line: steps/s of the FP64 path (value), with the FP32-MFMA and split-BF16 paths
-and their relative error against FP64 beside it, the live roofline of the dominant kernel (the
-binary contraction, HIP events on the library's stream around the kernels of every 8th timed
-step: events around every step would add about 14 us to each) and the FP64 CPU oracle timed on
-a bounded sample on the host cores.
+Prints ONE JSON line: steps/s of the FP64 path (value) over a timed window with no event
+records in it; the kernel durations of the roofline come from HIP events the library records on
+its own stream around every step's kernels in a separate, untimed pass.  Beside it: the FP32-MFMA
+and split-BF16 paths (speed, and their relative error and number / energy drift against FP64
+over the configuration's full step count), the same comparison at the largest configuration
+(transport_2d, where the split-BF16 contraction is MFMA-bound), the serving loop, and the FP64
+CPU oracle timed on a bounded sample on the host cores.
 """
 import argparse
 import json
@@ -26,7 +28,7 @@ sys.path.insert(0, str(ROOT / "diplodocustransport.jl_amd"))
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 PEAK_HBM_GBS = 8000.0
-TIME_EVERY = 8   # kernel durations from HIP events on every 8th timed step
+TIMING_STEPS = 40   # untimed steps with HIP events around every step's kernels
 
 DESCRIPTIONS = {
     "transport_1d": "2 species x 256 states (32 p x 8 u x 1 phi), 128 periodic x cells, 65536 "
@@ -62,10 +64,36 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True):
+def t_save_marks(w, marks):
+    """Save times after the given step counts (exact multiples of the workload's dt)."""
+    out, t, k = [0.0], 0.0, 0
+    for m in marks[1:]:
+        while k < m:
+            t += w.dt
+            k += 1
+        out.append(t)
+    return out
+
+
+def species_totals(w, f, weights=None):
+    import numpy as np
+    N, C = w.ps.n_momentum, w.ps.n_space
+    fc = np.asarray(f).reshape(C, N)
+    if weights is not None:
+        fc = fc * weights[None, :]
+    return np.array([fc[:, o:o + s.size].sum() for o, s in zip(w.ps.species_offset,
+                                                              w.ps.species)])
+
+
+def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True, accuracy_steps=0,
+            device_flux=False):
+    """Timed window (no event records), then an untimed pass with HIP events around every
+    step's kernels (their average durations), then, when accuracy_steps > 0, a Solve of that
+    many steps from f0 whose final state the accuracy comparison uses."""
     from dtx import workloads
     m = workloads.make_struct(w, precision=precision, use_monolithic=False,
-                              moment_correction=moment_correction)
+                              moment_correction=moment_correction,
+                              device_flux=device_flux and w.flux_spec is not None)
     work = m.work()
     m.reset()
     t, big = 0.0, 1e300
@@ -73,25 +101,69 @@ def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True):
         dt, _ = m(t, big, w.dt, verbose)
         t += dt
     m.synchronize()
-    m.kernel_timing(enable=True, reset=True, every=TIME_EVERY if steps >= 10 * TIME_EVERY else 1)
     t0 = time.perf_counter()
     for _ in range(steps):
         dt, _ = m(t, big, w.dt, verbose)
         t += dt
     m.synchronize()
     t1 = time.perf_counter()
+    m.kernel_timing(enable=True, reset=True, every=1)
+    for _ in range(TIMING_STEPS):
+        dt, _ = m(t, big, w.dt, verbose)
+        t += dt
     kt = m.kernel_timing(enable=False, reset=True)
-    f_end = m.f
-    num, en = m.moments()
     res = dict(ms_per_step=(t1 - t0) * 1e3 / steps, steps_per_s=steps / (t1 - t0),
                bin_ms=kt.bin_ms / max(1, kt.bin_launches),
                update_ms=kt.update_ms / max(1, kt.update_launches),
                bin_flops=work.bin_flops, gain_bytes=work.gain_bytes, bin_bytes=work.bin_bytes,
                update_bytes=work.update_bytes, items=work.bin_launch_items,
-               device_gb=m.device_bytes() / 1e9, f=f_end, number=float(num.sum()),
-               energy=float(en.sum()))
+               device_gb=m.device_bytes() / 1e9)
+    if accuracy_steps:
+        out, n = m.solve_native(w.dt, t_save_marks(w, [0, accuracy_steps]), Verbose=0)
+        assert n == accuracy_steps
+        f = out.f[-1]
+        res.update(f=f, number=species_totals(w, f),
+                   energy=species_totals(w, f, w.ps.energy_weights()))
     m.close()
     return res
+
+
+def accuracy(r, base):
+    """Relative error in f (states above 1e-8 of the largest) and the largest relative number
+    and energy drift per species, against the FP64 run of the same steps."""
+    import numpy as np
+    fp = base["f"]
+    mask = np.abs(fp) > 1e-8 * np.max(np.abs(fp))
+    return dict(rel_linf_f=float(np.max(np.abs(r["f"][mask] - fp[mask]) / np.abs(fp[mask]))),
+                number_drift=float(np.max(np.abs(r["number"] - base["number"]) /
+                                          np.abs(base["number"]))),
+                energy_drift=float(np.max(np.abs(r["energy"] - base["energy"]) /
+                                          np.abs(base["energy"]))))
+
+
+def run_largest(steps, warmup):
+    """transport_2d (1,048,576 unknowns, 500 steps): FP64 against FP32 and split-BF16, speed over
+    a timed window and accuracy over the full 500 steps (flux operators built on the device,
+    which the tests show equal to the host CSC)."""
+    from dtx import workloads
+    w = workloads.transport_2d()
+    out = {"workload": f"transport_2d: {DESCRIPTIONS['transport_2d']}",
+           "accuracy_steps": w.steps, "timed_steps": steps}
+    res = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        log(f"[bench] transport_2d {prec}: {warmup} warmup + {steps} timed steps, "
+            f"{w.steps}-step accuracy run")
+        res[prec] = run_gpu(w, prec, steps, warmup, accuracy_steps=w.steps, device_flux=True)
+        log(f"[bench] transport_2d {prec}: {res[prec]['steps_per_s']:.1f} steps/s")
+    base = res["fp64"]
+    out["fp64_steps_per_s"] = round(base["steps_per_s"], 2)
+    out["fp64_binary_kernel_us"] = round(base["bin_ms"] * 1e3, 1)
+    for prec in ("fp32", "bf16x3"):
+        r = res[prec]
+        out[prec] = dict(steps_per_s=round(r["steps_per_s"], 2),
+                         speedup_vs_fp64=round(r["steps_per_s"] / base["steps_per_s"], 3),
+                         binary_kernel_us=round(r["bin_ms"] * 1e3, 1), **accuracy(r, base))
+    return out
 
 
 def fp64_kernels_per_step(w):
@@ -186,34 +258,67 @@ def run_cpu_optimized(w, sample_steps):
                        f"on {threads} threads of {cpu_model()}")
 
 
-def run_serving(count, serial_sample):
+def request_work(spec):
+    """Math FLOPs and packed FP32 gain-operator bytes of one grouped step of a request: per
+    interaction a + b -> a + b (a <= b) the gain channels of its output species, 2 No pairs C
+    FLOPs and No pairs 4 bytes each, and per species the FP64 loss GEMV, 2 Ns N C FLOPs."""
+    Ns = spec.p_num * spec.u_num * spec.h_num
+    S, C = spec.species, spec.x_num
+    N = S * Ns
+    flops = bytes_ = 0.0
+    for a in range(S):
+        for b in range(a, S):
+            pairs = Ns * (Ns + 1) // 2 if a == b else Ns * Ns
+            outs = 1 if a == b else 2
+            flops += outs * 2.0 * Ns * pairs * C
+            bytes_ += outs * 4.0 * Ns * pairs
+    flops += S * 2.0 * Ns * N * C
+    bytes_ += S * 8.0 * Ns * N
+    return flops, bytes_
+
+
+def run_serving(count):
     """Serving workload: `count` seeded requests (p in {16,32,64}, u in {4,8}, x in {1,32,128},
-    1-2 species, 200-2000 steps).  Capacity = solves/s with every request queued at t = 0; then
-    Poisson arrivals at 80% of that capacity give the time-to-solution percentiles.  The serial
-    plain-FP64 rate runs the first `serial_sample` requests one at a time."""
+    1-2 species, 200-2000 steps).  Capacity = solves/s with every request queued at t = 0 (FP32
+    grouped kernels with the FP64 moment-rate correction); then Poisson arrivals at 80% of that
+    capacity give the time-to-solution percentiles; then every request one at a time in plain
+    FP64 (a server with one slot), whose results give the served FP32 number / energy drift."""
+    import numpy as np
     from dtx import serving
     specs = serving.request_mix(count=count, rate_per_s=1e9)
     log(f"[bench] serving capacity: {count} requests queued at t=0 (FP32 grouped kernels)")
-    res, wall, _ = serving.run(specs, precision="fp32", realtime=False)
+    res_cap, wall, _ = serving.run(specs, precision="fp32", realtime=False)
     capacity = count / wall
+    flops = sum(request_work(sp)[0] * sp.steps for sp in specs)
+    gbytes = sum(request_work(sp)[1] * sp.steps for sp in specs)
     rate = 0.8 * capacity
-    specs = serving.request_mix(count=count, rate_per_s=rate)
+    specs_p = serving.request_mix(count=count, rate_per_s=rate)
     log(f"[bench] serving: Poisson arrivals at {rate:.2f}/s")
-    res, wall, _ = serving.run(specs, precision="fp32", realtime=True)
+    res, wall_p, _ = serving.run(specs_p, precision="fp32", realtime=True)
     lat = serving.latency_stats(res)
-    serial = None
-    if serial_sample:
-        log(f"[bench] serial plain FP64: first {serial_sample} requests one at a time")
-        t0 = time.perf_counter()
-        for sp in specs[:serial_sample]:
-            serving.run([sp], precision="fp64", realtime=False)
-        serial = serial_sample / (time.perf_counter() - t0)
-    return dict(requests=count, precision="fp32", hbm_budget_gb=200,
+    log(f"[bench] serial plain FP64: all {count} requests one at a time")
+    res64, wall64, _ = serving.run(specs, precision="fp64", realtime=False, max_active=1)
+    serial = count / wall64
+    dn = max(abs(a.number_total - b.number_total) / abs(b.number_total)
+             for a, b in zip(res_cap, res64))
+    de = max(abs(a.energy_total - b.energy_total) / abs(b.energy_total)
+             for a, b in zip(res_cap, res64))
+    tf = flops / wall / 1e12
+    gbs = gbytes / wall / 1e9
+    return dict(requests=count, precision="fp32", moment_correction=True, hbm_budget_gb=200,
                 capacity_solves_per_s=round(capacity, 3), arrival_rate_per_s=round(rate, 3),
-                solves_per_s=round(count / wall, 3), p50_time_to_solution_s=round(lat["p50_s"], 3),
+                solves_per_s=round(count / wall_p, 3),
+                p50_time_to_solution_s=round(lat["p50_s"], 3),
                 p99_time_to_solution_s=round(lat["p99_s"], 3),
-                serial_fp64_solves_per_s=round(serial, 3) if serial else None,
-                serial_fp64_sample=serial_sample)
+                serial_fp64_solves_per_s=round(serial, 3), serial_fp64_requests=count,
+                capacity_vs_serial_fp64=round(capacity / serial, 2),
+                number_drift_vs_fp64=dn, energy_drift_vs_fp64=de,
+                roofline=dict(achieved_tflops=round(tf, 2),
+                              mfma_frac_fp32=round(tf / PEAK_TFLOPS["fp32"], 4),
+                              operator_gbs=round(gbs, 1),
+                              hbm_frac=round(gbs / PEAK_HBM_GBS, 4),
+                              note="math FLOPs and packed FP32 operator bytes of every grouped "
+                                   "step of the capacity run over its wall time"))
 
 
 def run_flux_setup(name):
@@ -245,9 +350,25 @@ def run_flux_setup(name):
         times = {"host_assembly_s": round(t1 - t0, 4), "host_csc_to_ell_upload_s":
                  round(t2 - t1, 4), "device_synth_s": round(t3 - t2, 5)}
     host = times["host_assembly_s"] + times["host_csc_to_ell_upload_s"]
+    # the two operators at this size: the same device footprint (ELL width) and the same state
+    # after a step from f0 (the handle's last operator is the device-built one)
+    ref = workloads.make_struct(w)
+    bytes_dev, bytes_host = m.device_bytes(), ref.device_bytes()
+    m.reset()
+    m(0.0, 1e300, w.dt, 0)
+    ref(0.0, 1e300, w.dt, 0)
+    fd, fh = m.f, ref.f
+    rel = float(np.max(np.abs(fd - fh)) / np.max(np.abs(fh)))
+    ref.close()
+    m.close()
     print(json.dumps({"flux_setup": dict(workload=name, n=n, nnz=int(F[0][-1] - 1), **times,
-                                         speedup=round(host / times["device_synth_s"], 1))}))
+                                         speedup=round(host / times["device_synth_s"], 1),
+                                         device_bytes_equal=bytes_dev == bytes_host,
+                                         step_rel_linf=rel,
+                                         step_bitwise=bool(np.array_equal(fd, fh)))}))
     del keep
+    if bytes_dev != bytes_host or rel > 1e-13:
+        sys.exit("flux set-up: the device-built operator differs from the host CSC")
 
 
 def main():
@@ -269,7 +390,12 @@ def main():
                     help="skip the FP32 / BF16x3 runs without the FP64 moment-rate correction")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=256)
-    ap.add_argument("--serving-serial-sample", type=int, default=16)
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the full-length accuracy runs of the precision modes")
+    ap.add_argument("--no-largest", action="store_true",
+                    help="skip the transport_2d FP64 / FP32 / split-BF16 comparison")
+    ap.add_argument("--largest-steps", type=int, default=50,
+                    help="timed steps per precision of the transport_2d comparison")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (5 times), the "
                          "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -309,30 +435,30 @@ def main():
         runs.append((prec, True))
         if prec != "fp64" and not args.no_uncorrected:
             runs.append((prec, False))
+    acc_steps = 0 if args.no_accuracy else w.steps
     for prec, corr in runs:
         key = prec if corr or prec == "fp64" else prec + "_uncorrected"
-        log(f"[bench] {key}: {args.warmup} warmup + {steps} timed steps")
-        res[key] = run_gpu(w, prec, steps, args.warmup, moment_correction=corr)
+        log(f"[bench] {key}: {args.warmup} warmup + {steps} timed steps, {TIMING_STEPS} event-"
+            f"timed steps, {acc_steps}-step accuracy run")
+        res[key] = run_gpu(w, prec, steps, args.warmup, moment_correction=corr,
+                           accuracy_steps=acc_steps)
         r = res[key]
         log(f"[bench] {key}: {r['steps_per_s']:.1f} steps/s, binary kernel "
             f"{r['bin_ms']*1e3:.1f} us, update kernel {r['update_ms']*1e3:.1f} us, "
             f"{r['items']} work items")
     base = res["fp64"]
-    fp = base["f"]
     prec_report = {}
     for key, r in res.items():
         if key == "fp64":
             continue
-        mask = np.abs(fp) > 1e-8 * np.max(np.abs(fp))
         prec_report[key] = dict(
             steps_per_s=round(r["steps_per_s"], 2),
             speedup_vs_fp64=round(r["steps_per_s"] / base["steps_per_s"], 3),
-            rel_linf_f=float(np.max(np.abs(r["f"][mask] - fp[mask]) / np.abs(fp[mask]))),
-            number_drift=abs(r["number"] - base["number"]) / abs(base["number"]),
-            energy_drift=abs(r["energy"] - base["energy"]) / abs(base["energy"]),
             binary_kernel_us=round(r["bin_ms"] * 1e3, 2),
             binary_tflops=round(r["bin_flops"] / (r["bin_ms"] * 1e-3) / 1e12, 2),
             binary_hbm_gbs=round(r["bin_bytes"] / (r["bin_ms"] * 1e-3) / 1e9, 1))
+        if acc_steps:
+            prec_report[key].update(accuracy_steps=acc_steps, **accuracy(r, base))
     # The contraction is MFMA-bound when its arithmetic intensity exceeds the FP64 ridge
     # (78.6 TFLOP/s / 8 TB/s), else HBM-bound (single-cell boxes stream the whole operator).
     ai = base["bin_flops"] / base["bin_bytes"]
@@ -353,9 +479,12 @@ def main():
                     traffic=TRAFFIC.get((w.name, kernel)), kernel=kernel,
                     flops_per_launch=base["bin_flops"], bytes_per_launch=base["bin_bytes"],
                     kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
+    largest = None
+    if not args.no_largest and args.workload == "transport_1d":
+        largest = run_largest(args.largest_steps, 5)
     serving_line = None
     if not args.no_serving and args.workload == "transport_1d":
-        serving_line = run_serving(args.serving_requests, args.serving_serial_sample)
+        serving_line = run_serving(args.serving_requests)
     cpu = cpu_opt = None
     if not args.no_cpu and args.workload in ("transport_1d", "homogeneous_box"):
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle steps")
@@ -382,6 +511,7 @@ def main():
         "precision_modes": prec_report,
         "update_kernel_us": round(base["update_ms"] * 1e3, 2),
         "roofline": roof,
+        "largest_solve": largest,
         "serving": serving_line,
         "cpu_baseline": cpu,
         "cpu_optimized": cpu_opt,

@@ -39,10 +39,13 @@ def _compare(host, dev, dt, steps):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("case", ["x5_euler", "x2_euler", "x5_symplectic", "xy_euler"])
+@pytest.mark.parametrize("case", ["x5_euler", "x2_euler", "x5_symplectic", "xy_euler",
+                                  "transport_1d_full"])
 def test_device_flux_matches_host_csc(case):
     method = "symplectic" if "symplectic" in case else "euler"
-    if case.startswith("xy"):
+    if case == "transport_1d_full":  # 65,536 rows: the full transport_1d operator
+        w = workloads.transport_1d(steps=6)
+    elif case.startswith("xy"):
         w = workloads.transport_2d(p_num=4, u_num=2, h_num=2, x_num=4, y_num=3, steps=6)
     else:
         w = workloads.tiny(x_num=5 if case.startswith("x5") else 2, steps=6, monolithic=False)
