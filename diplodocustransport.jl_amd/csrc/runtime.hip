@@ -89,6 +89,12 @@ struct dtx_handle {
     double* d_loss = nullptr;
     bool loss_present = false;
     BinWork* d_items = nullptr;     // loss and gain items of one launch
+    // the same items with their partial slots and moment-rate sums in the device's `scratch`
+    // array (dtx_step_noalloc), when they fit it
+    BinWork* d_items_scratch = nullptr;
+    double* scratch = nullptr;      // this device's address of `scratch` (null: unavailable)
+    int64_t scratch_need = 0;       // bytes of scratch the no-alloc step needs
+    std::vector<double> ring_host;  // host copy of the status ring (read without allocating)
     std::vector<BinWork> items_host;  // host copy (grouped launches)
     int32_t nitems = 0, tc = 16, ldc = 0, max_slots = 0, part_slots = 0;
     double* d_part = nullptr;
@@ -142,6 +148,36 @@ struct dtx_handle {
     std::vector<TimedEvent> events;
     dtx_kernel_times times{};
 };
+
+// ---------------------------------------------------------------- no-alloc scratch -------
+// Device scratch of dtx_step_noalloc: the contraction's partial slots, then the per-cell sums
+// of its moment-rate items.  One array per device, held until the process ends and shared by
+// every no-alloc call on that device (calls that overlap on one device race on it).  Host code
+// must never use the name: each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current (dtx_create).
+constexpr size_t SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) double scratch[SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+// This device's address of `scratch` (the device must be current), or null.
+static double* scratch_address(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_addr[device] = (double*)ptr;
+    }
+    return g_scratch_addr[device];
+}
 
 // --------------------------------------------------------------------------- helpers ------
 
@@ -502,6 +538,7 @@ static int pack_loss_channels(dtx_handle* h) {
 // Build the grouped work tables for the handle's channels and cell count.
 static int build_work(dtx_handle* h) {
     dfree(h, &h->d_items, h->nitems);
+    dfree(h, &h->d_items_scratch, h->nitems);
     dfree(h, &h->d_part, h->part_stride * h->part_slots);
     dfree(h, &h->d_nslot, h->N);
     dfree(h, &h->d_nslot32, h->N);
@@ -683,6 +720,18 @@ static int build_work(dtx_handle* h) {
         if (!rc) rc = upload(h, h->d_items, items.data(), h->nitems);
         if (rc) return rc;
     }
+    // the no-alloc step's items: the same work with its outputs in `scratch`
+    h->scratch_need = (h->part_stride * h->part_slots + (int64_t)h->n_mpart * h->ldc) * 8;
+    if (h->nitems && h->scratch && (size_t)h->scratch_need <= SCRATCH_BYTES) {
+        double* mbase = h->scratch + h->part_stride * h->part_slots;
+        std::vector<BinWork> si = items;
+        for (BinWork& w : si)
+            w.part = w.kind == CH_MOMENT ? mbase + (w.part - h->d_mpart)
+                                         : h->scratch + (w.part - h->d_part);
+        int rc = dalloc(h, &h->d_items_scratch, h->nitems);
+        if (!rc) rc = upload(h, h->d_items_scratch, si.data(), h->nitems);
+        if (rc) return rc;
+    }
     std::vector<int32_t> nslot(h->N, 0);
     for (int s = 0; s < h->S; ++s)
         for (int k = 0; k < h->sp_size[s]; ++k) nslot[h->sp_off[s] + k] = next_slot[s];
@@ -842,7 +891,7 @@ static void print_status(const dtx_handle* h, const double* st, int verbose) {
 static int drain_status(dtx_handle* h) {
     const int64_t pending = h->step_counter - h->reported;
     if (pending <= 0) return DTX_OK;
-    std::vector<double> ring((size_t)STATUS_RING * STATUS_WORDS);
+    std::vector<double>& ring = h->ring_host;  // sized by dtx_create
     HIP_CHECK(h, hipMemcpy(ring.data(), h->d_status, ring.size() * sizeof(double),
                            hipMemcpyDeviceToHost));
     const int64_t first = std::max<int64_t>(h->reported, h->step_counter - STATUS_RING);
@@ -876,7 +925,7 @@ static void fill_stats(const double* st, dtx_step_stats* out) {
 constexpr double kMaxIntervalSteps = 1e8;
 
 static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start, double dt,
-                            int save, int which_ell) {
+                            int save, int which_ell, bool use_scratch = false) {
     UpdateArgs a{};
     a.f = h->d_f[h->cur];
     a.f_out = h->d_f[h->cur ^ 1];
@@ -907,6 +956,10 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
         a.nslot32 = h->d_nslot32;
     }
     a.part_stride = h->part_stride;
+    if (use_scratch) {  // the no-alloc step's slots and moment-rate sums
+        if (!h->correct_active) a.part = h->scratch;
+        else a.mpart = h->scratch + h->part_stride * h->part_slots;
+    }
     a.emi = h->d_emi;
     a.active = h->d_active;
     const EllMatrix& E = h->ell[which_ell];
@@ -980,25 +1033,33 @@ static CorrectArgs make_correct_args(const dtx_handle* h) {
     return ca;
 }
 
-// Enqueue one functor call with an already clipped dt.
-static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save) {
+// Enqueue one functor call with an already clipped dt.  use_scratch: the contraction's slots
+// and moment-rate sums in the device's `scratch` array, and no kernel timing (no events).
+static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save,
+                        bool use_scratch = false) {
     const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
     const bool bin = h->binary && h->nitems > 0;
+    const bool timing = h->timing && !use_scratch;
     TimedEvent ev{};
     if (bin) {
-        int rc = timed_begin(h, 0, &ev);
+        int rc = timing ? timed_begin(h, 0, &ev) : DTX_OK;
         if (rc) return rc;
+        const BinWork* items = use_scratch ? h->d_items_scratch : h->d_items;
         if (h->wide)
-            HIP_CHECK(h, launch_binary_wide(h->precision, h->wide, h->d_items, h->nitems, h->stream));
-        else HIP_CHECK(h, launch_binary(h->precision, h->tc, h->d_items, h->nitems, h->stream));
-        if (h->correct_active) HIP_CHECK(h, launch_gain_correct(make_correct_args(h), h->stream));
-        rc = timed_end(h, &ev);
+            HIP_CHECK(h, launch_binary_wide(h->precision, h->wide, items, h->nitems, h->stream));
+        else HIP_CHECK(h, launch_binary(h->precision, h->tc, items, h->nitems, h->stream));
+        if (h->correct_active) {
+            CorrectArgs ca = make_correct_args(h);
+            if (use_scratch) ca.part = h->scratch;
+            HIP_CHECK(h, launch_gain_correct(ca, h->stream));
+        }
+        rc = timing ? timed_end(h, &ev) : DTX_OK;
         if (rc) return rc;
     }
-    int rc = timed_begin(h, 1, &ev);
+    int rc = timing ? timed_begin(h, 1, &ev) : DTX_OK;
     if (rc) return rc;
     if (h->method == DTX_METHOD_FORWARD_EULER) {
-        UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_F);
+        UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_F, use_scratch);
         if (finalize_is_single(a)) {
             a.block_part = h->d_block_part + h->bp_parity * h->bp_words;
             a.mom_part = h->d_mom_part + h->bp_parity * h->mp_words;
@@ -1011,12 +1072,12 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
             HIP_CHECK(h, launch_euler_update(a, h->stream));
         }
     } else {
-        UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_P);
+        UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_P, use_scratch);
         HIP_CHECK(h, launch_symplectic_momentum(a, h->stream));
-        UpdateArgs b = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_X);
+        UpdateArgs b = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_X, use_scratch);
         HIP_CHECK(h, launch_symplectic_space(b, h->stream));
     }
-    rc = timed_end(h, &ev);
+    rc = timing ? timed_end(h, &ev) : DTX_OK;
     if (rc) return rc;
     if (h->step_verbose.size() != (size_t)STATUS_RING) h->step_verbose.assign(STATUS_RING, 0);
     h->step_verbose[(size_t)(h->step_counter % STATUS_RING)] = (int8_t)verbose;
@@ -1508,6 +1569,8 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
         return fail(nullptr, rc, "%s", msg.c_str());
     }
     h->step_verbose.assign(STATUS_RING, 0);
+    h->ring_host.assign((size_t)STATUS_RING * STATUS_WORDS, 0.0);
+    h->scratch = scratch_address(cfg->device);  // device current (hipSetDevice above)
     *out = h.release();
     return DTX_OK;
 }
@@ -1963,6 +2026,45 @@ int dtx_step(dtx_handle* h, double t_start, double t_stop, double dt, int32_t ve
     }
     const int64_t idx = h->step_counter;
     rc = enqueue_step(h, t_start, dt, verbose, save);
+    if (rc) return rc;
+    if (dt_out) *dt_out = dt;
+    if (save_out) *save_out = save;
+    if (stats) {
+        HIP_CHECK(h, stream_sync(h));
+        double st[STATUS_WORDS];
+        HIP_CHECK(h, hipMemcpy(st, h->d_status + (idx % STATUS_RING) * STATUS_WORDS,
+                               sizeof st, hipMemcpyDeviceToHost));
+        fill_stats(st, stats);
+        rc = drain_status(h);
+        if (rc) return rc;
+    }
+    return DTX_OK;
+}
+
+int dtx_step_noalloc(dtx_handle* h, double t_start, double t_stop, double dt, int32_t verbose,
+                     double* dt_out, int32_t* save_out, dtx_step_stats* stats) {
+    if (!h) return DTX_ERR_INVALID;
+    if (!h->vectors_set)
+        return fail(h, DTX_ERR_STATE, "dtx_set_vectors must be called before stepping");
+    if (h->binary && h->binary_dirty)
+        return fail(h, DTX_ERR_STATE, "dtx_step_noalloc: the operators are not packed yet (a "
+                    "dtx_step, dtx_solve or dtx_work call packs them)");
+    const bool bin = h->binary && h->nitems > 0;
+    if (bin && !h->scratch)
+        return fail(h, DTX_ERR_HIP, "dtx_step_noalloc: the device scratch array is unavailable");
+    if (bin && (size_t)h->scratch_need > SCRATCH_BYTES)
+        return fail(h, DTX_ERR_UNSUPPORTED, "dtx_step_noalloc: the step needs %lld bytes of "
+                    "scratch, more than the %zu available", (long long)h->scratch_need,
+                    SCRATCH_BYTES);
+    int save;
+    if (t_start + dt >= t_stop) {  // src/SteppingMethods.jl:18-23
+        dt = t_stop - t_start;
+        save = 1;
+    } else {
+        save = 0;
+    }
+    const int64_t idx = h->step_counter;
+    int rc = enqueue_step(h, t_start, dt, verbose, save, /*use_scratch=*/true);
     if (rc) return rc;
     if (dt_out) *dt_out = dt;
     if (save_out) *save_out = save;

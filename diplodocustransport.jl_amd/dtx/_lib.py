@@ -95,6 +95,7 @@ SIGNATURES = [
     ("dtx_synth_binary", I32, [C.c_void_p, U64, D]),
     ("dtx_synth_flux", I32, [C.c_void_p, C.POINTER(FluxSynth)]),
     ("dtx_step", I32, [C.c_void_p, D, D, D, I32, PD, PI32, C.POINTER(StepStats)]),
+    ("dtx_step_noalloc", I32, [C.c_void_p, D, D, D, I32, PD, PI32, C.POINTER(StepStats)]),
     ("dtx_solve", I32, [C.c_void_p, D, I64, PD, I32, PD, PD, PI64]),
     ("dtx_reset", I32, [C.c_void_p]),
     ("dtx_get_f", I32, [C.c_void_p, PD]),

@@ -201,6 +201,20 @@ int dtx_synth_flux(dtx_handle* h, const dtx_flux_synth* spec);
 int dtx_step(dtx_handle* h, double t_start, double t_stop, double dt, int32_t verbose,
              double* dt_out, int32_t* save_out, dtx_step_stats* stats);
 
+/* dtx_step without allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, semantics and results as dtx_step, and it calls no
+ * hipMalloc, hipFree or other allocating routine: the contraction's partial sums live in a
+ * 128 MiB array that the library holds on each device it runs on until the process ends.
+ * Limits: the handle's operators must already be packed (any dtx_step, dtx_solve or dtx_work
+ * call packs them; DTX_ERR_STATE otherwise); a step whose partial sums exceed 128 MiB is
+ * refused with DTX_ERR_UNSUPPORTED (C x N x 8 bytes per partial slot).  Every call on a device
+ * shares that device's array: no-alloc steps of different handles that overlap in time race,
+ * so order them on one stream or synchronise between them (steps of one handle are ordered on
+ * its stream).  Kernel timing (dtx_kernel_timing) does not cover these steps. */
+int dtx_step_noalloc(dtx_handle* h, double t_start, double t_stop, double dt, int32_t verbose,
+                     double* dt_out, int32_t* save_out, dtx_step_stats* stats);
+
 /* Solve(method, dt_initial, t_save; Verbose) (src/Solver.jl:1-64) without the JLD2 write:
  * resets f to f_init, steps through every save interval without host synchronisation and writes
  * the n_save states to f_out (n_save x n, row i = output.f[i+1]) and t_out (n_save).  The
