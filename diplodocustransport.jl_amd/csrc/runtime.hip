@@ -1090,6 +1090,14 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     return DTX_OK;
 }
 
+// Free an ELL matrix's arrays and mark it absent.
+static void release_ell(dtx_handle* h, EllMatrix& E) {
+    dfree(h, &E.col, E.n * E.width);
+    dfree(h, &E.val, E.n * E.width);
+    E.width = 0;
+    E.nnz = 0;
+}
+
 // CSC (1-based) -> ELL with slots sorted by column inside each row.
 static int set_ell(dtx_handle* h, EllMatrix& E, int64_t n, const int64_t* colptr,
                    const int64_t* rowval, const double* nzval) {
@@ -1109,11 +1117,7 @@ static int set_ell(dtx_handle* h, EllMatrix& E, int64_t n, const int64_t* colptr
         width = std::max<int64_t>(width, (int64_t)r.size());
         nnz += (int64_t)r.size();
     }
-    dfree(h, &E.col, E.n * E.width);
-    dfree(h, &E.val, E.n * E.width);
-    E.n = n;
-    E.width = (int32_t)width;
-    E.nnz = nnz;
+    release_ell(h, E);
     if (width == 0) return DTX_OK;
     std::vector<int32_t> col((size_t)(width * n), -1);
     std::vector<double> val((size_t)(width * n), 0.0);
@@ -1125,13 +1129,23 @@ static int set_ell(dtx_handle* h, EllMatrix& E, int64_t n, const int64_t* colptr
             val[(size_t)s * n + r] = v[s].second;
         }
     }
-    int rc = dalloc(h, &E.col, width * n);
-    if (rc) return rc;
-    rc = dalloc(h, &E.val, width * n);
-    if (rc) return rc;
-    rc = upload(h, E.col, col.data(), width * n);
-    if (rc) return rc;
-    return upload(h, E.val, val.data(), width * n);
+    int32_t* dcol = nullptr;
+    double* dval = nullptr;
+    int rc = dalloc(h, &dcol, width * n);
+    if (!rc) rc = dalloc(h, &dval, width * n);
+    if (!rc) rc = upload(h, dcol, col.data(), width * n);
+    if (!rc) rc = upload(h, dval, val.data(), width * n);
+    if (rc) {  // the matrix stays absent; nothing is left allocated
+        dfree(h, &dcol, width * n);
+        dfree(h, &dval, width * n);
+        return rc;
+    }
+    E.col = dcol;
+    E.val = dval;
+    E.n = n;
+    E.width = (int32_t)width;
+    E.nnz = nnz;
+    return DTX_OK;
 }
 
 // ------------------------------------------------------------------- grouped stepping -------
@@ -1666,10 +1680,15 @@ int dtx_synth_flux(dtx_handle* h, const dtx_flux_synth* sp) {
     a.axes = sp->axes;
     a.periodic = sp->periodic ? 1 : 0;
     a.stats = d_stats;
-    // the matrices the method reads; the first count pass also writes 1/Ap and Vol
+    // the matrices the method reads; the first count pass also writes 1/Ap and Vol.  Every exit
+    // below frees the temporaries, and a matrix is marked present only when both its arrays
+    // exist.
     const int which_list[2] = {DTX_FLUX_X, DTX_FLUX_P};
     const bool euler = cfg.method == DTX_METHOD_FORWARD_EULER;
     const int nwhich = euler ? 1 : 2;
+    auto hip_fail = [&](hipError_t e, const char* what) {
+        return fail(h, DTX_ERR_HIP, "dtx_synth_flux: %s: %s", what, hipGetErrorString(e));
+    };
     for (int q = 0; q < nwhich && !rc; ++q) {
         a.which = euler ? DTX_FLUX_F : which_list[q];
         a.width = 0;
@@ -1678,26 +1697,42 @@ int dtx_synth_flux(dtx_handle* h, const dtx_flux_synth* sp) {
         unsigned long long st[2] = {0, 0};
         rc = upload(h, d_stats, st, 2);
         if (rc) break;
-        HIP_CHECK(h, launch_synth_flux(a, h->stream));
-        HIP_CHECK(h, hipMemcpyAsync(st, d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
-        HIP_CHECK(h, stream_sync(h));
+        hipError_t e = launch_synth_flux(a, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(st, d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = stream_sync(h);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "count pass");
+            break;
+        }
         EllMatrix& E = h->ell[a.which];
-        dfree(h, &E.col, E.n * E.width);
-        dfree(h, &E.val, E.n * E.width);
+        release_ell(h, E);
+        const int32_t width = (int32_t)st[0];
+        if (width == 0) continue;
+        int32_t* dcol = nullptr;
+        double* dval = nullptr;
+        rc = dalloc(h, &dcol, (int64_t)width * h->n);
+        if (!rc) rc = dalloc(h, &dval, (int64_t)width * h->n);
+        if (!rc) {
+            a.width = width;
+            a.col = dcol;
+            a.val = dval;
+            a.inv_ap = nullptr;
+            a.vol = nullptr;
+            e = launch_synth_flux(a, h->stream);
+            if (e == hipSuccess) e = stream_sync(h);
+            if (e != hipSuccess) rc = hip_fail(e, "fill pass");
+        }
+        if (rc) {
+            dfree(h, &dcol, (int64_t)width * h->n);
+            dfree(h, &dval, (int64_t)width * h->n);
+            break;
+        }
+        E.col = dcol;
+        E.val = dval;
         E.n = h->n;
-        E.width = (int32_t)st[0];
+        E.width = width;
         E.nnz = (int64_t)st[1];
-        if (E.width == 0) continue;
-        rc = dalloc(h, &E.col, E.width * E.n);
-        if (!rc) rc = dalloc(h, &E.val, E.width * E.n);
-        if (rc) break;
-        a.width = E.width;
-        a.col = E.col;
-        a.val = E.val;
-        a.inv_ap = nullptr;
-        a.vol = nullptr;
-        HIP_CHECK(h, launch_synth_flux(a, h->stream));
-        HIP_CHECK(h, stream_sync(h));
     }
     dfree(h, &d_bounds, nb);
     dfree(h, &d_stats, 2);
@@ -1742,6 +1777,67 @@ int dtx_set_binary_dense(dtx_handle* h, int64_t N, const double* M) {
                     }
                 }
             }
+    h->binary = true;
+    h->binary_dirty = true;
+    return DTX_OK;
+}
+
+int dtx_set_binary_csc(dtx_handle* h, int64_t N, const int64_t* colptr, const int64_t* rowval,
+                       const double* nzval) {
+    if (!h) return DTX_ERR_INVALID;
+    if (N != h->N)
+        return fail(h, DTX_ERR_INVALID, "M_Bin is (%lld^2 x %lld), expected N = %d",
+                    (long long)N, (long long)N, h->N);
+    if (!colptr || (colptr[N] > 1 && (!rowval || !nzval)))
+        return fail(h, DTX_ERR_INVALID, "null CSC array");
+    const int64_t NN = N * N;
+    for (int64_t j = 0; j < N; ++j) {  // validate before touching the handle's operator
+        const int64_t k0 = colptr[j] - 1, k1 = colptr[j + 1] - 1;
+        if (k0 < 0 || k1 < k0)
+            return fail(h, DTX_ERR_INVALID, "colptr is not a valid 1-based CSC pointer");
+        for (int64_t q = k0; q < k1; ++q)
+            if (rowval[q] < 1 || rowval[q] > NN)
+                return fail(h, DTX_ERR_INVALID, "rowval out of range");
+    }
+    h->hch.clear();
+    h->synth_binary = false;
+    h->loss_host.assign((size_t)N * N, 0.0);
+    h->loss_nonzero = false;
+    // species of every state, and a pair's position in its channel
+    std::vector<int> sp((size_t)N);
+    for (int s = 0; s < h->S; ++s)
+        for (int k = h->sp_off[s]; k < h->sp_off[s + 1]; ++k) sp[(size_t)k] = s;
+    for (int so = 0; so < h->S; ++so)  // every channel exists, as the dense form builds them
+        for (int s1 = 0; s1 < h->S; ++s1)
+            for (int s2 = s1; s2 < h->S; ++s2) (void)host_channel(h, so, s1, s2);
+    // Stored entry M[k,i,j] (row (i-1)N + k, column j): entries with k == i or k == j go to the
+    // FP64 loss matrix at (k, the other index); the others to the pair (min, max) of their
+    // channel.  Each pair row receives its two halves M[k,i,j] and M[k,j,i] in CSC order,
+    // which gives the dense form's sum M[k,lo,hi] + M[k,hi,lo] bit for bit (0 + a + b).
+    for (int64_t j = 0; j < N; ++j)
+        for (int64_t q = colptr[j] - 1; q < colptr[j + 1] - 1; ++q) {
+            const double v = nzval[q];
+            if (v == 0.0) continue;
+            const int64_t r = rowval[q] - 1;
+            const int64_t i = r / N, k = r % N;
+            if (k == i || k == j) {
+                const int64_t other = (k == i) ? j : i;
+                h->loss_host[(size_t)k + (size_t)N * other] += v;
+                h->loss_nonzero = true;
+                continue;
+            }
+            int64_t lo = i, hi = j;
+            if (sp[(size_t)i] > sp[(size_t)j] || (sp[(size_t)i] == sp[(size_t)j] && i > j)) {
+                lo = j;
+                hi = i;
+            }
+            const int so = sp[(size_t)k], s1 = sp[(size_t)lo], s2 = sp[(size_t)hi];
+            HostChannel& ch = host_channel(h, so, s1, s2);
+            const int a = (int)(lo - h->sp_off[s1]), b = (int)(hi - h->sp_off[s2]);
+            const int64_t p = (s1 == s2) ? tri_index(a, b, ch.N1) : (int64_t)a * ch.N2 + b;
+            double& dst = ch.S[(size_t)p * ch.No + (size_t)(k - h->sp_off[so])];
+            dst = dst + v;
+        }
     h->binary = true;
     h->binary_dirty = true;
     return DTX_OK;
@@ -1857,6 +1953,38 @@ int dtx_set_emission_dense(dtx_handle* h, int64_t n, const double* M) {
                 continue;
             }
             blocks[(size_t)ci * N * N + (size_t)(i % N) + (size_t)N * (j % N)] = v;
+        }
+    }
+    return dtx_set_emission_blocks(h, blocks.data());
+}
+
+int dtx_set_emission_csc(dtx_handle* h, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                         const double* nzval) {
+    if (!h) return DTX_ERR_INVALID;
+    if (n != h->n) return fail(h, DTX_ERR_INVALID, "M_Emi must be n x n with n = %lld",
+                               (long long)h->n);
+    if (!colptr || (colptr[n] > 1 && (!rowval || !nzval)))
+        return fail(h, DTX_ERR_INVALID, "null CSC array");
+    const int N = h->N, C = h->C;
+    std::vector<double> blocks((size_t)C * N * N, 0.0);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t k0 = colptr[j] - 1, k1 = colptr[j + 1] - 1;
+        if (k0 < 0 || k1 < k0)
+            return fail(h, DTX_ERR_INVALID, "colptr is not a valid 1-based CSC pointer");
+        const int64_t cj = j / N;
+        for (int64_t q = k0; q < k1; ++q) {
+            const int64_t i = rowval[q] - 1;
+            if (i < 0 || i >= n) return fail(h, DTX_ERR_INVALID, "rowval out of range");
+            const double v = nzval[q];
+            if (i / N != cj) {
+                if (v != 0.0)
+                    return fail(h, DTX_ERR_UNSUPPORTED,
+                                "M_Emi has a non-zero outside the per-cell diagonal blocks "
+                                "(row %lld, column %lld)", (long long)i + 1, (long long)j + 1);
+                continue;
+            }
+            double& dst = blocks[(size_t)cj * N * N + (size_t)(i % N) + (size_t)N * (j % N)];
+            dst = dst + v;
         }
     }
     return dtx_set_emission_blocks(h, blocks.data());

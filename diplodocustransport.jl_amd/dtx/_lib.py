@@ -84,11 +84,13 @@ SIGNATURES = [
     ("dtx_last_error", C.c_char_p, [C.c_void_p]),
     ("dtx_set_flux_csc", I32, [C.c_void_p, I32, I64, PI64, PI64, PD]),
     ("dtx_set_binary_dense", I32, [C.c_void_p, I64, PD]),
+    ("dtx_set_binary_csc", I32, [C.c_void_p, I64, PI64, PI64, PD]),
     ("dtx_add_binary_gain", I32, [C.c_void_p, I32, I32, I32, PD]),
     ("dtx_add_binary_loss", I32, [C.c_void_p, I32, I32, PD]),
     ("dtx_set_binary_domain", I32, [C.c_void_p, I64, PI64]),
     ("dtx_set_emission_dense", I32, [C.c_void_p, I64, PD]),
     ("dtx_set_emission_blocks", I32, [C.c_void_p, PD]),
+    ("dtx_set_emission_csc", I32, [C.c_void_p, I64, PI64, PI64, PD]),
     ("dtx_set_vectors", I32, [C.c_void_p, PD, PD, PD, PD]),
     ("dtx_set_moment_weights", I32, [C.c_void_p, PD]),
     ("dtx_set_moment_correction", I32, [C.c_void_p, I32]),

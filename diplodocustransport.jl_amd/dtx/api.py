@@ -45,7 +45,9 @@ class HIPBackend(BackendType):
 @dataclass
 class BinaryMatricesStruct:
     """M_Bin as the (N^2 x N) column-major tensor (any array whose Fortran-order memory is
-    M_Bin, e.g. a 3-D M[k,i,j] Fortran array), and/or the channel form (gains, losses).
+    M_Bin, e.g. a 3-D M[k,i,j] Fortran array) or as a sparse N^2 x N matrix (Bin_sparse: a
+    1-based (colptr, rowval, nzval) CSC tuple or a scipy sparse matrix), and/or the channel form
+    (gains, losses).
     Binary_list non-empty <=> binary interactions are on (src/SteppingStructs.jl:46)."""
     M_Bin: Optional[np.ndarray] = None
     Binary_list: list = field(default_factory=list)
@@ -58,7 +60,8 @@ class BinaryMatricesStruct:
 
 @dataclass
 class EmissionMatricesStruct:
-    """M_Emi dense n x n (column-major, block diagonal per cell) or per-cell blocks."""
+    """M_Emi dense n x n (column-major, block diagonal per cell), sparse n x n (Emi_sparse: a
+    1-based CSC tuple or a scipy sparse matrix), or per-cell blocks."""
     M_Emi: Optional[np.ndarray] = None
     Emission_list: list = field(default_factory=list)
     blocks: Optional[np.ndarray] = None
@@ -85,6 +88,23 @@ class OutputStruct:
 
 def _f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _is_sparse(M) -> bool:
+    """A SparseMatrixCSC given as a 1-based (colptr, rowval, nzval) tuple, or a scipy sparse
+    matrix."""
+    return isinstance(M, tuple) or hasattr(M, "tocsc")
+
+
+def _as_julia_csc(M, shape):
+    """(colptr, rowval, nzval) 1-based Int64 CSC with rows ascending in each column."""
+    if isinstance(M, tuple):
+        return _csc_args(M)
+    if tuple(M.shape) != tuple(shape):
+        raise ValueError(f"sparse matrix is {M.shape}, expected {shape}")
+    A = M.tocsc(copy=True)
+    A.sort_indices()
+    return _csc_args((A.indptr.astype(np.int64) + 1, A.indices.astype(np.int64) + 1, A.data))
 
 
 def _csc_args(csc):
@@ -148,6 +168,12 @@ class _SteppingStruct:
                 _lib.check(self.lib.dtx_synth_binary(h, int(BinM.synth_seed),
                                                      float(BinM.synth_scale)), h,
                            "dtx_synth_binary")
+            elif BinM.M_Bin is not None and _is_sparse(BinM.M_Bin):
+                # Bin_sparse: the CSC goes to the library as is (no dense N^2 x N copy)
+                N = PhaseSpace.n_momentum
+                cp, rv, nz = _as_julia_csc(BinM.M_Bin, (N * N, N))
+                _lib.check(self.lib.dtx_set_binary_csc(h, N, _lib.i64ptr(cp), _lib.i64ptr(rv),
+                                                       _lib.dptr(nz)), h, "dtx_set_binary_csc")
             elif BinM.M_Bin is not None:
                 M = np.asarray(BinM.M_Bin, dtype=np.float64)
                 Mf = M.reshape(-1, order="F") if M.flags.f_contiguous else \
@@ -177,6 +203,11 @@ class _SteppingStruct:
                 B = np.ascontiguousarray(EmiM.blocks, dtype=np.float64)
                 _lib.check(self.lib.dtx_set_emission_blocks(h, _lib.dptr(B)), h,
                            "dtx_set_emission_blocks")
+            elif _is_sparse(EmiM.M_Emi):  # Emi_sparse
+                cp, rv, nz = _as_julia_csc(EmiM.M_Emi, (n, n))
+                _lib.check(self.lib.dtx_set_emission_csc(h, n, _lib.i64ptr(cp), _lib.i64ptr(rv),
+                                                         _lib.dptr(nz)), h,
+                           "dtx_set_emission_csc")
             else:
                 M = np.asfortranarray(EmiM.M_Emi, dtype=np.float64).reshape(-1, order="F")
                 _lib.check(self.lib.dtx_set_emission_dense(h, n, _lib.dptr(M)), h,

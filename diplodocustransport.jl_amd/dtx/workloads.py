@@ -224,6 +224,16 @@ def emission_dense(w: Workload) -> Optional[np.ndarray]:
     return M
 
 
+def binary_csc(w: Workload):
+    """The monolithic M_Bin as the reference's sparse form (Bin_sparse): the (N^2 x N) matrix as
+    a 1-based (colptr, rowval, nzval) CSC of its non-zero entries."""
+    import scipy.sparse as sp
+    N = w.ps.n_momentum
+    A = sp.csc_matrix(np.asarray(w.M_Bin).reshape(N * N, N, order="F"))
+    A.sort_indices()
+    return (A.indptr.astype(np.int64) + 1, A.indices.astype(np.int64) + 1, A.data.copy())
+
+
 def emission_csc(w: Workload):
     """The per-cell emission blocks as the reference's sparse M_Emi (SparseMatrixCSC, 1-based
     Int64, every block entry stored): (colptr, rowval, nzval), or None."""
@@ -245,9 +255,10 @@ def emission_csc(w: Workload):
 
 
 def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=None,
-                moment_correction=True, device_flux=False):
+                moment_correction=True, device_flux=False, sparse=False):
     """ForwardEulerStruct / ForwardSymplecticEulerStruct on the HIP backend for a workload;
-    device_flux: the flux stencils, Ap and Vol built on the device from w.flux_spec."""
+    device_flux: the flux stencils, Ap and Vol built on the device from w.flux_spec;
+    sparse: the monolithic M_Bin and the emission passed as CSC (Bin_sparse / Emi_sparse)."""
     from .api import (BinaryMatricesStruct, EmissionMatricesStruct, FluxMatricesStruct,
                       ForwardEulerStruct, ForwardSymplecticEulerStruct)
     mono = (w.M_Bin is not None) if use_monolithic is None else use_monolithic
@@ -255,12 +266,15 @@ def make_struct(w: Workload, precision="fp64", method="euler", use_monolithic=No
         BinM = BinaryMatricesStruct(Binary_list=["synthetic"], Domain=w.domain,
                                     synth_seed=w.synth_seed)
     elif mono:
-        BinM = BinaryMatricesStruct(M_Bin=w.M_Bin, Binary_list=["synthetic"], Domain=w.domain)
+        BinM = BinaryMatricesStruct(M_Bin=binary_csc(w) if sparse else w.M_Bin,
+                                    Binary_list=["synthetic"], Domain=w.domain)
     else:
         BinM = BinaryMatricesStruct(Binary_list=["synthetic"], Domain=w.domain, gains=w.gains,
                                     losses=w.losses)
-    EmiM = EmissionMatricesStruct(Emission_list=["synthetic"] if w.emission_blocks is not None
-                                  else [], blocks=w.emission_blocks)
+    has_emi = w.emission_blocks is not None
+    EmiM = EmissionMatricesStruct(Emission_list=["synthetic"] if has_emi else [],
+                                  blocks=None if (sparse and has_emi) else w.emission_blocks,
+                                  M_Emi=emission_csc(w) if (sparse and has_emi) else None)
     if device_flux:
         if w.flux_spec is None:
             raise ValueError(f"workload {w.name} has no flux stencil to build on the device")

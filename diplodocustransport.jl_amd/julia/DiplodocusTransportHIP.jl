@@ -159,7 +159,8 @@ Creates the libdtx simulation for `method` and uploads its operators.  Called fr
 `method.f` is a DtxState and `method.HIP` holds the handle.
 """
 function dtx_attach!(method, PhaseSpace, Initial::Vector{Float64}, Injection::Vector{Float64},
-                     BinM, EmiM, FluxM, backend::HIPBackend; symplectic::Bool=false)
+                     BinM, EmiM, FluxM, backend::HIPBackend; symplectic::Bool=false,
+                     flux_synth=nothing)
     Momentum = PhaseSpace.Momentum
     Space = PhaseSpace.Space
     S = length(PhaseSpace.name_list)
@@ -183,17 +184,29 @@ function dtx_attach!(method, PhaseSpace, Initial::Vector{Float64}, Injection::Ve
                            (Ptr{Cvoid}, Int32, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
                            h.ptr, Int32(which), Int64(n), F.colptr, F.rowval, F.nzval))
     end
-    if symplectic
+    if flux_synth !== nothing
+        # the synthetic Cartesian stencils, 1 ./ Ap and Vol built on the device instead of the
+        # FluxM CSC upload; dtx_set_vectors then takes C_NULL for Ap and Vol
+        dtx_synth_flux_handle!(h, flux_synth...)
+    elseif symplectic
         set_csc(1, FluxM.X_Flux)     # DTX_FLUX_X
         set_csc(2, FluxM.P_Flux)     # DTX_FLUX_P
     else
         set_csc(0, FluxM.X_Flux + FluxM.P_Flux)   # F_Flux (src/SteppingStructs.jl:60)
     end
     if !isempty(BinM.Binary_list)
-        M = Matrix{Float64}(BinM.M_Bin)
-        size(M) == (N^2, N) || error("M_Bin must be (N^2 x N) with N = $N")
-        dtx_check(h, ccall((:dtx_set_binary_dense, libdtx), Cint,
-                           (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(N), M))
+        size(BinM.M_Bin) == (N^2, N) || error("M_Bin must be (N^2 x N) with N = $N")
+        if BinM.M_Bin isa AbstractSparseMatrix
+            # Bin_sparse: the CSC as Julia holds it (src/SteppingStructs.jl:178-179)
+            B = SparseMatrixCSC{Float64,Int64}(BinM.M_Bin)
+            dtx_check(h, ccall((:dtx_set_binary_csc, libdtx), Cint,
+                               (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                               h.ptr, Int64(N), B.colptr, B.rowval, B.nzval))
+        else
+            M = Matrix{Float64}(BinM.M_Bin)
+            dtx_check(h, ccall((:dtx_set_binary_dense, libdtx), Cint,
+                               (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(N), M))
+        end
         dom = BinM.Domain
         if dom === nothing
             dtx_check(h, ccall((:dtx_set_binary_domain, libdtx), Cint,
@@ -205,14 +218,24 @@ function dtx_attach!(method, PhaseSpace, Initial::Vector{Float64}, Injection::Ve
         end
     end
     if !isempty(EmiM.Emission_list)
-        E = Matrix{Float64}(EmiM.M_Emi)
-        dtx_check(h, ccall((:dtx_set_emission_dense, libdtx), Cint,
-                           (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(n), E))
+        if EmiM.M_Emi isa AbstractSparseMatrix
+            # Emi_sparse (src/SteppingStructs.jl:183-184): no dense n x n copy
+            E = SparseMatrixCSC{Float64,Int64}(EmiM.M_Emi)
+            dtx_check(h, ccall((:dtx_set_emission_csc, libdtx), Cint,
+                               (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                               h.ptr, Int64(n), E.colptr, E.rowval, E.nzval))
+        else
+            E = Matrix{Float64}(EmiM.M_Emi)
+            dtx_check(h, ccall((:dtx_set_emission_dense, libdtx), Cint,
+                               (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(n), E))
+        end
     end
+    ap, vol = flux_synth === nothing ?
+        (Vector{Float64}(FluxM.Ap_Flux), Vector{Float64}(FluxM.Vol)) :
+        (Ptr{Float64}(C_NULL), Ptr{Float64}(C_NULL))
     dtx_check(h, ccall((:dtx_set_vectors, libdtx), Cint,
                        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                       h.ptr, Vector{Float64}(FluxM.Ap_Flux), Vector{Float64}(FluxM.Vol),
-                       Initial, Injection))
+                       h.ptr, ap, vol, Initial, Injection))
     dtx_check(h, ccall((:dtx_set_moment_weights, libdtx), Cint, (Ptr{Cvoid}, Ptr{Float64}),
                        h.ptr, dtx_energy_weights(PhaseSpace)))
     method.HIP = h
@@ -280,10 +303,20 @@ Builds the synthetic Cartesian flux operators (upwind advection along the given 
 synchrotron cooling along p when sync_K > 0), 1 ./ Ap and Vol on the device, in place of the
 X_Flux / P_Flux CSC upload (src/Fluxes/BuildFluxMatrices.jl:10-184).  `pr_list[s]`,
 `ur_list[s]`, `hr_list[s]` are the bounds of species s; `mass` and `charge` one value per species.
+On a method already attached it replaces the uploaded flux operators, 1 ./ Ap and Vol (FluxM is
+then no longer what the method steps); to build them on the device from the start, pass the
+same arguments to the constructor as `dtx_attach!(...; flux_synth=(xr, yr, zr, pr_list,
+ur_list, hr_list, mass, charge))` (keywords as a trailing NamedTuple are not accepted there:
+sync_K, axes and periodic go in as positional `sync_K`, `axes`, `periodic` after `charge`).
 """
 function dtx_synth_flux!(method, xr, yr, zr, pr_list, ur_list, hr_list, mass, charge;
                          sync_K=0.0, axes=(:x,), periodic=true)
-    h = method.HIP
+    dtx_synth_flux_handle!(method.HIP, xr, yr, zr, pr_list, ur_list, hr_list, mass, charge,
+                           sync_K, axes, periodic)
+end
+
+function dtx_synth_flux_handle!(h::DtxHandle, xr, yr, zr, pr_list, ur_list, hr_list, mass,
+                                charge, sync_K=0.0, axes=(:x,), periodic=true)
     pad(v) = ntuple(i -> i <= length(v) ? Float64(v[i]) : 0.0, DTX_MAX_SPECIES)
     bufs = map(v -> Vector{Float64}(v),
                (xr, yr, zr, reduce(vcat, pr_list), reduce(vcat, ur_list), reduce(vcat, hr_list)))

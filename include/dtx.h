@@ -120,6 +120,14 @@ int dtx_set_flux_csc(dtx_handle* h, int32_t which, int64_t n, const int64_t* col
  * Replaces CuArray(BinM.M_Bin) (src/SteppingStructs.jl:66). */
 int dtx_set_binary_dense(dtx_handle* h, int64_t N, const double* M_Bin);
 
+/* The same tensor as Julia's SparseMatrixCSC{Float64,Int64} (Bin_sparse, N^2 x N, 1-based
+ * colptr (N+1), rowval and nzval (nnz); src/Collisions/BuildBinaryMatrices.jl:44-52), without
+ * densifying it.  Replaces CuSparseMatrixCSC(BinM.M_Bin) (src/SteppingStructs.jl:178-179).
+ * The stored entries are folded into the same packed operator as dtx_set_binary_dense builds
+ * from the dense array, bit for bit. */
+int dtx_set_binary_csc(dtx_handle* h, int64_t N, const int64_t* colptr, const int64_t* rowval,
+                       const double* nzval);
+
 /* Channel form of the same tensor, for sizes where the monolithic array does not fit.
  * Gain: dB[k] += sum_{i in s1, j in s2} G[k + No*(i + N1*j)] f_i f_j for k in s_out, which is
  * what GainMatrix_to_M_Bin! contributes (G/2 at (k,i,j) and (k,j,i),
@@ -137,6 +145,12 @@ int dtx_set_binary_domain(dtx_handle* h, int64_t count, const int64_t* domain);
  * library keeps only the C diagonal N x N blocks and rejects off-block non-zeros.
  * Replaces CuArray(EmiM.M_Emi) (src/SteppingStructs.jl:67). */
 int dtx_set_emission_dense(dtx_handle* h, int64_t n, const double* M_Emi);
+/* EmiM.M_Emi as SparseMatrixCSC{Float64,Int64} (Emi_sparse, n x n, 1-based;
+ * src/Collisions/BuildEmissionMatrices.jl:28-39), without densifying it.  Replaces
+ * CuSparseMatrixCSC(EmiM.M_Emi) (src/SteppingStructs.jl:183-184).  Stored entries outside the
+ * per-cell diagonal blocks must be zero. */
+int dtx_set_emission_csc(dtx_handle* h, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                         const double* nzval);
 /* The same operator given directly as C blocks of N x N column-major (cell-major order). */
 int dtx_set_emission_blocks(dtx_handle* h, const double* blocks);
 

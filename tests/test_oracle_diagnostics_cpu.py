@@ -147,3 +147,18 @@ def test_sparse_and_dense_emission():
     assert np.all(np.isnan(a.pre_clamp))
     assert not np.any(np.isnan(b.pre_clamp[:2 * N]))   # cells 0, 1 untouched by the sparse one
     assert np.all(np.isnan(b.pre_clamp[2 * N:]))
+
+
+def test_binary_csc_is_the_monolithic_tensor():
+    """The sparse M_Bin handed to dtx_set_binary_csc holds exactly the dense tensor's non-zero
+    entries at M_Bin[(i-1)N + k, j] (1-based), rows ascending in each column."""
+    w = workloads.tiny(steps=2)
+    colptr, rowval, nzval = workloads.binary_csc(w)
+    N = w.ps.n_momentum
+    D = np.zeros((N * N, N))
+    for j in range(N):
+        for q in range(colptr[j] - 1, colptr[j + 1] - 1):
+            D[rowval[q] - 1, j] += nzval[q]
+        assert np.all(np.diff(rowval[colptr[j] - 1:colptr[j + 1] - 1]) > 0)
+    np.testing.assert_array_equal(D, w.M_Bin.reshape(N * N, N, order="F"))
+    assert nzval.size == np.count_nonzero(w.M_Bin)
