@@ -226,7 +226,7 @@ hipError_t launch_synth_moment_packed(double* AN, double* AE, int64_t nchunks, i
 
 // Synthetic Cartesian flux stencils built on the device (synth_flux_kernel): upwind advection
 // along x and / or y, synchrotron radiation reaction along p.
-constexpr int FLUX_MAX_ROW = 8;  // entries per row: diagonal, 2 per space axis, p +- 1
+constexpr int FLUX_MAX_ROW = 10;  // entries per row: diagonal, 2 per space axis, p +- 1, u +- 1
 struct FluxSynthArgs {
     const double *xr, *yr, *zr;        // cell bounds (x_num + 1, ...)
     const double *pr, *ur, *hr;        // momentum bounds of every species, concatenated

@@ -286,6 +286,30 @@ __device__ void flux_row(const FluxSynthArgs& a, int64_t r, FluxRow& R) {
             flux_push(P, r, Im * hmr / norm);
             flux_push(P, r - 1, Im * hml / norm_of(ip - 1));
         }
+        // the J stencil along u (:420-557), pushed after I as BuildFluxMatrices pushes them
+        // (:117-120): J(u) = K (1/2) m u (u^2 - 1) (h0 - h1) (2 asinh(p1/m) - 2 asinh(p0/m))
+        // dt dx dy dz (src/Fluxes/FluxFunctionsEmissionForces.jl:84-150), closed u boundaries
+        double Jp = 0.0, Jm = -0.0;
+        if (m != 0.0 && a.charge[s] != 0.0) {
+            const double ash = 2.0 * asinh(p1 / m) - 2.0 * asinh(p0 / m);
+            auto jforce = [&](double uu) {
+                return a.sync_K * (0.5 * m * uu * (uu * uu - 1.0) * (h0 - h1) * ash);
+            };
+            Jp = vol * jforce(u1);
+            Jm = -(vol * jforce(u0));
+        }
+        const double jpr = Jp > 0.0 ? 0.0 : 1.0, jpl = 1.0 - jpr;
+        const double jmr = Jm > 0.0 ? 1.0 : 0.0, jml = 1.0 - jmr;
+        auto norm_u = [&](int q) { return (p1 - p0) * (ur[q + 1] - ur[q]) * dh; };
+        const int px = a.px[s];
+        if (iu < a.py[s] - 1) {
+            flux_push(P, r + px, Jp * jpr / norm_u(iu + 1));
+            flux_push(P, r, Jp * jpl / norm);
+        }
+        if (iu > 0) {
+            flux_push(P, r, Jm * jmr / norm);
+            flux_push(P, r - px, Jm * jml / norm_u(iu - 1));
+        }
     }
     R.cnt = 0;
     for (int e = 0; e < X.cnt; ++e) flux_push(R, X.col[e], X.val[e]);

@@ -291,10 +291,13 @@ def flux_matrices(ps: PhaseSpace, sync_K: float = 0.0, periodic: bool = True,
     Space axis 'y': the perpendicular component <v><sqrt(1-u^2)><sin(phi)>.
     Each face flux carries (t1-t0) and the two transverse widths; the entries are divided by
     MomentumSpaceNorm (dp du dphi), exactly as Fill_B_Flux! pushes them (:801-833).
-    Momentum axis p (sync_K > 0): synchrotron radiation reaction,
+    Momentum axes p and u (sync_K > 0): synchrotron radiation reaction,
     I(p) = -K/(3m) p sqrt(m^2+p^2) (g(u1)-g(u0)) dphi dt dx dy dz with g(u) = 3u-u^3
-    (src/Fluxes/FluxFunctionsEmissionForces.jl:46-70), closed p boundaries, zero for massless or
-    neutral species (zero-valued entries still pushed), stencil of Fill_I_Flux! (:308-408).
+    (src/Fluxes/FluxFunctionsEmissionForces.jl:46-70), stencil of Fill_I_Flux! (:308-408), and
+    J(u) = K (1/2) m u (u^2-1) (h0-h1) (2 asinh(p1/m) - 2 asinh(p0/m)) dt dx dy dz (:84-150),
+    stencil of Fill_J_Flux! (:420-557), pushed after I; closed boundaries, zero for massless or
+    neutral species (zero-valued entries still pushed).  The K (phi) flux of the synchrotron
+    force is zero in every mode (:152-215); its stored zeros are not pushed.
     Zero-weighted upwind entries are pushed too, so the CSC keeps them as explicit zeros."""
     n, N, C = ps.n, ps.n_momentum, ps.n_space
     dx, dy, dz = deltaVector(ps.xr), deltaVector(ps.yr), deltaVector(ps.zr)
@@ -373,6 +376,33 @@ def flux_matrices(ps: PhaseSpace, sync_K: float = 0.0, periodic: bool = True,
                                          (a, a, Ip * hpl / norm, okp),
                                          (a, a, Im * hmr / norm, okm),
                                          (a, bm, Im * hml / norm_m, okm)):
+                PI.append(rows[ok]); PJ.append(cols[ok]); PV.append(vals[ok])
+            # the J stencil along u (Fill_J_Flux!, src/Fluxes/BuildFluxMatrices.jl:420-557),
+            # pushed after the I entries as BuildFluxMatrices pushes them (:117-120):
+            # J(u) = K (1/2) m u (u^2 - 1) (h0 - h1) (2 asinh(p1/m) - 2 asinh(p0/m)) dt dx dy dz
+            # (JFluxFunction, src/Fluxes/FluxFunctionsEmissionForces.jl:84-150, Ani / Axi),
+            # closed u boundaries, zero-valued entries for massless or neutral species
+            if m != 0.0 and s.charge != 0.0:
+                ash = 2.0 * np.arcsinh(pr[P + 1] / m) - 2.0 * np.arcsinh(pr[P] / m)
+                Jfn = lambda uu: sync_K * (0.5 * m * uu * (uu * uu - 1.0) * (h0 - h1) * ash)  # noqa
+                Jp = vol * Jfn(u1)[None, :]
+                Jm = -(vol * Jfn(u0)[None, :])
+            else:
+                Jp = np.zeros((C, s.size))
+                Jm = -np.zeros((C, s.size))
+            hpr, hpl, hmr, hml = _upwind(Jp, Jm)
+            up = (U < s.u_num - 1)
+            dn = (U > 0)
+            bp = a + s.p_num   # u+1 is p_num states further (p fastest, then u)
+            bm = a - s.p_num
+            norm_p = np.where(up, np.roll(norm, -s.p_num), 1.0)
+            norm_m = np.where(dn, np.roll(norm, s.p_num), 1.0)
+            okp = np.broadcast_to(up[None, :], a.shape)
+            okm = np.broadcast_to(dn[None, :], a.shape)
+            for rows, cols, vals, ok in ((a, bp, Jp * hpr / norm_p, okp),
+                                         (a, a, Jp * hpl / norm, okp),
+                                         (a, a, Jm * hmr / norm, okm),
+                                         (a, bm, Jm * hml / norm_m, okm)):
                 PI.append(rows[ok]); PJ.append(cols[ok]); PV.append(vals[ok])
 
     def build(I, J, V):

@@ -185,8 +185,8 @@ int dtx_synth_binary(dtx_handle* h, uint64_t seed, double scale);
  * dtx_set_flux_csc uploads: upwind advection along x and / or y with the bin-averaged velocity
  * component (the B / C stencils, :708-974; Cartesian face fluxes,
  * src/Fluxes/FluxFunctionsCoordinate.jl:264-267) and synchrotron radiation reaction along p
- * (the I stencil, :275-413, with src/Fluxes/FluxFunctionsEmissionForces.jl:46-70), closed p
- * boundaries.  Builds the matrices the method reads (ForwardEuler: F = X + P; symplectic: X and
+ * and u (the I and J stencils, :275-557, with src/Fluxes/FluxFunctionsEmissionForces.jl:10-150),
+ * closed p and u boundaries.  Builds the matrices the method reads (ForwardEuler: F = X + P; symplectic: X and
  * P) with the entries, explicit zeros, duplicate sums and column order that dtx_set_flux_csc
  * gives for the host-assembled CSC, and writes 1 ./ Ap (Ap = dx dy dz per cell, :145-154) and
  * Vol = dt0 dx dy dz (src/Fluxes/FluxFunctionsCoordinate.jl:313-315), so dtx_set_vectors may
@@ -200,7 +200,7 @@ typedef struct {
     const double* hr;  /* phi bounds: pz_num[s] + 1 each                                     */
     double mass[DTX_MAX_SPECIES];
     double charge[DTX_MAX_SPECIES];
-    double sync_K;     /* synchrotron coefficient; 0: no momentum flux (P empty)             */
+    double sync_K;     /* synchrotron coefficient; 0: no momentum fluxes (P empty)           */
     int32_t axes;      /* bit 0: advection along x, bit 1: along y (axes of one cell skipped) */
     int32_t periodic;  /* 1: periodic space boundaries, 0: closed                            */
 } dtx_flux_synth;

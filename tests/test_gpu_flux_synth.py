@@ -6,7 +6,8 @@ Both handles step the same state with the same binary operators, so any differen
 entry, in 1/Ap or in Vol shows in f.  The ELL width (the longest row after duplicate sums) shows
 in the device bytes, which must agree.  Bar: 1e-13 relative (L-inf); the x stencil uses only
 sqrt and IEEE division and is expected to agree bit for bit, the y stencil and the synchrotron
-force call cos, asin and pow, whose device versions may differ from the host's by an ulp.
+forces (I along p, J along u) call cos, asin, pow and asinh, whose device versions may differ
+from the host's by an ulp.
 """
 import numpy as np
 import pytest
@@ -46,7 +47,7 @@ def test_device_flux_matches_host_csc(case):
     if case == "transport_1d_full":  # 65,536 rows: the full transport_1d operator
         w = workloads.transport_1d(steps=6)
     elif case.startswith("xy"):
-        w = workloads.transport_2d(p_num=4, u_num=2, h_num=2, x_num=4, y_num=3, steps=6)
+        w = workloads.transport_2d(p_num=4, u_num=4, h_num=2, x_num=4, y_num=3, steps=6)
     else:
         w = workloads.tiny(x_num=5 if case.startswith("x5") else 2, steps=6, monolithic=False)
     print(f"case {case}: n {w.ps.n}, spec {w.flux_spec}", flush=True)
@@ -58,7 +59,7 @@ def test_device_flux_matches_host_csc(case):
 def test_device_flux_closed_boundaries_and_z_cells():
     """Closed x / y boundaries (clamped neighbours drop the boundary faces), z_num = 2 (the
     transverse widths and the cell order), non-uniform x cells; fluxes only."""
-    ps = PhaseSpace(workloads.species_pair(4, 2, 2), x_num=3, y_num=4, z_num=2, x_up=2.0)
+    ps = PhaseSpace(workloads.species_pair(4, 4, 2), x_num=3, y_num=4, z_num=2, x_up=2.0)
     spec = dict(sync_K=0.3, axes="xy", periodic=False)
     X, P = synth.flux_matrices(ps, **spec)
     Ap, Vol = synth.ap_vol(ps)

@@ -98,8 +98,10 @@ def test_flux_csc_matches_dense_and_conserves():
     for j in range(n):
         r = rowval[colptr[j] - 1:colptr[j + 1] - 1]
         assert np.all(np.diff(r) > 0)
-    # about 5 stored entries per row (self, x +-1, p +-1), explicit zeros kept
-    assert 4.5 <= (colptr[-1] - 1) / n <= 5.0
+    # stored entries per row: self, x +-1, p +-1 and u +-1 except at the closed p and u
+    # boundaries, explicit zeros kept
+    p, u = 4, 4
+    assert (colptr[-1] - 1) / n == 1 + 2 + 2 * (p - 1) / p + 2 * (u - 1) / u
 
 
 def test_courant_one_pulse_advects_one_cell():
@@ -205,3 +207,28 @@ def test_moments_closed_form():
     g = np.zeros(s.size)
     g[4] = 1.0
     assert O.moments(g, s)[1] == grids.energy_weight_T00(s.pr, 1.0)[4]
+
+
+def test_synchrotron_j_flux_aligns_pitch_angle():
+    """The J (pitch-angle) part of the synchrotron radiation reaction alone (one p bin, so no I
+    neighbours): conserves number, keeps f mirror-symmetric in u, moves particles towards
+    |u| = 1 (alignment with B), and passes nothing through u = +-1."""
+    sp = [grids.Species("Ele", 1.0, -1.0, 1, 8, 1, p_low=0.0, p_up=0.5)]
+    ps = grids.PhaseSpace(sp, x_num=1)
+    _, P = synth.flux_matrices(ps, sync_K=1.0, axes="")
+    D = synth.csc_to_dense(P, ps.n)
+    Ap, Vol = synth.ap_vol(ps)
+    assert np.count_nonzero(D) > 0
+    assert np.max(np.abs(D.sum(axis=0))) <= 1e-14 * np.max(np.abs(D))   # conservation
+    f0 = np.ones(ps.n)
+    prob = O.OracleProblem(ps, Ap, Vol, np.zeros(ps.n), F=P)
+    st = O.Stepper(prob, f0)
+    dt = 0.1 / np.max(np.abs(np.diag(D)) / Ap)
+    st(0.0, 1.0, dt, 0)
+    f1 = st.f
+    assert abs(f1.sum() - f0.sum()) <= 1e-14 * f0.sum()
+    np.testing.assert_allclose(f1, f1[::-1], rtol=1e-14)               # u -> -u symmetry
+    umid = np.abs(0.5 * (sp[0].ur[1:] + sp[0].ur[:-1]))
+    assert umid @ f1 > umid @ f0                                       # towards |u| = 1
+    assert f1[0] > f0[0] and f1[-1] > f0[-1]                            # outer bins gain
+    assert f1[3] < f0[3] and f1[4] < f0[4]                              # u ~ 0 bins lose
