@@ -385,7 +385,8 @@ def main():
     ap.add_argument("--cpu-opt-steps", type=int, default=5,
                     help="timed steps of the optimised CPU column (batched DGEMM)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
+    ap.add_argument("--precisions", default="fp64,fp32,bf16x3",
+                    help="contraction precisions to time (fp64 is always run)")
     ap.add_argument("--no-uncorrected", action="store_true",
                     help="skip the FP32 / BF16x3 runs without the FP64 moment-rate correction")
     ap.add_argument("--no-serving", action="store_true")
@@ -431,7 +432,11 @@ def main():
         f"C={w.ps.n_space} n={w.ps.n} dt={w.dt:.4e}")
     res = {}
     runs = []
-    for prec in args.precisions.split(","):
+    precisions = [p for p in args.precisions.split(",") if p]
+    if "fp64" not in precisions:  # the line's value and every comparison are the FP64 run's
+        log("[bench] fp64 added to --precisions: the reported value is the FP64 rate")
+        precisions.insert(0, "fp64")
+    for prec in precisions:
         runs.append((prec, True))
         if prec != "fp64" and not args.no_uncorrected:
             runs.append((prec, False))

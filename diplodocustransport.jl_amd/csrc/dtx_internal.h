@@ -133,7 +133,16 @@ struct UpdateArgs {
     double s;               // dt_scale
     double t_start, dt;
     int32_t save;
+    // batched finalisation (Forward Euler, small grids): the update writes its step's scalars
+    // to fin_rec[fin_slot] (FIN_WORDS doubles) for finalize_batch; null: not batched
+    double* fin_rec;
+    int32_t fin_slot;
 };
+
+// Steps whose finalisation is batched into one launch (their block partials and scalars are
+// kept in rings of this many slots), and the doubles of one step's record.
+constexpr int FIN_BATCH = 64;
+constexpr int FIN_WORDS = 8;
 
 constexpr int UPDATE_STATES = 64;  // states per update workgroup (four threads each)
 constexpr int UPDATE_THREADS = 256;  // threads per update and finalize workgroup
@@ -193,11 +202,14 @@ hipError_t launch_binary_wide(int precision, int ns, const BinWork* items, int n
 hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,
                             int rows, int nktiles, double* A, hipStream_t stream);
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream);  // update + finalisation
-// Forward Euler update without its finalisation, running the single-workgroup finalisation of
-// the previous step (prev, or none) beside it; only for grids where finalize_is_single holds.
-// The two steps' block partials must be in different buffers.
-hipError_t launch_euler_update_deferred(const UpdateArgs& a, const UpdateArgs* prev,
-                                        hipStream_t stream);
+// Forward Euler update without its finalisation (the step's record goes to a.fin_rec); only for
+// grids where finalize_is_single holds.  finalize_batch later finalises `count` such steps
+// whose records and block partials sit in ring slots first, first + 1, ... (mod FIN_BATCH;
+// block partials of slot q at base.block_part + q * bp_words), one workgroup per step, and
+// writes the moments of the last of them (base.mom_part holds that step's partials).
+hipError_t launch_euler_update_only(const UpdateArgs& a, hipStream_t stream);
+hipError_t launch_finalize_batch(const UpdateArgs& base, int64_t bp_words, int first, int count,
+                                 hipStream_t stream);
 bool finalize_is_single(const UpdateArgs& a);
 hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream);

@@ -95,7 +95,10 @@ __device__ inline double ell_row(const int32_t* col, const double* val, int widt
 // lane holds the same, order-fixed result.
 __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, bool valid,
                                      const double* coef) {
-    constexpr int SB = 8;  // slot loads in flight per thread
+#ifndef DTX_UPDATE_SB
+#define DTX_UPDATE_SB 8
+#endif
+    constexpr int SB = DTX_UPDATE_SB;  // slot loads in flight per thread
     const int64_t idx = (int64_t)c * a.N + k;
     double g = 0.0;
     if (a.gsum) {
@@ -429,11 +432,12 @@ __device__ inline void fold_status(const UpdateArgs& a, int nseg, int ngroups, d
 // Small grids (at most FINALIZE_SINGLE_GROUPS fold groups): one workgroup folds the per-block
 // partials (thread-strided, then the workgroup pattern) into the moments of every (cell, species)
 // and the status record of the step.  Loads are issued in batches ahead of the ordered adds.
-__device__ void finalize_single(const UpdateArgs& a, int nseg, double* red) {
+__device__ void finalize_single(const UpdateArgs& a, int nseg, double* red,
+                                bool moments = true) {
     constexpr int LB = 8;
     const int tid = threadIdx.x;
     const int64_t nblocks = (int64_t)a.C * nseg;
-    for (int64_t q = tid; q < (int64_t)a.C * a.S; q += UB) {
+    for (int64_t q = tid; moments && q < (int64_t)a.C * a.S; q += UB) {
         const int64_t cc = q / a.S, sp = q % a.S;
         double num = 0.0, en = 0.0;
         for (int g0 = 0; g0 < nseg; g0 += LB) {
@@ -520,12 +524,74 @@ __device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
 }
 
 // ----------------------------------------------------------------------- ForwardEuler ------
+// Flux row r spread over the four lanes of its state: lane q loads ELL slots q, q + 4, ...
+// (ELL_LANE of them, so rows up to 4 ELL_LANE wide in one pass), gathers their x and forms the
+// products; ell_sum then adds the products in slot order (ascending column) on lane q = 0, the
+// order and operations of ell_row.  Padding slots (column -1) contribute nothing.
+constexpr int ELL_LANE = 2;
+
+struct EllLane {
+    int32_t c[ELL_LANE];
+    double v[ELL_LANE];
+};
+
+__device__ inline void ell_issue(const UpdateArgs& a, int64_t r, int q, bool mine, int s0,
+                                 EllLane& L) {
+#pragma unroll
+    for (int j = 0; j < ELL_LANE; ++j) {
+        const int e = s0 + q + 4 * j;
+        const bool in = mine && e < a.ell_width;
+        L.c[j] = in ? a.ell_col[(int64_t)e * a.n + r] : -1;
+        L.v[j] = in ? a.ell_val[(int64_t)e * a.n + r] : 0.0;
+    }
+}
+
+// Adds the products of slots [s0, s0 + 4 ELL_LANE) to acc on lane q = 0 of each state.
+__device__ inline double ell_sum(const UpdateArgs& a, const EllLane& L, int s0, double acc) {
+    double x[ELL_LANE];
+#pragma unroll
+    for (int j = 0; j < ELL_LANE; ++j) x[j] = L.c[j] >= 0 ? a.f[L.c[j]] : 0.0;
+    double p[ELL_LANE];
+#pragma unroll
+    for (int j = 0; j < ELL_LANE; ++j) p[j] = L.c[j] >= 0 ? L.v[j] * x[j] : 0.0;
+    const int lane = threadIdx.x & 63, grp = lane & ~3;
+#pragma unroll
+    for (int j = 0; j < ELL_LANE; ++j)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            const double pv = __shfl(p[j], grp | qq, 64);
+            const int cv = __shfl(L.c[j], grp | qq, 64);
+            if (s0 + 4 * j + qq < a.ell_width && cv >= 0) acc = acc + pv;
+        }
+    return acc;
+}
+
 __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg, double* smem,
                                   double* red, int* ired) {
     double* fc = smem + UPDATE_STATES;    // f of the whole cell (emission / Domain only)
     const int tid = threadIdx.x, N = a.N;
     const int k0 = seg * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
+    const int k = k0 + (tid >> 2), q = tid & 3;
+    const bool mine = k < k1;
+    const int64_t r = base + k;
+    // Every load that does not depend on another is issued before the first wait: the state's
+    // own values, its flux row's columns and values (over the four lanes of the state), and,
+    // with the moment-rate correction, its gain and loss sums; the workgroup barriers below
+    // (cell load, coefficients) wait only for LDS, so these stay in flight across them.
+    double fk = 0.0, iap = 0.0, dinj = 0.0;
+    if (mine && q == 0) {
+        fk = a.f[r];
+        iap = a.inv_ap[r];
+        dinj = a.df_inj[r];
+    }
+    EllLane ell;
+    ell_issue(a, r, q, mine, 0, ell);
+    double gs = 0.0, ls = 0.0;
+    if (a.gsum && mine && q == 0) {
+        gs = a.gsum[r];
+        ls = a.part[r];
+    }
     const bool need_cell = a.emission || (a.binary && a.active);
     const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
     // update_Big_Bin! evaluates a cell when Domain is nothing, the cell is in Domain, or its f
@@ -535,20 +601,32 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
     double sumdf = 0.0, sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY,
            crf = INFINITY, fnew = 0.0;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
-    const int k = k0 + (tid >> 2), q = tid & 3;
-    const bool mine = k < k1;
     __shared__ double coef_lds[2 * DTX_MAX_SPECIES];
     const double* coef = a.gsum ? load_correction(a, c, k0, k1, coef_lds) : nullptr;
-    // the state's own loads and its flux row do not depend on the binary term: issue them first
-    const int64_t r = base + k;
-    double fk = 0.0, iap = 0.0, dinj = 0.0, dF = 0.0;
-    if (mine && q == 0) {
-        fk = a.f[r];
-        iap = a.inv_ap[r];
-        dinj = a.df_inj[r];
-        dF = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f) : 0.0;
+    double dB = 0.0;
+    if (a.binary) {
+        if (a.gsum) {
+            // the state's gain sum rescaled by (1 + alpha + beta E_k) of its species, plus its
+            // loss sum (binary_term's corrected form, on the sums loaded above)
+            double g = 0.0;
+            if (mine && active && q == 0) {
+                const double* cf = coef + 2 * a.state_species[k];
+                const double e = a.eweight ? a.eweight[k] : 0.0;
+                g = gs * (1.0 + (cf[0] + cf[1] * e)) + ls;
+            }
+            g = g + __shfl_xor(g, 1, 64);
+            g = g + __shfl_xor(g, 2, 64);
+            dB = g * ((mine && active) ? a.vol[c] : 0.0);
+        } else {
+            dB = binary_term(a, c, k, q, mine && active, coef);
+        }
     }
-    const double dB = a.binary ? binary_term(a, c, k, q, mine && active, coef) : 0.0;
+    double dF = ell_sum(a, ell, 0, 0.0);
+    for (int s0 = 4 * ELL_LANE; s0 < a.ell_width; s0 += 4 * ELL_LANE) {  // wider rows
+        EllLane more;
+        ell_issue(a, r, q, mine, s0, more);
+        dF = ell_sum(a, more, s0, dF);
+    }
     if (mine && q == 0) {
         sumf = fk;
         double df = -dF;
@@ -579,27 +657,41 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
                  INFINITY, INFINITY);
 }
 
-// Grid (C, nseg).
+// Grid (C, nseg).  With batched finalisation, workgroup (0, 0) also records the step's scalars.
 __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
     extern __shared__ double smem[];
     __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
+    if (a.fin_rec && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        double* rec = a.fin_rec + (int64_t)a.fin_slot * FIN_WORDS;
+        rec[0] = (double)a.step_index;
+        rec[1] = a.t_start;
+        rec[2] = a.dt;
+        rec[3] = (double)a.save;
+        rec[4] = (double)a.verbose;
+    }
     euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
 }
 
-// Grid (C, nseg), or (C + 1, nseg) with the previous step's finalisation folded in: workgroup
-// (C, 0) runs the single-workgroup finalisation of `fin` (that step's arguments; its block
-// partials are in the other of two buffers) beside the update, and (C, seg > 0) exit.  A kernel
-// of its own: the finalisation's registers would lower the occupancy of the plain update.
-__global__ __launch_bounds__(UB) void euler_update_fin_kernel(UpdateArgs a, UpdateArgs fin) {
-    extern __shared__ double smem[];
+// Batched finalisation: workgroup i finalises the pending step in ring slot (first + i) mod
+// FIN_BATCH from its record and block partials, with the arithmetic of the step's own
+// finalize_single; only the last step's moments are written (the moments are those of the
+// current state).
+__global__ __launch_bounds__(UB) void finalize_batch_kernel(UpdateArgs base, int nseg,
+                                                            int64_t bp_words, int first,
+                                                            int count) {
     __shared__ double red[(UB / 64) * RED_VALUES];
-    __shared__ int ired[UB / 64];
-    if ((int)blockIdx.x == a.C) {
-        if (blockIdx.y == 0) finalize_single(fin, gridDim.y, red);
-        return;
-    }
-    euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
+    const int i = blockIdx.x;
+    const int slot = (first + i) % FIN_BATCH;
+    const double* rec = base.fin_rec + (int64_t)slot * FIN_WORDS;
+    UpdateArgs a = base;
+    a.block_part = base.block_part + (int64_t)slot * bp_words;
+    a.step_index = (int32_t)rec[0];
+    a.t_start = rec[1];
+    a.dt = rec[2];
+    a.save = (int32_t)rec[3];
+    a.verbose = (int32_t)rec[4];
+    finalize_single(a, nseg, red, i == count - 1);
 }
 
 // The two levels are two launches: every fold group in parallel, then the final reduction
@@ -1003,13 +1095,18 @@ hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
     return e != hipSuccess ? e : launch_finalize(a, stream);
 }
 
-hipError_t launch_euler_update_deferred(const UpdateArgs& a, const UpdateArgs* prev,
-                                        hipStream_t stream) {
+hipError_t launch_euler_update_only(const UpdateArgs& a, hipStream_t stream) {
     hipError_t e0 = launch_coef(a, stream);
     if (e0 != hipSuccess) return e0;
-    dim3 g = update_grid(a);
-    if (prev) g.x += 1;
-    euler_update_fin_kernel<<<g, UB, update_smem(a), stream>>>(a, prev ? *prev : a);
+    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_batch(const UpdateArgs& base, int64_t bp_words, int first, int count,
+                                 hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    finalize_batch_kernel<<<count, UB, 0, stream>>>(base, (int)update_grid(base).y, bp_words,
+                                                    first, count);
     return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 
 #include "bf16_util.h"
 #include "dtx_internal.h"
+#include "status_text.h"
 
 using namespace dtx;
 
@@ -44,6 +45,13 @@ struct TimedEvent {
 };
 
 }  // namespace
+
+// What the host prints for a step once its status record is read: the verbose level it was
+// stepped with and the end of its save interval (t_save in the verbose-3 line).
+struct StepNote {
+    int8_t verbose = 0;
+    double t_stop = 0.0;
+};
 
 struct dtx_handle {
     dtx_config cfg;
@@ -121,13 +129,14 @@ struct dtx_handle {
     int32_t *d_state_species = nullptr, *d_species_off = nullptr;
     double *d_moments = nullptr, *d_mom_part = nullptr, *d_block_part = nullptr,
            *d_status = nullptr;
-    // Forward Euler on small grids: a step's finalisation runs inside the next step's update
-    // launch (or before the next host synchronisation), so consecutive steps alternate between
-    // two buffers of block partials
-    int64_t bp_words = 0, mp_words = 0;   // doubles per buffer of block_part / mom_part
-    int bp_parity = 0;
-    bool fin_pending = false;
-    UpdateArgs fin_args{};
+    // Forward Euler on small grids: steps are finalised in batches (at FIN_BATCH pending steps
+    // or before the next host synchronisation); each pending step keeps its block partials in
+    // ring slot step % FIN_BATCH and its scalars in d_fin_rec
+    int64_t bp_words = 0, mp_words = 0;   // doubles per slot of block_part / mom_part
+    double* d_fin_rec = nullptr;          // [FIN_BATCH][FIN_WORDS]
+    int fin_count = 0;                    // pending steps
+    int64_t fin_first = 0;                // step index of the first pending step
+    UpdateArgs fin_base{};                // arguments of the last pending step
     uint8_t* d_active = nullptr;
     EllMatrix ell[3];
     double* d_emi = nullptr;
@@ -139,7 +148,7 @@ struct dtx_handle {
 
     int64_t step_counter = 0;
     int64_t reported = 0;
-    std::vector<int8_t> step_verbose;  // verbose of steps not yet reported (ring order)
+    std::vector<StepNote> step_notes;  // steps not yet reported (ring order)
     int64_t device_bytes = 0;
     std::vector<void*> allocations;
 
@@ -230,14 +239,22 @@ static void dfree(dtx_handle* h, T** p, int64_t count) {
     }
 }
 
-// Synchronise the handle's stream after running a finalisation still pending from the last
-// step, so the status ring and the moments are complete when the host reads them.
+// Enqueue the finalisation of the pending steps (one launch for all of them).
+static hipError_t flush_finalize(dtx_handle* h) {
+    if (h->fin_count == 0) return hipSuccess;
+    UpdateArgs base = h->fin_base;
+    base.block_part = h->d_block_part;  // the ring
+    const int n = h->fin_count;
+    h->fin_count = 0;
+    return launch_finalize_batch(base, h->bp_words, (int)(h->fin_first % FIN_BATCH), n,
+                                 h->stream);
+}
+
+// Synchronise the handle's stream after finalising the pending steps, so the status ring and
+// the moments are complete when the host reads them.
 static hipError_t stream_sync(dtx_handle* h) {
-    if (h->fin_pending) {
-        h->fin_pending = false;
-        hipError_t e = launch_finalize(h->fin_args, h->stream);
-        if (e != hipSuccess) return e;
-    }
+    hipError_t e = flush_finalize(h);
+    if (e != hipSuccess) return e;
     return hipStreamSynchronize(h->stream);
 }
 
@@ -849,40 +866,38 @@ static int ensure_ready(dtx_handle* h) {
     return DTX_OK;
 }
 
-static void print_shortest(char* out, size_t cap, double v) {
-    if (std::isinf(v)) { snprintf(out, cap, v > 0 ? "Inf" : "-Inf"); return; }
-    if (std::isnan(v)) { snprintf(out, cap, "NaN"); return; }
-    for (int prec = 1; prec <= 17; ++prec) {
-        snprintf(out, cap, "%.*g", prec, v);
-        if (strtod(out, nullptr) == v) return;
-    }
-}
-
-static void print_status(const dtx_handle* h, const double* st, int verbose) {
-    char a[40], b[40], c[40], d[40], e[40], f[40], g[40], t[40];
-    if (h->method == DTX_METHOD_FORWARD_EULER && st[ST_FINITE] == 0.0)
-        printf("non-finite value in df calculation\n");
+// The functor's console lines, worded as src/SteppingMethods.jl:49,87-93 (ForwardEuler) and
+// :222-228 (ForwardSymplecticEuler), numbers in Julia's form.
+static void print_status(const dtx_handle* h, const double* st, const StepNote& note) {
+    const bool euler = h->method == DTX_METHOD_FORWARD_EULER;
+    if (euler && st[ST_FINITE] == 0.0) printf("non-finite value in df calculation\n");
+    const int verbose = note.verbose;
     if (verbose < 1 || verbose > 3) return;
     const double cr = st[ST_CR];
-    print_shortest(a, sizeof a, cr);
-    print_shortest(t, sizeof t, st[ST_TSTART]);
-    print_shortest(b, sizeof b, st[ST_DT]);
-    if (verbose == 1 && cr > 1.0) {
+    // only lines that are printed get formatted (the shortest-digit search is costly)
+    if (verbose == 1 && !(cr > 1.0)) return;
+    char a[40], b[40], c[40], d[40], e[40], f[40], g[40], t[40], ts[40];
+    auto cr_text = [&](char* out, double v) { status_text::julia_float(out, 40, euler ? v : status_text::round_sig3(v)); };
+    cr_text(a, cr);
+    status_text::julia_float(t, sizeof t, st[ST_TSTART]);
+    status_text::julia_float(b, sizeof b, st[ST_DT]);
+    if (verbose == 1) {
         printf("Cr = %s, t=%s, dt=%s, system may be unstable\n", a, t, b);
     } else if (verbose == 2) {
-        printf("\rCr = %s, t=%s, dt=%s\n", a, t, b);
-    } else if (verbose == 3) {
-        print_shortest(c, sizeof c, st[ST_CR_BIN]);
-        print_shortest(d, sizeof d, st[ST_CR_EMI]);
-        print_shortest(e, sizeof e, st[ST_CR_FLUX]);
-        if (h->method == DTX_METHOD_FORWARD_EULER) {
-            printf("Cr = %s,Cr_Bin = %s, Cr_Emi = %s, Cr_Flux = %s, t=%s, dt=%s\n", a, c, d, e,
-                   t, b);
+        printf(euler ? "\rCr = %s, t=%s, dt=%s\n" : "Cr = %s, t=%s, dt=%s\n", a, t, b);
+    } else {
+        cr_text(c, st[ST_CR_BIN]);
+        cr_text(d, st[ST_CR_EMI]);
+        cr_text(e, st[ST_CR_FLUX]);
+        status_text::julia_float(ts, sizeof ts, note.t_stop);
+        if (euler) {
+            printf("Cr = %s,Cr_Bin = %s, Cr_Emi = %s, Cr_Flux = %s, t=%s, t_save =%s, dt=%s\n",
+                   a, c, d, e, t, ts, b);
         } else {
-            print_shortest(f, sizeof f, st[ST_CR_MOM]);
-            print_shortest(g, sizeof g, st[ST_CR_SPACE]);
+            cr_text(f, st[ST_CR_MOM]);
+            cr_text(g, st[ST_CR_SPACE]);
             printf("Cr = %s, Cr_Bin = %s, Cr_Emi = %s, Cr_PFlux = %s, Cr_Momentum = %s, "
-                   "Cr_Space = %s,  t=%s, dt=%s\n", a, c, d, e, f, g, t, b);
+                   "Cr_Space = %s,  t=%s, t_save=%s , dt=%s\n", a, c, d, e, f, g, t, ts, b);
         }
     }
 }
@@ -897,7 +912,7 @@ static int drain_status(dtx_handle* h) {
     const int64_t first = std::max<int64_t>(h->reported, h->step_counter - STATUS_RING);
     for (int64_t s = first; s < h->step_counter; ++s) {
         const double* st = ring.data() + (s % STATUS_RING) * STATUS_WORDS;
-        print_status(h, st, h->step_verbose[(size_t)(s % STATUS_RING)]);
+        print_status(h, st, h->step_notes[(size_t)(s % STATUS_RING)]);
     }
     fflush(stdout);
     h->reported = h->step_counter;
@@ -1035,8 +1050,8 @@ static CorrectArgs make_correct_args(const dtx_handle* h) {
 
 // Enqueue one functor call with an already clipped dt.  use_scratch: the contraction's slots
 // and moment-rate sums in the device's `scratch` array, and no kernel timing (no events).
-static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, int save,
-                        bool use_scratch = false) {
+static int enqueue_step(dtx_handle* h, double t_start, double t_stop, double dt, int verbose,
+                        int save, bool use_scratch = false) {
     const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
     const bool bin = h->binary && h->nitems > 0;
     const bool timing = h->timing && !use_scratch;
@@ -1061,13 +1076,16 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     if (h->method == DTX_METHOD_FORWARD_EULER) {
         UpdateArgs a = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_F, use_scratch);
         if (finalize_is_single(a)) {
-            a.block_part = h->d_block_part + h->bp_parity * h->bp_words;
-            a.mom_part = h->d_mom_part + h->bp_parity * h->mp_words;
-            HIP_CHECK(h, launch_euler_update_deferred(a, h->fin_pending ? &h->fin_args : nullptr,
-                                                      h->stream));
-            h->fin_args = a;
-            h->fin_pending = true;
-            h->bp_parity ^= 1;
+            // batched finalisation: this step's block partials and scalars go to its ring slot
+            const int slot = (int)(h->step_counter % FIN_BATCH);
+            a.block_part = h->d_block_part + slot * h->bp_words;
+            a.fin_rec = h->d_fin_rec;
+            a.fin_slot = slot;
+            HIP_CHECK(h, launch_euler_update_only(a, h->stream));
+            if (h->fin_count == 0) h->fin_first = h->step_counter;
+            h->fin_count++;
+            h->fin_base = a;
+            if (h->fin_count == FIN_BATCH) HIP_CHECK(h, flush_finalize(h));
         } else {
             HIP_CHECK(h, launch_euler_update(a, h->stream));
         }
@@ -1079,8 +1097,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double dt, int verbose, i
     }
     rc = timing ? timed_end(h, &ev) : DTX_OK;
     if (rc) return rc;
-    if (h->step_verbose.size() != (size_t)STATUS_RING) h->step_verbose.assign(STATUS_RING, 0);
-    h->step_verbose[(size_t)(h->step_counter % STATUS_RING)] = (int8_t)verbose;
+    h->step_notes[(size_t)(h->step_counter % STATUS_RING)] = StepNote{(int8_t)verbose, t_stop};
     h->cur ^= 1;
     h->step_counter++;
     if (h->step_counter - h->reported >= STATUS_RING / 2) {
@@ -1418,8 +1435,7 @@ int group_step(Group* g, const double* dt, int verbose) {
     }
     for (int m = 0; m < count; ++m) {
         dtx_handle* h = g->members[m];
-        if (h->step_verbose.size() != (size_t)STATUS_RING) h->step_verbose.assign(STATUS_RING, 0);
-        h->step_verbose[(size_t)(h->step_counter % STATUS_RING)] = (int8_t)verbose;
+        h->step_notes[(size_t)(h->step_counter % STATUS_RING)] = StepNote{(int8_t)verbose, 0.0};
         h->cur ^= 1;
         h->step_counter++;
         h->reported = h->step_counter;  // grouped steps are not echoed
@@ -1539,8 +1555,9 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
         (std::max(C * nseg, C * (int64_t)h->S) + UPDATE_THREADS - 1) / UPDATE_THREADS;
     h->bp_words = (C * nseg + nfold) * BP_WORDS;
     h->mp_words = C * nseg * h->S * 2;
-    rc |= dalloc(hp, &h->d_block_part, 2 * h->bp_words);
-    rc |= dalloc(hp, &h->d_mom_part, 2 * h->mp_words);
+    rc |= dalloc(hp, &h->d_block_part, (int64_t)FIN_BATCH * h->bp_words);
+    rc |= dalloc(hp, &h->d_mom_part, h->mp_words);
+    rc |= dalloc(hp, &h->d_fin_rec, (int64_t)FIN_BATCH * FIN_WORDS);
     rc |= dalloc(hp, &h->d_status, (int64_t)STATUS_RING * STATUS_WORDS);
     if (h->method == DTX_METHOD_FORWARD_SYMPLECTIC_EULER) {
         rc |= dalloc(hp, &h->d_fmom, n);
@@ -1582,7 +1599,7 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
         dtx_destroy(h.release());
         return fail(nullptr, rc, "%s", msg.c_str());
     }
-    h->step_verbose.assign(STATUS_RING, 0);
+    h->step_notes.assign(STATUS_RING, StepNote{});
     h->ring_host.assign((size_t)STATUS_RING * STATUS_WORDS, 0.0);
     h->scratch = scratch_address(cfg->device);  // device current (hipSetDevice above)
     *out = h.release();
@@ -2153,7 +2170,7 @@ int dtx_step(dtx_handle* h, double t_start, double t_stop, double dt, int32_t ve
         save = 0;
     }
     const int64_t idx = h->step_counter;
-    rc = enqueue_step(h, t_start, dt, verbose, save);
+    rc = enqueue_step(h, t_start, t_stop, dt, verbose, save);
     if (rc) return rc;
     if (dt_out) *dt_out = dt;
     if (save_out) *save_out = save;
@@ -2192,7 +2209,7 @@ int dtx_step_noalloc(dtx_handle* h, double t_start, double t_stop, double dt, in
         save = 0;
     }
     const int64_t idx = h->step_counter;
-    int rc = enqueue_step(h, t_start, dt, verbose, save, /*use_scratch=*/true);
+    int rc = enqueue_step(h, t_start, t_stop, dt, verbose, save, /*use_scratch=*/true);
     if (rc) return rc;
     if (dt_out) *dt_out = dt;
     if (save_out) *save_out = save;
@@ -2245,7 +2262,7 @@ int dtx_solve(dtx_handle* h, double dt_initial, int64_t n_save, const double* t_
                 dt = t_stop - t_start;
                 save = 1;
             }
-            rc = enqueue_step(h, t_start, dt, verbose, save);
+            rc = enqueue_step(h, t_start, t_stop, dt, verbose, save);
             if (rc) break;
             ++steps;
             t_start += dt;

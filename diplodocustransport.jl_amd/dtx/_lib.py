@@ -13,7 +13,8 @@ import numpy as np
 
 MAX_SPECIES = 16
 PKG_DIR = Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_DIR / "libdtx.so"
+# the library next to the package, or the one DTX_LIBRARY names (as the Julia glue reads it)
+LIB_PATH = Path(os.environ.get("DTX_LIBRARY", PKG_DIR / "libdtx.so"))
 
 DTX_OK = 0
 ERRORS = {-1: "DTX_ERR_INVALID", -2: "DTX_ERR_OOM", -3: "DTX_ERR_HIP", -4: "DTX_ERR_STATE",

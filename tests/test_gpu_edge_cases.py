@@ -49,6 +49,42 @@ def test_hand_cases_on_gpu(case, verbose):
     m.close()
 
 
+# The console lines of one step (t_start = 0, t_stop = 10, dt = 1), written out from the hand
+# cases' diagnostics and the reference's println text (src/SteppingMethods.jl:49,87-93 and
+# :222-228; Float64 values as Julia prints them, Cr of the symplectic lines to 3 digits).
+NONFINITE_LINE = "non-finite value in df calculation\n"
+PRINTED = {
+    ("euler_binary_flux", 0): "",
+    ("euler_binary_flux", 1): "Cr = 2.0, t=0.0, dt=1.0, system may be unstable\n",
+    ("euler_binary_flux", 2): "\rCr = 2.0, t=0.0, dt=1.0\n",
+    ("euler_binary_flux", 3): "Cr = 2.0,Cr_Bin = 2.0, Cr_Emi = 0.0, Cr_Flux = 0.5, t=0.0, "
+                              "t_save =10.0, dt=1.0\n",
+    ("euler_nonfinite", 0): NONFINITE_LINE,
+    ("euler_nonfinite", 1): NONFINITE_LINE,
+    ("euler_nonfinite", 2): NONFINITE_LINE + "\rCr = 0.5, t=0.0, dt=1.0\n",
+    ("euler_nonfinite", 3): NONFINITE_LINE + "Cr = 0.5,Cr_Bin = 0.0, Cr_Emi = 0.0, "
+                            "Cr_Flux = 0.5, t=0.0, t_save =10.0, dt=1.0\n",
+    ("symplectic_pflux_nan", 0): "",
+    ("symplectic_pflux_nan", 1): "",
+    ("symplectic_pflux_nan", 2): "Cr = 0.25, t=0.0, dt=1.0\n",
+    ("symplectic_pflux_nan", 3): "Cr = 0.25, Cr_Bin = 0.0, Cr_Emi = 0.0, Cr_PFlux = -0.0, "
+                                 "Cr_Momentum = -0.5, Cr_Space = 0.5,  t=0.0, t_save=10.0 , "
+                                 "dt=1.0\n",
+}
+
+
+@pytest.mark.parametrize("verbose", [0, 1, 2, 3])
+@pytest.mark.parametrize("name", ["euler_binary_flux", "euler_nonfinite", "symplectic_pflux_nan"])
+def test_printed_lines(name, verbose, capfd):
+    case = next(c for c in H.CASES if c.name == name)
+    m = _hand_struct(case)
+    capfd.readouterr()
+    m(0.0, 10.0, 1.0, verbose, stats=True)  # the stats call reads and prints the step's record
+    out = capfd.readouterr().out
+    m.close()
+    assert out == PRINTED[(name, verbose)]
+
+
 FINITE = [-1.0, -1e-12, -0.0, 0.0, H.THR10, float(np.nextafter(H.THR10, 0.0)),
           float(np.nextafter(H.THR10, 1.0)), H.THR20, float(np.nextafter(H.THR20, 0.0)), 1e-300,
           5e-324, 3.0]
