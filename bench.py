@@ -12,8 +12,8 @@ records in it; the kernel durations of the roofline come from HIP events the lib
 its own stream around every step's kernels in a separate, untimed pass.  Beside it: the FP32-MFMA
 and split-BF16 paths (speed, and their relative error and number / energy drift against FP64
 over the configuration's full step count), the same comparison at the largest configuration
-(transport_2d, where the split-BF16 contraction is MFMA-bound), the serving loop, and the FP64
-CPU oracle timed on a bounded sample on the host cores.
+(transport_2d, where the split-BF16 contraction is MFMA-bound), the serving loop, the
+collision-matrix ingest, and the FP64 CPU oracle timed on a bounded sample on the host cores.
 """
 import argparse
 import json
@@ -321,6 +321,39 @@ def run_serving(count):
                                    "step of the capacity run over its wall time"))
 
 
+def run_collision_ingest():
+    """Collision-matrix ingest (dtx_add_binary_interaction) at transport_1d's momentum grid
+    (2 species x 256 states, one cell): three interactions, 0.86 GB of DiplodocusCollisions-
+    shaped arrays, Ani.  Wall time of the three ingest calls (host C++, threads over the output
+    px bins) and of the first step after them (packing the FP64 channels into the device
+    operator, upload, the step itself); GB/s of arrays read by the ingest.  Its parity is the
+    GPU tests' (oracle at small sizes, the vectorised collision term at this size)."""
+    import os
+    import numpy as np
+    from dtx import collisions, workloads
+    from dtx.api import (BinaryMatricesStruct, EmissionMatricesStruct, FluxMatricesStruct,
+                         ForwardEulerStruct)
+    ps, blist, arrays, f0, Ap, Vol = workloads.collision_ingest()
+    BinM = collisions.BuildBinaryMatrices(ps, blist, None, arrays=arrays)
+    nbytes = sum(a.nbytes for it in BinM.interactions
+                 for a in (it.GainMatrix3, it.GainMatrix4, it.LossMatrix1, it.LossMatrix2))
+    # a handle without collisions; the interactions are added to it alone, timed
+    m = ForwardEulerStruct(ps, f0, np.zeros(ps.n), BinaryMatricesStruct(),
+                           EmissionMatricesStruct(), FluxMatricesStruct(Ap, None, None, Vol))
+    t0 = time.perf_counter()
+    m._add_interactions(m._h, ps, BinM)
+    t1 = time.perf_counter()
+    m(0.0, 1e300, 1e-6 * ps.dt0, 0)
+    m.synchronize()
+    t2 = time.perf_counter()
+    m.close()
+    return dict(workload="collision_ingest: 2 species x 256 states (p 32 x u 8), one cell, "
+                         "3 interactions, Ani", array_bytes=nbytes,
+                ingest_s=round(t1 - t0, 4), ingest_gb_per_s=round(nbytes / (t1 - t0) / 1e9, 2),
+                first_step_pack_upload_s=round(t2 - t1, 4),
+                host_threads=min(16, os.cpu_count() or 1))
+
+
 def run_flux_setup(name):
     """Set-up time of the flux operators, Ap and Vol: the host path (flux_matrices + X + P as
     CSC, ap_vol, then dtx_set_flux_csc's CSC -> ELL conversion and upload) against
@@ -390,6 +423,8 @@ def main():
     ap.add_argument("--no-uncorrected", action="store_true",
                     help="skip the FP32 / BF16x3 runs without the FP64 moment-rate correction")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip the collision-matrix ingest timing")
     ap.add_argument("--serving-requests", type=int, default=256)
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the full-length accuracy runs of the precision modes")
@@ -487,6 +522,12 @@ def main():
     largest = None
     if not args.no_largest and args.workload == "transport_1d":
         largest = run_largest(args.largest_steps, 5)
+    ingest = None
+    if not args.no_ingest and args.workload == "transport_1d":
+        log("[bench] collision-matrix ingest (3 interactions, transport_1d momentum grid)")
+        ingest = run_collision_ingest()
+        log(f"[bench] ingest {ingest['ingest_s']} s ({ingest['ingest_gb_per_s']} GB/s), first "
+            f"step with pack + upload {ingest['first_step_pack_upload_s']} s")
     serving_line = None
     if not args.no_serving and args.workload == "transport_1d":
         serving_line = run_serving(args.serving_requests)
@@ -518,6 +559,7 @@ def main():
         "roofline": roof,
         "largest_solve": largest,
         "serving": serving_line,
+        "collision_ingest": ingest,
         "cpu_baseline": cpu,
         "cpu_optimized": cpu_opt,
     }

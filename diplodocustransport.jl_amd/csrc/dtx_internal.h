@@ -277,5 +277,22 @@ int group_sync(Group* g);
 // read-only views of a handle for the server
 const dtx_config* handle_config(const dtx_handle* h);
 bool handle_has_vectors(const dtx_handle* h);
+// records msg as the handle's last error and returns code
+int handle_fail(dtx_handle* h, int code, const char* msg);
+// first state of every species in a cell (S + 1 entries)
+const int32_t* handle_species_offsets(const dtx_handle* h);
+
+// Host-side assembly of a handle's FP64 binary operator (runtime.hip).  A channel (s_out; s1 <=
+// s2) holds S[pair * No + k], pair = (a, b) of states a in s1, b in s2 (a <= b when s1 == s2:
+// upper-triangle row-major, else a * N2 + b); the loss matrix is N x N, [k + N * other].
+struct BinaryHostView {
+    double* S;
+    int32_t No, N1, N2;
+    bool tri;
+};
+int binary_host_begin(dtx_handle* h, const char* who);  // loss matrix allocated when absent
+BinaryHostView binary_host_channel(dtx_handle* h, int s_out, int s1, int s2);
+double* binary_host_loss(dtx_handle* h);
+void binary_host_end(dtx_handle* h, bool loss_nonzero);  // marks the operator for repacking
 
 }  // namespace dtx

@@ -1249,6 +1249,31 @@ const dtx_config* handle_config(const dtx_handle* h) { return &h->cfg; }
 
 bool handle_has_vectors(const dtx_handle* h) { return h->vectors_set; }
 
+int binary_host_begin(dtx_handle* h, const char* who) {
+    if (h->synth_binary)
+        return fail(h, DTX_ERR_STATE, "%s: cannot add host channels to a synthetic operator",
+                    who);
+    if (h->loss_host.empty()) h->loss_host.assign((size_t)h->N * h->N, 0.0);
+    return DTX_OK;
+}
+
+BinaryHostView binary_host_channel(dtx_handle* h, int s_out, int s1, int s2) {
+    HostChannel& ch = host_channel(h, s_out, s1, s2);
+    return BinaryHostView{ch.S.data(), ch.No, ch.N1, ch.N2, ch.kind == CH_TRI};
+}
+
+double* binary_host_loss(dtx_handle* h) { return h->loss_host.data(); }
+
+void binary_host_end(dtx_handle* h, bool loss_nonzero) {
+    h->loss_nonzero = h->loss_nonzero || loss_nonzero;
+    h->binary = true;
+    h->binary_dirty = true;
+}
+
+int handle_fail(dtx_handle* h, int code, const char* msg) { return fail(h, code, "%s", msg); }
+
+const int32_t* handle_species_offsets(const dtx_handle* h) { return h->sp_off.data(); }
+
 hipStream_t group_stream(const Group* g) { return g->stream; }
 
 int group_sync(Group* g) {

@@ -78,6 +78,15 @@ class FluxSynth(C.Structure):
                 ("axes", I32), ("periodic", I32)]
 
 
+BIN_ANI, BIN_AXI, BIN_ISO = 0, 1, 2
+
+
+class BinaryInteraction(C.Structure):
+    _fields_ = [("species", I32 * 4), ("mode", I32), ("sparse", I32), ("gain_scale", D),
+                ("loss_scale", D), ("gain3", PD), ("gain4", PD), ("loss1", PD), ("loss2", PD),
+                ("dpy", PD), ("dpz", PD)]
+
+
 # (name, restype, argtypes) for every entry point of include/dtx.h
 SIGNATURES = [
     ("dtx_create", I32, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
@@ -88,6 +97,7 @@ SIGNATURES = [
     ("dtx_set_binary_csc", I32, [C.c_void_p, I64, PI64, PI64, PD]),
     ("dtx_add_binary_gain", I32, [C.c_void_p, I32, I32, I32, PD]),
     ("dtx_add_binary_loss", I32, [C.c_void_p, I32, I32, PD]),
+    ("dtx_add_binary_interaction", I32, [C.c_void_p, C.POINTER(BinaryInteraction)]),
     ("dtx_set_binary_domain", I32, [C.c_void_p, I64, PI64]),
     ("dtx_set_emission_dense", I32, [C.c_void_p, I64, PD]),
     ("dtx_set_emission_blocks", I32, [C.c_void_p, PD]),

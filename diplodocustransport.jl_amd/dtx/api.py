@@ -56,6 +56,11 @@ class BinaryMatricesStruct:
     losses: list = field(default_factory=list)    # dtx.synth.LossChannel
     synth_seed: Optional[int] = None              # generate on the device instead
     synth_scale: float = 1.0
+    # collision arrays per interaction (dtx.collisions.BuildBinaryMatrices): the operator is
+    # assembled from them by dtx_add_binary_interaction
+    interactions: list = field(default_factory=list)
+    Bin_Mode: object = None
+    Bin_sparse: bool = False
 
 
 @dataclass
@@ -168,6 +173,8 @@ class _SteppingStruct:
                 _lib.check(self.lib.dtx_synth_binary(h, int(BinM.synth_seed),
                                                      float(BinM.synth_scale)), h,
                            "dtx_synth_binary")
+            elif BinM.interactions:
+                self._add_interactions(h, PhaseSpace, BinM)
             elif BinM.M_Bin is not None and _is_sparse(BinM.M_Bin):
                 # Bin_sparse: the CSC goes to the library as is (no dense N^2 x N copy)
                 N = PhaseSpace.n_momentum
@@ -227,6 +234,27 @@ class _SteppingStruct:
         # FP32 / BF16x3: FP64 number and energy rates of the gain term (no effect in FP64)
         _lib.check(self.lib.dtx_set_moment_correction(h, int(bool(MomentCorrection))), h,
                    "dtx_set_moment_correction")
+
+    def _add_interactions(self, h, PhaseSpace, BinM):
+        """Each resolved interaction through dtx_add_binary_interaction, in Binary_list order
+        (the Fill_M_Bin! calls of LoadMatrices_Binary, src/Collisions/LoadMatrices.jl:136)."""
+        from .collisions import Ani, angle_widths
+        dpy, dpz = angle_widths(PhaseSpace)
+        mode = (BinM.Bin_Mode or Ani()).code
+        for it in BinM.interactions:
+            arrs = [np.asfortranarray(a, dtype=np.float64).reshape(-1, order="F")
+                    for a in (it.GainMatrix3, it.GainMatrix4, it.LossMatrix1, it.LossMatrix2)]
+            x = _lib.BinaryInteraction()
+            x.species[:] = [int(v) for v in it.name_locs]
+            x.mode = mode
+            x.sparse = int(bool(BinM.Bin_sparse))
+            x.gain_scale = float(it.gain_scale)
+            x.loss_scale = float(it.loss_scale)
+            x.gain3, x.gain4, x.loss1, x.loss2 = (_lib.dptr(a) for a in arrs)
+            x.dpy, x.dpz = _lib.dptr(dpy), _lib.dptr(dpz)
+            _lib.check(self.lib.dtx_add_binary_interaction(h, C.byref(x)), h,
+                       "dtx_add_binary_interaction")
+            del arrs  # the library has read them
 
     @staticmethod
     def _flux_sum(n, FluxM):

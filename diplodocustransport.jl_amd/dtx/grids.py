@@ -131,6 +131,9 @@ class PhaseSpace:
     z_up: float = 1.0
     dt0: float = 1.0
     name_list: list = field(default_factory=list)
+    # Characteristic.Bin_Norm (src/StructsAndDictionaries.jl:41): DiplodocusCollisions' rates in
+    # the units of the step; multiplies every collision array on the way in
+    Bin_Norm: float = 1.0
 
     def __post_init__(self):
         if not self.name_list:

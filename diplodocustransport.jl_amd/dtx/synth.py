@@ -149,6 +149,32 @@ def assemble_M_Bin(ps: PhaseSpace, gains, losses) -> np.ndarray:
     return M
 
 
+def collision_arrays(ps: PhaseSpace, locs, seed: int, scale: float = 1.0):
+    """DiplodocusCollisions-shaped arrays of one interaction 1 + 2 -> 3 + 4 (species `locs`,
+    0-based): GainMatrix3 (px3+2, py3, pz3, px1, py1, pz1, px2, py2, pz2), GainMatrix4 likewise
+    for species 4, LossMatrix1 (px1, py1, pz1, px2, py2, pz2) and LossMatrix2 (2 against 1), all
+    Fortran-ordered.  Gains are lognormal values from numpy's generator seeded with (seed,
+    locs) (under/overflow px3 bins zero; host arrays only, nothing regenerates them on the
+    device); the losses are the gains summed over the outgoing states, L1[b, c] =
+    sum_a G3[a; b, c] and L2[c, b] = sum_a G4[a; b, c], so every collision conserves the number
+    of particles."""
+    dims = [(ps.species[s].p_num, ps.species[s].u_num, ps.species[s].h_num) for s in locs]
+    d1, d2, d3, d4 = dims
+    n1, n2 = int(np.prod(d1)), int(np.prod(d2))
+    rng = np.random.default_rng([seed] + [int(v) for v in locs])
+    out = []
+    for d in (d3, d4):
+        G = np.zeros((d[0] + 2,) + d[1:] + d1 + d2, order="F")
+        G[1:-1] = rng.lognormal(sigma=1.0, size=d + d1 + d2) * scale
+        out.append(G)
+    G3, G4 = out
+    g3 = G3.reshape(-1, n1, n2, order="F").sum(axis=0)
+    g4 = G4.reshape(-1, n1, n2, order="F").sum(axis=0)
+    L1 = np.asfortranarray(g3.reshape(d1 + d2, order="F"))
+    L2 = np.asfortranarray(g4.T.reshape(d2 + d1, order="F"))
+    return G3, G4, L1, L2
+
+
 def max_loss_rate(ps: PhaseSpace, losses, f_cell: np.ndarray, vol: float, ap: float) -> float:
     """max_a  Vol/Ap * sum_c L[a,c] f_c  over one cell (the stiff diagonal rate)."""
     off = ps.species_offset

@@ -153,6 +153,28 @@ def transport_1d(p_num=32, u_num=8, h_num=1, x_num=128, steps=2000, seed=None,
                     gains, losses, M, flux_spec=dict(sync_K=K, axes="x", periodic=True))
 
 
+# Interactions of the collision-ingest case: like-particle, Compton-like and photon pair
+# production, over the two species of transport_1d.
+INGEST_INTERACTIONS = (("Ele", "Ele", "Ele", "Ele"), ("Ele", "Pho", "Ele", "Pho"),
+                       ("Pho", "Pho", "Ele", "Ele"))
+
+
+def collision_ingest(p_num=32, u_num=8, h_num=1, seed=None, theta=30.0):
+    """Collision arrays at transport_1d's momentum grid (2 species x p_num u_num h_num states),
+    one cell: (PhaseSpace, Binary_list, arrays provider for BuildBinaryMatrices, f0, Ap, Vol).
+    The arrays (synth.collision_arrays) conserve number; theta keeps f above the clamp."""
+    from .collisions import BinaryStruct
+    seed = synth.config_seed(5) if seed is None else seed
+    ps = PhaseSpace(species_pair(p_num, u_num, h_num), x_num=1)
+    f0 = synth.initial_state(ps, theta=theta, seed=seed, noise=0.0)
+    Ap, Vol = synth.ap_vol(ps)
+    blist = [BinaryStruct(*n) for n in INGEST_INTERACTIONS]
+
+    def arrays(names, corrected):
+        return synth.collision_arrays(ps, tuple(ps.name_list.index(n) for n in names), seed)
+    return ps, blist, arrays, f0, Ap, Vol
+
+
 def precision_box(p_num=64, u_num=8, h_num=2, num_species=3, steps=1000, seed=None):
     seed = synth.config_seed(2) if seed is None else seed
     sp = [Species("Ele", 1.0, -1.0, p_num, u_num, h_num),

@@ -157,10 +157,17 @@ end
 Creates the libdtx simulation for `method` and uploads its operators.  Called from the
 `elseif Backend isa HIPBackend` branch of the stepping-struct constructors; afterwards
 `method.f` is a DtxState and `method.HIP` holds the handle.
+
+With `binary_arrays` (one `(name_locs, GainMatrix3, GainMatrix4, LossMatrix1, LossMatrix2,
+GainScale, LossScale)` per interaction, as LoadMatrices_Binary forms them) the binary operator
+is assembled on the handle from the collision arrays (`binary_mode = (mode=Ani(), sparse=false)`
+as Bin_Mode / Bin_sparse) and `BinM.M_Bin` is not read: BuildBinaryMatrices' N^2 x N array need
+not be built.
 """
 function dtx_attach!(method, PhaseSpace, Initial::Vector{Float64}, Injection::Vector{Float64},
                      BinM, EmiM, FluxM, backend::HIPBackend; symplectic::Bool=false,
-                     flux_synth=nothing)
+                     flux_synth=nothing, binary_arrays=nothing,
+                     binary_mode=(mode=nothing, sparse=false))
     Momentum = PhaseSpace.Momentum
     Space = PhaseSpace.Space
     S = length(PhaseSpace.name_list)
@@ -195,14 +202,25 @@ function dtx_attach!(method, PhaseSpace, Initial::Vector{Float64}, Injection::Ve
         set_csc(0, FluxM.X_Flux + FluxM.P_Flux)   # F_Flux (src/SteppingStructs.jl:60)
     end
     if !isempty(BinM.Binary_list)
-        size(BinM.M_Bin) == (N^2, N) || error("M_Bin must be (N^2 x N) with N = $N")
-        if BinM.M_Bin isa AbstractSparseMatrix
+        if binary_arrays !== nothing
+            # the collision arrays of every interaction, in Binary_list order: tuples
+            # (name_locs, GainMatrix3, GainMatrix4, LossMatrix1, LossMatrix2, GainScale,
+            # LossScale) as LoadMatrices_Binary forms them, assembled on the handle
+            for (locs, G3, G4, L1, L2, gs, ls) in binary_arrays
+                dtx_add_binary_interaction!(h, PhaseSpace, locs, G3, G4, L1, L2;
+                                            mode=get(binary_mode, :mode, nothing),
+                                            sparse=get(binary_mode, :sparse, false),
+                                            GainScale=gs, LossScale=ls)
+            end
+        elseif BinM.M_Bin isa AbstractSparseMatrix
+            size(BinM.M_Bin) == (N^2, N) || error("M_Bin must be (N^2 x N) with N = $N")
             # Bin_sparse: the CSC as Julia holds it (src/SteppingStructs.jl:178-179)
             B = SparseMatrixCSC{Float64,Int64}(BinM.M_Bin)
             dtx_check(h, ccall((:dtx_set_binary_csc, libdtx), Cint,
                                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
                                h.ptr, Int64(N), B.colptr, B.rowval, B.nzval))
         else
+            size(BinM.M_Bin) == (N^2, N) || error("M_Bin must be (N^2 x N) with N = $N")
             M = Matrix{Float64}(BinM.M_Bin)
             dtx_check(h, ccall((:dtx_set_binary_dense, libdtx), Cint,
                                (Ptr{Cvoid}, Int64, Ptr{Float64}), h.ptr, Int64(N), M))
@@ -329,3 +347,54 @@ function dtx_synth_flux_handle!(h::DtxHandle, xr, yr, zr, pr_list, ur_list, hr_l
     end
     return nothing
 end
+
+# dtx_binary_interaction of include/dtx.h (field order and types as in C)
+struct DtxBinaryInteraction
+    species::NTuple{4,Int32}
+    mode::Int32
+    sparse::Int32
+    gain_scale::Float64
+    loss_scale::Float64
+    gain3::Ptr{Float64}
+    gain4::Ptr{Float64}
+    loss1::Ptr{Float64}
+    loss2::Ptr{Float64}
+    dpy::Ptr{Float64}
+    dpz::Ptr{Float64}
+end
+
+const DTX_BIN_MODES = Dict(:Ani => Int32(0), :Axi => Int32(1), :Iso => Int32(2))
+
+"""
+    dtx_add_binary_interaction!(h, PhaseSpace, name_locs, GainMatrix3, GainMatrix4,
+                                LossMatrix1, LossMatrix2; mode=Ani(), sparse=false,
+                                GainScale=1.0, LossScale=1.0)
+
+The `Fill_M_Bin!` call of `LoadMatrices_Binary` (src/Collisions/LoadMatrices.jl:136) on the
+handle: adds one interaction's gain and loss terms, with the Ani / Axi / Iso averaging, straight
+to the packed operator (no N^2 x N `M_Bin`).  `name_locs` are 1-based as in the reference;
+`GainScale` / `LossScale` already include `Bin_Norm` (LoadMatrices.jl:129-133).  `h` is a
+`DtxHandle` or a method whose `HIP` field holds one.
+"""
+function dtx_add_binary_interaction!(h::DtxHandle, PhaseSpace, name_locs, GainMatrix3,
+                                     GainMatrix4, LossMatrix1, LossMatrix2; mode=nothing,
+                                     sparse::Bool=false, GainScale=1.0, LossScale=1.0)
+    m = mode === nothing ? :Ani : nameof(typeof(mode))
+    haskey(DTX_BIN_MODES, m) || error("Bin_Mode must be Ani(), Axi() or Iso()")
+    Grids = PhaseSpace.Grids
+    bufs = (Array{Float64}(GainMatrix3), Array{Float64}(GainMatrix4),
+            Array{Float64}(LossMatrix1), Array{Float64}(LossMatrix2),
+            Vector{Float64}(reduce(vcat, Grids.dpy_list)),
+            Vector{Float64}(reduce(vcat, Grids.dpz_list)))
+    GC.@preserve bufs begin
+        x = DtxBinaryInteraction(ntuple(i -> Int32(name_locs[i] - 1), 4), DTX_BIN_MODES[m],
+                                 Int32(sparse ? 1 : 0), Float64(GainScale), Float64(LossScale),
+                                 map(pointer, bufs)...)
+        dtx_check(h, ccall((:dtx_add_binary_interaction, libdtx), Cint,
+                           (Ptr{Cvoid}, Ref{DtxBinaryInteraction}), h.ptr, x))
+    end
+    return nothing
+end
+
+dtx_add_binary_interaction!(method, args...; kw...) =
+    dtx_add_binary_interaction!(method.HIP, args...; kw...)

@@ -136,6 +136,40 @@ int dtx_add_binary_gain(dtx_handle* h, int32_t s_out, int32_t s1, int32_t s2, co
 /* Loss: dB[a] -= f_a * sum_{c in s2} L[a + N1*c] f_c for a in s1, which is what
  * LossMatrix_to_M_Bin! contributes (-L/2 at (a,a,c) and (a,c,a), :206-284). */
 int dtx_add_binary_loss(dtx_handle* h, int32_t s1, int32_t s2, const double* L);
+
+/* Bin_Mode: how the collision arrays are averaged over the angles (src/Collisions/
+ * BuildBinaryMatrices.jl:7): ANI keeps them, AXI averages over phi (pz), ISO over u and phi. */
+enum dtx_bin_mode { DTX_BIN_ANI = 0, DTX_BIN_AXI = 1, DTX_BIN_ISO = 2 };
+
+/* One binary interaction name1 + name2 -> name3 + name4 as DiplodocusCollisions' arrays hold it
+ * (the Output of BinaryFileLoad_Matrix, src/Collisions/LoadMatrices.jl:122-133).  Arrays are
+ * Float64 in Julia's (column-major) memory order; px/py/pz counts are those of dtx_config. */
+typedef struct {
+    int32_t species[4];   /* name1_loc .. name4_loc, 0-based                                  */
+    int32_t mode;         /* enum dtx_bin_mode                                                */
+    int32_t sparse;       /* 1: Bin_sparse assembly (ANI gains below 1e-16 of their column's
+                             maximum over px3 are dropped, BuildBinaryMatrices.jl:163-173)   */
+    double gain_scale;    /* GainScale * Bin_Norm: every gain entry is multiplied by it first */
+    double loss_scale;    /* LossScale * Bin_Norm                                             */
+    const double* gain3;  /* GainMatrix3 (px3+2, py3, pz3, px1, py1, pz1, px2, py2, pz2): the
+                             first and last px3 bins (under/overflow) are not used           */
+    const double* gain4;  /* GainMatrix4 (px4+2, py4, pz4, px1, py1, pz1, px2, py2, pz2)      */
+    const double* loss1;  /* LossMatrix1 (px1, py1, pz1, px2, py2, pz2)                       */
+    const double* loss2;  /* LossMatrix2 (px2, py2, pz2, px1, py1, pz1)                       */
+    const double* dpy;    /* u bin widths of every species in order, py_num[s] each
+                             (Grids.dpy_list, deltaVector of the u bounds)                   */
+    const double* dpz;    /* phi bin widths, pz_num[s] each (Grids.dpz_list)                  */
+} dtx_binary_interaction;
+
+/* Adds one interaction's gain and loss terms to the handle's operator, as Fill_M_Bin! adds them
+ * to M_Bin (src/Collisions/BuildBinaryMatrices.jl:81-112, GainMatrix_to_M_Bin! :114-204,
+ * LossMatrix_to_M_Bin! :206-284): val * w / 2 at M[a, b, c] and M[a, c, b] for every gain entry
+ * and -val * w / 2 at M[a, a, c] and M[a, c, a] for every loss entry, with the ANI / AXI / ISO
+ * averages and weights.  The terms go straight into the packed operator's FP64 channels and loss
+ * matrix (no N^2 x N array); calls accumulate, in the order of Binary_list.  Replaces the
+ * Fill_M_Bin! call of LoadMatrices_Binary (src/Collisions/LoadMatrices.jl:136) and the upload
+ * of the M_Bin it fills. */
+int dtx_add_binary_interaction(dtx_handle* h, const dtx_binary_interaction* x);
 /* BinM.Domain: 0-based cell offsets that carry binary interactions; count < 0 means `nothing`
  * (all cells).  (src/Collisions/CollisionDomains.jl:6-38, src/SteppingMethods.jl:369) */
 int dtx_set_binary_domain(dtx_handle* h, int64_t count, const int64_t* domain);

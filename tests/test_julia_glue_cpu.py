@@ -1,0 +1,69 @@
+"""The Julia glue (julia/DiplodocusTransportHIP.jl) cannot run here (no Julia in the image), so
+its contract with include/dtx.h is checked as text: every ccall'd symbol is declared in the
+header, and every Julia struct mirror lists the C struct's fields in the same order with the
+same types (Int32 <-> int32_t, Float64 <-> double, Ptr{Float64} <-> const double*,
+NTuple{K,T} <-> T name[K]), which gives the same memory layout."""
+import re
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+GLUE = ROOT / "diplodocustransport.jl_amd" / "julia" / "DiplodocusTransportHIP.jl"
+HEADER = ROOT / "include" / "dtx.h"
+
+MIRRORS = {"DtxConfig": "dtx_config", "DtxStepStats": "dtx_step_stats",
+           "DtxFluxSynth": "dtx_flux_synth", "DtxBinaryInteraction": "dtx_binary_interaction"}
+
+
+def c_fields(header, name):
+    m = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + r";", header)
+    assert m, name
+    body = re.sub(r"/\*.*?\*/", "", m.group(1), flags=re.S)
+    out = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        ctype, names = re.match(r"((?:const )?\w+\*?)\s+(.*)", decl).groups()
+        for nm in names.split(","):
+            nm = nm.strip()
+            arr = re.match(r"(\w+)\[(\w+)\]", nm)
+            if arr:
+                out.append((arr.group(1), ctype, arr.group(2)))
+            else:
+                out.append((nm.lstrip("*"), ctype + ("*" if nm.startswith("*") else ""), None))
+    return out
+
+
+def julia_fields(glue, name):
+    m = re.search(r"\nstruct " + name + r"\n(.*?)\nend", glue, re.S)
+    assert m, name
+    out = []
+    for line in m.group(1).splitlines():
+        line = line.strip()
+        if line:
+            field, jtype = line.split("::")
+            out.append((field, jtype))
+    return out
+
+
+JL_OF_C = {"int32_t": "Int32", "double": "Float64", "const double*": "Ptr{Float64}"}
+
+
+def test_struct_mirrors_match_the_header():
+    header, glue = HEADER.read_text(), GLUE.read_text()
+    consts = {"DTX_MAX_SPECIES": "DTX_MAX_SPECIES", "4": "4"}
+    for jl, cname in MIRRORS.items():
+        cf, jf = c_fields(header, cname), julia_fields(glue, jl)
+        assert [f for f, _, _ in cf] == [f for f, _ in jf], jl
+        for (f, ctype, count), (_, jtype) in zip(cf, jf):
+            want = JL_OF_C[ctype]
+            if count is not None:
+                want = f"NTuple{{{consts[count]},{want}}}"
+            assert jtype == want, (jl, f, jtype, want)
+
+
+def test_every_ccall_is_declared():
+    header, glue = HEADER.read_text(), GLUE.read_text()
+    declared = set(re.findall(r"\b(dtx_\w+)\s*\(", header))
+    called = set(re.findall(r"ccall\(\(:(\w+),", glue))
+    assert called and called <= declared, called - declared

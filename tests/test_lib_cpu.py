@@ -48,7 +48,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
     structs = {"dtx_config": _lib.Config, "dtx_step_stats": _lib.StepStats,
                "dtx_work_info": _lib.WorkInfo, "dtx_kernel_times": _lib.KernelTimes,
                "dtx_server_config": _lib.ServerConfig, "dtx_request": _lib.Request,
-               "dtx_request_result": _lib.RequestResult, "dtx_flux_synth": _lib.FluxSynth}
+               "dtx_request_result": _lib.RequestResult, "dtx_flux_synth": _lib.FluxSynth,
+               "dtx_binary_interaction": _lib.BinaryInteraction}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
              'int main(void) {']
     for cname, py in structs.items():
