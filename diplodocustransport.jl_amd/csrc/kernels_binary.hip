@@ -771,6 +771,179 @@ __device__ __forceinline__ void wide_item(const BinWork& w, SmemWide<NS>& sm) {
     }
 }
 
+// The wide item with the operator loaded straight into LDS (global_load_lds_dwordx4: lane l of
+// a wave's load lands at LDS base + 16 l, so each wave moves one 1 KB fragment-order block of
+// the packed operator per instruction, with no registers in between).  Three operator stages
+// let the loads of chunk ch + 2 be in flight while chunk ch computes and chunk ch + 1 arrives;
+// with two workgroups per CU (NS = 2) a register stage of that depth does not fit in the 256
+// VGPRs of two waves per SIMD.  The state split (B, from L2) and the row multipliers f_i come
+// the same way, B one chunk ahead: into a single buffer, free once every wave has read its
+// operands, at NS = 2 (two stages would not fit two workgroups in the LDS), and into two
+// alternating buffers at NS = 4.  Per chunk each wave issues 4 operator loads (16 bytes a lane),
+// 1 f_i load (4 bytes a lane: four waves move the 1 KB of a chunk's f_i, and at NS = 4 the
+// other four load a second copy, so every wave issues the same loads; with the 2-byte form,
+// 2 bytes a lane over eight waves, the multipliers came out wrong) and 8 / NS state loads, so
+// its wait counts
+// are uniform: at the top of an iteration only the operator group of the next chunk (5 loads)
+// may be in flight.
+template <int NS>
+struct SmemWideDma {
+    static constexpr int NB = NS == 4 ? 2 : 1;   // state buffers
+    union {
+        struct {
+            u32x4 A0[2][4 * NS * 64];   // stage 0: hi, lo; 4 NS blocks (slices x row blocks)
+            u32x4 A1[2][4 * NS * 64];
+            u32x4 A2[2][4 * NS * 64];
+            double F0[WIDE * NS / 2];   // f_i of the chunk's row per cell of the tile (NS / 2
+            double F1[WIDE * NS / 2];   // copies, the first one read)
+            double F2[WIDE * NS / 2];
+            u32x4 B[NB][2][8 * 64];     // hi, lo; cell blocks 0..7
+        } s;
+        float T[2 * NS][16 * EPI_PITCH];   // epilogue: each wave's 16 cells x 64 rows
+    };
+    uint32_t ctab[MAX_RANGE];
+};
+
+#define DTX_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+
+template <int BYTES>
+__device__ __forceinline__ void dma_lds(const GLB void* src, void* lds) {
+    static_assert(BYTES == 16 || BYTES == 4, "16- or 4-byte lanes");
+    if constexpr (BYTES == 16) __builtin_amdgcn_global_load_lds((GLB void*)src, DTX_LDS(lds), 16, 0, 0);
+    else __builtin_amdgcn_global_load_lds((GLB void*)src, DTX_LDS(lds), 4, 0, 0);
+}
+
+template <int NS>
+__device__ __forceinline__ void wide_item_dma(const BinWork& w, SmemWideDma<NS>& sm) {
+    using Sm = SmemWideDma<NS>;
+    constexpr int NB = Sm::NB;
+    constexpr int BQ = 8 / NS;        // state blocks a wave
+    uint32_t* ctab = sm.ctab;
+    const ItemView v = load_item(w, WIDE, 2);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wk = wave % NS, wc = wave / NS;   // this wave: rows 64 wk.., cells 64 wc..
+    const int64_t slice = w.a_ktile_stride;     // bf16 elements per packed 64-row slice
+    const int kt0 = NS * w.ktile;
+    // this wave's operator loads: half (hi / lo) wave / NS of slice wave % NS, its 4 row blocks
+    const int dh = wave / NS, ds = wave % NS;
+    const bool present = (kt0 + ds) * TK < v.rows;   // else slice 0 again (rows never written)
+    const GLB u32x4* Ag = (const GLB u32x4*)((const GLB uint16_t*)((const GLB char*)w.A) +
+                                             (int64_t)(present ? kt0 + ds : kt0) * slice +
+                                             (int64_t)v.c_begin * TK * KC * 2) +
+                          dh * 256 + lane;
+    // this wave's state loads: half wave / NS, cell blocks BQ (wave % NS) .. + BQ - 1
+    const GLB u32x4* Bg = (const GLB u32x4*)((const GLB uint16_t*)v.B + (dh ? v.b_lo : 0)) +
+                          ((int64_t)(v.c0 >> 4) + BQ * ds) * 64 + lane;
+    load_ctab(v, ctab);
+
+    auto issue_a = [&](int ch, u32x4 (&A)[2][4 * NS * 64], double* F) {
+        const GLB u32x4* src = Ag + (int64_t)ch * (TK * KC * 2 / 8);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dma_lds<16>(src + r * 64, &A[dh][(4 * ds + r) * 64]);
+        // wave w: bytes 256 (w % 4) .. + 255 of f_i into copy w / 4
+        const uint32_t ce = ctab[ch];
+        const GLB char* fsrc = (const GLB char*)(v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0) +
+                               256 * (wave & 3) + 4 * lane;
+        dma_lds<4>(fsrc, (char*)F + 1024 * (wave >> 2) + 256 * (wave & 3));
+    };
+    auto issue_b = [&](int ch) {
+        const int j0 = (int)(ctab[ch] >> 16);
+        const GLB u32x4* src = Bg + (int64_t)(j0 >> 5) * v.ldb * 64;
+#pragma unroll
+        for (int q = 0; q < BQ; ++q)
+            dma_lds<16>(src + q * 64, &sm.s.B[NB == 1 ? 0 : (ch & 1)][dh][(BQ * ds + q) * 64]);
+    };
+
+    f4 acc[4][4], tot[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = tot[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+    const int nch = v.c_end - v.c_begin;
+    // one chunk: wait for its loads (only the next chunk's operator group may stay in flight),
+    // read the operands, issue the loads of the state one chunk and of the operator two chunks
+    // ahead (with one state buffer, after a second barrier that frees it), then the MFMAs and
+    // the row-end fold tot += f_i * acc (FP32)
+    auto step = [&](int ch, u32x4 (&A)[2][4 * NS * 64], const double* F,
+                    u32x4 (&Anext)[2][4 * NS * 64], double* Fnext) {
+        if (ch + 1 < nch) asm volatile("s_waitcnt vmcnt(5)\n\ts_barrier" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        const int bb = NB == 1 ? 0 : (ch & 1);
+        bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ah[r] = __builtin_bit_cast(bf16x8, A[0][(4 * wk + r) * 64 + lane]);
+            al[r] = __builtin_bit_cast(bf16x8, A[1][(4 * wk + r) * 64 + lane]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            bh[c] = __builtin_bit_cast(bf16x8, sm.s.B[bb][0][(4 * wc + c) * 64 + lane]);
+            bl[c] = __builtin_bit_cast(bf16x8, sm.s.B[bb][1][(4 * wc + c) * 64 + lane]);
+        }
+        if constexpr (NB == 1) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if (ch + 1 < nch) issue_b(ch + 1);
+        if (ch + 2 < nch) issue_a(ch + 2, Anext, Fnext);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+            }
+        const uint32_t ip = ctab[ch] & 0xFFFFu;
+        if (ch + 1 >= nch || (ctab[ch + 1] & 0xFFFFu) != ip) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float fi = (float)F[64 * wc + 16 * c + (lane & 15)];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        tot[r][c][e] = __builtin_fmaf(fi, acc[r][c][e], tot[r][c][e]);
+                    acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+        }
+    };
+    if (nch > 0) {
+        issue_a(0, sm.s.A0, sm.s.F0);
+        issue_b(0);
+        if (nch > 1) issue_a(1, sm.s.A1, sm.s.F1);
+    }
+    // chunk ch uses stage ch % 3 and loads chunk ch + 2 into stage (ch + 2) % 3 (unrolled by
+    // three, so every stage is a fixed LDS array and the compiler's own waits can tell them
+    // apart)
+    for (int ch = 0; ch < nch; ch += 3) {
+        step(ch, sm.s.A0, sm.s.F0, sm.s.A2, sm.s.F2);
+        if (ch + 1 >= nch) break;
+        step(ch + 1, sm.s.A1, sm.s.F1, sm.s.A0, sm.s.F0);
+        if (ch + 2 >= nch) break;
+        step(ch + 2, sm.s.A2, sm.s.F2, sm.s.A1, sm.s.F1);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();   // the epilogue's T aliases the stages
+    // write the item's FP32 partial sums through LDS, as wide_item does
+    float* tw = sm.T[wave];
+    const int krow = kt0 * TK + 64 * wk + lane;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                tw[(lane & 15) * EPI_PITCH + 16 * r + 4 * (lane >> 4) + e] = tot[r][c][e];
+        __syncthreads();
+        for (int q = 0; q < 16; ++q) {
+            const int cell = v.c0 + 64 * wc + 16 * c + q;
+            if (cell < v.cells && krow < v.rows)
+                ((GLB float*)v.part)[(int64_t)cell * v.N + v.row_base + krow] =
+                    tw[q * EPI_PITCH + lane];
+        }
+        __syncthreads();
+    }
+}
+
 // LOSS items take the 256-thread FP64 body.  With eight waves the upper four end at once, and
 // the barriers of the body then wait for the four that remain (a workgroup barrier counts only
 // waves that have not ended).
@@ -780,13 +953,14 @@ __global__ __launch_bounds__(128 * NS) void bin_bf16x3_wide_kernel(
     __shared__ union {
         SmemF64<64> d;
         SmemWide<NS> b;
+        SmemWideDma<NS> m;
     } sm;
     const BinWork& w = items[blockIdx.x];
     if (w.kind >= CH_LOSS) {
         if (threadIdx.x >= 256) return;
         f64_item<64>(w, sm.d);
     } else {
-        wide_item<NS>(w, sm.b);
+        wide_item_dma<NS>(w, sm.m);
     }
 }
 
