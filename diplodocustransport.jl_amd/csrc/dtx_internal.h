@@ -181,6 +181,13 @@ struct CorrectArgs {
 
 // ---- kernel launchers (kernels_*.hip) ----------------------------------------------------
 hipError_t launch_gain_correct(const CorrectArgs& a, hipStream_t stream);
+// Corrected Forward Euler in one launch (slot sums, coefficients, update; bitwise the same as
+// launch_gain_correct + launch_euler_update), one workgroup per (cell, species), for layouts
+// fused_update_fits accepts: four threads per state in the sums, every species starting on an
+// update segment and at most 16 segments long.  finalize: also launch the step's finalisation.
+bool fused_update_fits(int N, int S, const int32_t* species_off, int tps);
+hipError_t launch_euler_update_fused(const UpdateArgs& a, const CorrectArgs& ca, bool finalize,
+                                     hipStream_t stream);
 // Grouped forms (serving): member g of the table owns blocks [start[g], start[g+1]) of the
 // launch, which runs the member's own grid with its own arithmetic.  The coefficient launch
 // covers the members whose update grid is too large to derive them per workgroup (coef set).

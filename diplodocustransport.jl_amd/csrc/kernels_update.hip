@@ -52,13 +52,14 @@ __device__ inline double wave_min(double v) {
     return v;
 }
 
+template <int NT = UB>
 __device__ int block_or(int v, int* red) {
     const int tid = threadIdx.x;
     v = __any(v) ? 1 : 0;
     if ((tid & 63) == 0) red[tid >> 6] = v;
     __syncthreads();
     int r = 0;
-    for (int w = 0; w < UB / 64; ++w) r |= red[w];
+    for (int w = 0; w < NT / 64; ++w) r |= red[w];
     __syncthreads();
     return r;
 }
@@ -193,9 +194,10 @@ __device__ inline void wave_reduce_batch(double* v, uint32_t summask) {
     }
 }
 
-__device__ inline void wave_put(double x, int i, bool is_sum, double* red) {
+// t: the thread's index in its group of UB threads (red: that group's array)
+__device__ inline void wave_put(double x, int i, bool is_sum, double* red, int t = threadIdx.x) {
     const double w = is_sum ? wave_sum(x) : wave_min(x);
-    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * RED_VALUES + i] = w;
+    if ((t & 63) == 0) red[(t >> 6) * RED_VALUES + i] = w;
 }
 
 __device__ inline double wave_combine(int t, bool is_sum, const double* red) {
@@ -209,21 +211,39 @@ __device__ inline double wave_combine(int t, bool is_sum, const double* red) {
 // the moment partials of the gain sums (workgroups of the sums launch, lane-strided in order)
 // and the moment-rate item sums (lane-strided in list order), one butterfly, then the 2x2
 // system.  Every update workgroup of the cell runs the same arithmetic, so they agree exactly.
-__device__ inline void correction_coef(const UpdateArgs& a, int c, int s, double* out) {
-    const int lane = threadIdx.x & 63;
-    const int64_t p = (int64_t)c * a.S + s;
-    const int ns = a.species_off[s + 1] - a.species_off[s];
-    const int nz = (ns + a.zspan - 1) / a.zspan;
-    double v[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int z = lane; z < nz; z += 64)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) v[i] = v[i] + a.cpart[(p * a.zb + z) * 3 + i];
+// v[3 + m], m = 0 (number) and 1 (energy): the lane's moment-rate item sums of (cell c,
+// species s), lane-strided in list order.
+__device__ inline void mom_rate_sums(const UpdateArgs& a, int c, int s, int lane, double* v) {
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         const int t0 = a.mom_off[2 * s + m], t1 = a.mom_off[2 * s + m + 1];
         for (int t = t0 + lane; t < t1; t += 64)
             v[3 + m] = v[3 + m] + a.mpart[(int64_t)a.mom_list[t] * a.ldc + c];
     }
+}
+
+// v[0..2]: the lane's sums of the gain-sum moment partials cp[z][3], z = lane, lane + 64, ...
+__device__ inline void cpart_sums(const double* cp, int nz, int lane, double* v) {
+    for (int z = lane; z < nz; z += 64)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] = v[i] + cp[z * 3 + i];
+}
+
+__device__ inline void coef_solve(const UpdateArgs& a, double* v, int lane, double* out);
+
+__device__ inline void correction_coef(const UpdateArgs& a, int c, int s, double* out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)c * a.S + s;
+    const int ns = a.species_off[s + 1] - a.species_off[s];
+    const int nz = (ns + a.zspan - 1) / a.zspan;
+    double v[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    cpart_sums(a.cpart + p * a.zb * 3, nz, lane, v);
+    mom_rate_sums(a, c, s, lane, v);
+    coef_solve(a, v, lane, out);
+}
+
+// One butterfly over the wave, then (lane 0) the 2x2 system into out[0..1].
+__device__ inline void coef_solve(const UpdateArgs& a, double* v, int lane, double* out) {
     wave_reduce_batch(v, 0xFFu);
     if (lane != 0) return;
     const double S0 = v[0], S1 = v[1], S2 = v[2], RN = v[3], RE = v[4];
@@ -269,33 +289,37 @@ __global__ __launch_bounds__(UB) void correct_coef_kernel(UpdateArgs a) {
 // Per-species moment partials of this block's new states and the per-block partials (sums and
 // Cr minima), in one fused reduction.  Thread (tid & 3) == 0 of each state holds its values;
 // the finalize kernels fold the partials in a fixed order.
+// t: the thread's index in its group of UB threads (red: the group's array); write: the group
+// owns a segment (every group of the workgroup takes part in the barrier).
 __device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, bool mine, int k,
                              double fnew, double* red, double sumdf, double sumf, double cr,
-                             double crb, double cre, double crf, double crm, double crs) {
-    const int tid = threadIdx.x;
+                             double crb, double cre, double crf, double crm, double crs,
+                             int t = threadIdx.x, bool write = true) {
+    const int tid = t;
     const int64_t b = (int64_t)c * nseg + seg;
     const int S = a.S;
     const int sp = mine ? a.state_species[k] : -1;
     const double ew = (mine && a.eweight) ? a.eweight[k] : 0.0;
     for (int s = 0; s < S; ++s) {
-        wave_put(s == sp ? fnew : 0.0, 2 * s, true, red);
-        wave_put((s == sp && a.eweight) ? fnew * ew : 0.0, 2 * s + 1, true, red);
+        wave_put(s == sp ? fnew : 0.0, 2 * s, true, red, t);
+        wave_put((s == sp && a.eweight) ? fnew * ew : 0.0, 2 * s + 1, true, red, t);
     }
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const int w0 = 2 * S;
-    wave_put(sumdf, w0 + BP_SUMDF, true, red);
-    wave_put(sumf, w0 + BP_SUMF, true, red);
+    wave_put(sumdf, w0 + BP_SUMDF, true, red, t);
+    wave_put(sumf, w0 + BP_SUMF, true, red, t);
     if (want_cr) {
-        wave_put(cr, w0 + BP_CR, false, red);
+        wave_put(cr, w0 + BP_CR, false, red, t);
         if (a.verbose == 3) {
-            wave_put(crb, w0 + BP_CR_BIN, false, red);
-            wave_put(cre, w0 + BP_CR_EMI, false, red);
-            wave_put(crf, w0 + BP_CR_FLUX, false, red);
-            wave_put(crm, w0 + BP_CR_MOM, false, red);
-            wave_put(crs, w0 + BP_CR_SPACE, false, red);
+            wave_put(crb, w0 + BP_CR_BIN, false, red, t);
+            wave_put(cre, w0 + BP_CR_EMI, false, red, t);
+            wave_put(crf, w0 + BP_CR_FLUX, false, red, t);
+            wave_put(crm, w0 + BP_CR_MOM, false, red, t);
+            wave_put(crs, w0 + BP_CR_SPACE, false, red, t);
         }
     }
     __syncthreads();
+    if (!write) return;
     if (tid < w0) {
         a.mom_part[(b * S + (tid >> 1)) * 2 + (tid & 1)] = wave_combine(tid, true, red);
     } else if (tid < w0 + BP_WORDS) {
@@ -512,15 +536,16 @@ __device__ void finalize_single(const UpdateArgs& a, int nseg, double* red,
 
 // Load the whole cell into LDS (needed by the emission GEMV and the Domain all-zero test) and
 // return whether any entry is non-zero.
+template <int NT = UB>
 __device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
     const int64_t base = (int64_t)c * a.N;
     int nz = 0;
-    for (int k = threadIdx.x; k < a.N; k += UB) {
+    for (int k = threadIdx.x; k < a.N; k += NT) {
         const double v = a.f[base + k];
         fc[k] = v;
         nz |= (v != 0.0);
     }
-    return block_or(nz, ired);  // also orders the fc writes before use
+    return block_or<NT>(nz, ired);  // also orders the fc writes before use
 }
 
 // ----------------------------------------------------------------------- ForwardEuler ------
@@ -546,11 +571,16 @@ __device__ inline void ell_issue(const UpdateArgs& a, int64_t r, int q, bool min
     }
 }
 
-// Adds the products of slots [s0, s0 + 4 ELL_LANE) to acc on lane q = 0 of each state.
-__device__ inline double ell_sum(const UpdateArgs& a, const EllLane& L, int s0, double acc) {
-    double x[ELL_LANE];
+// The gathered f of the lane's slots.
+__device__ inline void ell_gather(const UpdateArgs& a, const EllLane& L, double* x) {
 #pragma unroll
     for (int j = 0; j < ELL_LANE; ++j) x[j] = L.c[j] >= 0 ? a.f[L.c[j]] : 0.0;
+}
+
+// Adds the products of slots [s0, s0 + 4 ELL_LANE) to acc on lane q = 0 of each state (x: the
+// gathered f of the lane's slots).
+__device__ inline double ell_sum_x(const EllLane& L, const double* x, int s0, int width,
+                                   double acc) {
     double p[ELL_LANE];
 #pragma unroll
     for (int j = 0; j < ELL_LANE; ++j) p[j] = L.c[j] >= 0 ? L.v[j] * x[j] : 0.0;
@@ -561,9 +591,15 @@ __device__ inline double ell_sum(const UpdateArgs& a, const EllLane& L, int s0, 
         for (int qq = 0; qq < 4; ++qq) {
             const double pv = __shfl(p[j], grp | qq, 64);
             const int cv = __shfl(L.c[j], grp | qq, 64);
-            if (s0 + 4 * j + qq < a.ell_width && cv >= 0) acc = acc + pv;
+            if (s0 + 4 * j + qq < width && cv >= 0) acc = acc + pv;
         }
     return acc;
+}
+
+__device__ inline double ell_sum(const UpdateArgs& a, const EllLane& L, int s0, double acc) {
+    double x[ELL_LANE];
+    ell_gather(a, L, x);
+    return ell_sum_x(L, x, s0, a.ell_width, acc);
 }
 
 __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg, double* smem,
@@ -1020,6 +1056,211 @@ hipError_t launch_coef_group(const UpdateArgs* table, const int32_t* start, int 
     if (count <= 0 || total_blocks <= 0) return hipSuccess;
     correct_coef_group_kernel<<<total_blocks, UB, 0, stream>>>(table, start, count);
     return hipGetLastError();
+}
+
+// ------------------------------------------ ForwardEuler with the correction fused in -----
+// One workgroup of FUSED_THREADS per (cell c, species s), for handles whose species all start
+// on a segment boundary (a multiple of UPDATE_STATES) and hold at most FUSED_SEGMENTS segments,
+// with four threads per state in the slot sums.  The workgroup is FUSED_GROUPS groups of UB
+// threads; pass p gives group G the species' segment z = FUSED_GROUPS p + G, with the thread
+// layout of both launches it replaces (state z 64 + t / 4, thread q = t % 4).
+//   phase 1: each state's gain and loss sums (correct_sums_body's order) and the segment's
+//            gain-moment partials (into LDS instead of cpart);
+//   phase 2: wave 0 solves the species' (alpha, beta) from the same partials and the
+//            moment-rate item sums (correction_coef's order);
+//   phase 3: each segment's update as euler_update_body evaluates it, with the sums from
+//            registers and its block partials written per segment.
+// Every result is bitwise that of correct_sums_kernel + euler_update_kernel.  The state's own
+// loads and its flux row are issued at the start, so they are in flight with the slot loads,
+// and the row's gathers right after the slot sums.  The chains of a workgroup do not overlap
+// those of the next round on its CU, so the runtime takes this form for grids of at most one
+// workgroup per CU (at transport_1d's 128 cells x 2 species it replaced 25.8 us of three
+// launches by 22 us; at 2048 workgroups it was slower than them).
+constexpr int FUSED_THREADS = 1024;
+constexpr int FUSED_GROUPS = FUSED_THREADS / UB;
+constexpr int FUSED_PASSES = 4;
+constexpr int FUSED_SEGMENTS = FUSED_GROUPS * FUSED_PASSES;  // 1024 states per species
+
+struct StateLoads {
+    double fk, iap, dinj;
+    EllLane ell;
+    double x[ELL_LANE];  // gathered f of the row's first slots (ell_gather)
+};
+
+__device__ inline void state_loads(const UpdateArgs& a, int64_t r, int q, bool mine,
+                                   StateLoads& L) {
+    L.fk = L.iap = L.dinj = 0.0;
+    if (mine && q == 0) {
+        L.fk = a.f[r];
+        L.iap = a.inv_ap[r];
+        L.dinj = a.df_inj[r];
+    }
+    ell_issue(a, r, q, mine, 0, L.ell);
+}
+
+__global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(UpdateArgs a,
+                                                                           CorrectArgs ca) {
+    extern __shared__ double fc[];  // f of the whole cell (emission / Domain only)
+    __shared__ double red[(FUSED_THREADS / 64) * RED_VALUES];
+    __shared__ double cpl[FUSED_SEGMENTS * 3];
+    __shared__ double coefl[2];
+    __shared__ int ired[FUSED_THREADS / 64];
+    const int c = blockIdx.x, s = blockIdx.y;
+    const int tid = threadIdx.x, G = tid >> 8, t = tid & (UB - 1), q = t & 3, N = a.N;
+    double* rg = red + G * (UB / 64) * RED_VALUES;
+    const int k0s = a.species_off[s], ns = a.species_off[s + 1] - k0s;
+    const int nz = (ns + UPDATE_STATES - 1) / UPDATE_STATES;
+    const int seg0 = k0s / UPDATE_STATES, nseg = (N + UPDATE_STATES - 1) / UPDATE_STATES;
+    if (a.fin_rec && c == 0 && s == 0 && tid == 0) {
+        double* rec = a.fin_rec + (int64_t)a.fin_slot * FIN_WORDS;
+        rec[0] = (double)a.step_index;
+        rec[1] = a.t_start;
+        rec[2] = a.dt;
+        rec[3] = (double)a.save;
+        rec[4] = (double)a.verbose;
+    }
+    // wave 0: the moment-rate item sums, issued before the slot loads
+    double cv[WB] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (tid < 64) mom_rate_sums(a, c, s, tid, cv);
+    // the first pass's state loads and flux row
+    StateLoads ld;
+    {
+        const int kk = G * UPDATE_STATES + (t >> 2);
+        const bool mine = G < nz && kk < ns;
+        state_loads(a, (int64_t)c * N + k0s + kk, q, mine, ld);
+    }
+    // phase 1: slot sums and gain-moment partials per segment
+    const int gsl = ca.gslots[s], nsl = ca.nslots[s];
+    double gv[FUSED_PASSES], lv[FUSED_PASSES];
+#pragma unroll
+    for (int p = 0; p < FUSED_PASSES; ++p) {
+        gv[p] = lv[p] = 0.0;
+        if (p * FUSED_GROUPS >= nz) break;  // the whole workgroup
+        const int z = p * FUSED_GROUPS + G;
+        const bool live = z < nz;
+        const int kk = z * UPDATE_STATES + (t >> 2);
+        const bool valid = live && kk < ns;
+        const int64_t idx = (int64_t)c * N + k0s + (valid ? kk : 0);
+        double g = 0.0, l = 0.0;
+        if (live) {
+            g = slot_sum(ca, idx, q, 0, gsl, ca.gain_f32 != 0);
+            l = slot_sum(ca, idx, q, gsl, nsl, false);
+        }
+#pragma unroll
+        for (int m = 1; m < 4; m <<= 1) {
+            g = g + __shfl_xor(g, m, 64);
+            l = l + __shfl_xor(l, m, 64);
+        }
+        gv[p] = g;
+        lv[p] = l;
+        // the first pass's flux gathers: in flight across the barriers up to phase 3
+        if (p == 0) ell_gather(a, ld.ell, ld.x);
+        const bool mine = valid && q == 0;
+        const double e = (mine && ca.eweight) ? ca.eweight[k0s + kk] : 0.0;
+        const double gm = mine ? g : 0.0;
+        double v[WB] = {gm, e * gm, (e * e) * gm, 0.0, 0.0, 0.0, 0.0, 0.0};
+        wave_reduce_batch(v, 0xFFu);
+        if ((t & 63) == 0)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) rg[(t >> 6) * RED_VALUES + i] = v[i];
+        __syncthreads();
+        if (live && t < 3) cpl[z * 3 + t] = wave_combine(t, true, rg);
+        __syncthreads();
+    }
+    // phase 2: (alpha, beta) of the species
+    if (tid < 64) {
+        cpart_sums(cpl, nz, tid, cv);
+        coef_solve(a, cv, tid, coefl);
+    }
+    const bool need_cell = a.emission || (a.binary && a.active);
+    const int any_nz = need_cell ? load_cell<FUSED_THREADS>(a, c, fc, ired) : 1;
+    if (!need_cell) __syncthreads();  // coefl (load_cell ends with barriers)
+    const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
+    const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
+    const double cf0 = coefl[0], cf1 = coefl[1];
+    // phase 3: the update of each segment
+#pragma unroll
+    for (int p = 0; p < FUSED_PASSES; ++p) {
+        if (p * FUSED_GROUPS >= nz) break;  // the whole workgroup
+        const int z = p * FUSED_GROUPS + G;
+        const bool live = z < nz;
+        const int kk = z * UPDATE_STATES + (t >> 2);
+        const bool mine = live && kk < ns;
+        const int k = k0s + kk;
+        const int64_t r = (int64_t)c * N + k;
+        if (p > 0) {
+            state_loads(a, r, q, mine, ld);
+            ell_gather(a, ld.ell, ld.x);
+        }
+        double sumdf = 0.0, sumf = 0.0, cr = INFINITY, crb = INFINITY, cre = INFINITY,
+               crf = INFINITY, fnew = 0.0;
+        double dB = 0.0;
+        if (a.binary) {
+            double g = 0.0;
+            if (mine && active && q == 0) {
+                const double e = a.eweight ? a.eweight[k] : 0.0;
+                g = gv[p] * (1.0 + (cf0 + cf1 * e)) + lv[p];
+            }
+            g = g + __shfl_xor(g, 1, 64);
+            g = g + __shfl_xor(g, 2, 64);
+            dB = g * ((mine && active) ? a.vol[c] : 0.0);
+        }
+        double dF = ell_sum_x(ld.ell, ld.x, 0, a.ell_width, 0.0);
+        for (int s0 = 4 * ELL_LANE; s0 < a.ell_width; s0 += 4 * ELL_LANE) {  // wider rows
+            EllLane more;
+            ell_issue(a, r, q, mine, s0, more);
+            dF = ell_sum(a, more, s0, dF);
+        }
+        if (mine && q == 0) {
+            const double fk = ld.fk;
+            sumf = fk;
+            double df = -dF;
+            double dE = 0.0;
+            if (a.emission) {
+                dE = emission_term(a, c, k, fc);
+                df = df + dE;
+            }
+            if (a.binary) df = df + dB;
+            df = df * (ld.iap * a.s);
+            sumdf = df;
+            if (want_cr) {
+                cr = cr_min(cr, df / fk);
+                if (a.verbose == 3) {
+                    if (a.binary) crb = cr_min(crb, dB / fk);
+                    if (a.emission) cre = cr_min(cre, dE / fk);
+                    crf = cr_min(crf, (-dF) / fk);
+                }
+            }
+            double x = fk + (df + ld.dinj * a.s);
+            x = strong_zero(x, x >= THR_1F_10);
+            x = strong_zero(x, x != INFINITY);
+            a.f_out[r] = x;
+            fnew = x;
+            publish(a, c, k, x);
+        }
+        finish_block(a, c, seg0 + (live ? z : 0), nseg, mine && q == 0, k, fnew, rg, sumdf, sumf,
+                     cr, crb, cre, crf, INFINITY, INFINITY, t, live);
+        __syncthreads();  // rg is reused by the next pass
+    }
+}
+
+bool fused_update_fits(int N, int S, const int32_t* species_off, int tps) {
+    if (tps != 4) return false;
+    for (int s = 0; s < S; ++s) {
+        if (species_off[s] % UPDATE_STATES != 0) return false;
+        const int nz = (species_off[s + 1] - species_off[s] + UPDATE_STATES - 1) / UPDATE_STATES;
+        if (nz > FUSED_SEGMENTS) return false;
+    }
+    return species_off[S] == N;
+}
+
+hipError_t launch_euler_update_fused(const UpdateArgs& a, const CorrectArgs& ca, bool finalize,
+                                     hipStream_t stream) {
+    const bool need_cell = a.emission || (a.binary && a.active);
+    const size_t smem = need_cell ? (size_t)a.N * sizeof(double) : 0;
+    euler_update_fused_kernel<<<dim3(a.C, a.S), FUSED_THREADS, smem, stream>>>(a, ca);
+    hipError_t e = hipGetLastError();
+    return (e != hipSuccess || !finalize) ? e : launch_finalize(a, stream);
 }
 
 __global__ void publish_state_kernel(UpdateArgs a) {

@@ -84,6 +84,7 @@ struct dtx_handle {
     // slots, and the corrected binary sum the update kernels read
     int32_t moment_correction = 1;  // requested (applies to FP32 / BF16x3)
     bool correct_active = false;
+    bool fused_update = false;  // corrected Forward Euler steps by euler_update_fused_kernel
     std::vector<PackedChannel> mchans;
     std::vector<double> eweight_host;
     // moment-rate correction: gain and loss sums per state, moment partials, and the
@@ -573,6 +574,7 @@ static int build_work(dtx_handle* h) {
     h->nitems = 0;
     h->max_slots = h->part_slots = 0;
     h->correct_active = false;
+    h->fused_update = false;
     const int C = h->C;
     const int ctiles = (C + h->tc - 1) / h->tc;
     // wide gain items: h->wide k tiles x WIDE cells each
@@ -799,6 +801,9 @@ static int build_work(dtx_handle* h) {
     if (!rc) rc = upload(h, h->d_mom_list, mlist.data(), h->n_mom_list);
     if (rc) return rc;
     h->correct_active = true;
+    // one launch where its grid, one workgroup per (cell, species), is one round on the CUs
+    h->fused_update = h->moment_correction != 2 && (int64_t)h->C * h->S <= h->num_cu &&
+                      fused_update_fits(h->N, h->S, h->sp_off.data(), h->tps);
     return DTX_OK;
 }
 
@@ -1055,6 +1060,14 @@ static int enqueue_step(dtx_handle* h, double t_start, double t_stop, double dt,
     const double s = dt / h->cfg.dt0;  // src/SteppingMethods.jl:27
     const bool bin = h->binary && h->nitems > 0;
     const bool timing = h->timing && !use_scratch;
+    // corrected Forward Euler: slot sums, coefficients and update in one launch where it fits
+    const bool fused = bin && h->correct_active && h->fused_update &&
+                       h->method == DTX_METHOD_FORWARD_EULER;
+    CorrectArgs fca{};
+    if (fused) {
+        fca = make_correct_args(h);
+        if (use_scratch) fca.part = h->scratch;
+    }
     TimedEvent ev{};
     if (bin) {
         int rc = timing ? timed_begin(h, 0, &ev) : DTX_OK;
@@ -1063,7 +1076,7 @@ static int enqueue_step(dtx_handle* h, double t_start, double t_stop, double dt,
         if (h->wide)
             HIP_CHECK(h, launch_binary_wide(h->precision, h->wide, items, h->nitems, h->stream));
         else HIP_CHECK(h, launch_binary(h->precision, h->tc, items, h->nitems, h->stream));
-        if (h->correct_active) {
+        if (h->correct_active && !fused) {
             CorrectArgs ca = make_correct_args(h);
             if (use_scratch) ca.part = h->scratch;
             HIP_CHECK(h, launch_gain_correct(ca, h->stream));
@@ -1081,11 +1094,14 @@ static int enqueue_step(dtx_handle* h, double t_start, double t_stop, double dt,
             a.block_part = h->d_block_part + slot * h->bp_words;
             a.fin_rec = h->d_fin_rec;
             a.fin_slot = slot;
-            HIP_CHECK(h, launch_euler_update_only(a, h->stream));
+            if (fused) HIP_CHECK(h, launch_euler_update_fused(a, fca, false, h->stream));
+            else HIP_CHECK(h, launch_euler_update_only(a, h->stream));
             if (h->fin_count == 0) h->fin_first = h->step_counter;
             h->fin_count++;
             h->fin_base = a;
             if (h->fin_count == FIN_BATCH) HIP_CHECK(h, flush_finalize(h));
+        } else if (fused) {
+            HIP_CHECK(h, launch_euler_update_fused(a, fca, true, h->stream));
         } else {
             HIP_CHECK(h, launch_euler_update(a, h->stream));
         }
@@ -2069,7 +2085,7 @@ int dtx_set_moment_correction(dtx_handle* h, int32_t enable) {
     if (h->binary && !h->binary_dirty)
         return fail(h, DTX_ERR_STATE,
                     "the moment-rate correction must be chosen before the first step");
-    h->moment_correction = enable ? 1 : 0;
+    h->moment_correction = enable == 2 ? 2 : (enable ? 1 : 0);
     return DTX_OK;
 }
 
@@ -2338,6 +2354,7 @@ int dtx_work(const dtx_handle* hc, dtx_work_info* out) {
     const double C = h->C, N = h->N;
     out->pairs = h->pairs_total;
     out->bin_launch_items = h->nitems;
+    out->fused_correction = h->correct_active && h->fused_update ? 1 : 0;
     out->gain_bytes = h->gain_bytes;
     if (h->binary) {
         out->bin_flops = h->bin_flops;  // gain contraction + FP64 loss channels

@@ -48,7 +48,7 @@ class StepStats(C.Structure):
 
 class WorkInfo(C.Structure):
     _fields_ = [("bin_flops", D), ("bin_bytes", D), ("update_bytes", D), ("gain_bytes", D),
-                ("pairs", I64), ("bin_launch_items", I32)]
+                ("pairs", I64), ("bin_launch_items", I32), ("fused_correction", I32)]
 
 
 class KernelTimes(C.Structure):

@@ -231,9 +231,11 @@ class _SteppingStruct:
         w = _f64(PhaseSpace.energy_weights())
         _lib.check(self.lib.dtx_set_moment_weights(h, _lib.dptr(w)), h,
                    "dtx_set_moment_weights")
-        # FP32 / BF16x3: FP64 number and energy rates of the gain term (no effect in FP64)
-        _lib.check(self.lib.dtx_set_moment_correction(h, int(bool(MomentCorrection))), h,
-                   "dtx_set_moment_correction")
+        # FP32 / BF16x3: FP64 number and energy rates of the gain term (no effect in FP64);
+        # MomentCorrection=2 keeps its three separate launches (the same arithmetic)
+        mc = 2 if (not isinstance(MomentCorrection, bool) and MomentCorrection == 2) else \
+            int(bool(MomentCorrection))
+        _lib.check(self.lib.dtx_set_moment_correction(h, mc), h, "dtx_set_moment_correction")
 
     def _add_interactions(self, h, PhaseSpace, BinM):
         """Each resolved interaction through dtx_add_binary_interaction, in Binary_list order

@@ -205,7 +205,10 @@ int dtx_set_moment_weights(dtx_handle* h, const double* energy_weight);
  * energy rates f^T W f of the gain term every step; the reduced-precision gain sums of each
  * (cell, species) are rescaled per state by (1 + a + b E_k) so that their number and energy
  * sums equal those rates.  No counterpart in the reference, which has no reduced-precision
- * path; FP64 handles ignore it.  Must be called before the first step. */
+ * path; FP64 handles ignore it.  Must be called before the first step.
+ * enable: 0 off; 1 on, Forward Euler steps summing the slots, solving the coefficients and
+ * updating in one launch where the species layout allows; 2 on, always as three launches
+ * (bitwise the same results; for checking). */
 int dtx_set_moment_correction(dtx_handle* h, int32_t enable);
 
 /* Synthetic workload support: generate lognormal(0,1)*scale gain channels for every species
@@ -296,6 +299,8 @@ typedef struct {
     double gain_bytes;     /* packed gain operator bytes (streamed once per step)         */
     int64_t pairs;         /* total packed pairs over channels                            */
     int32_t bin_launch_items;
+    int32_t fused_correction;  /* 1: corrected Forward Euler steps sum the slots, solve the
+                                  coefficients and update in one launch                    */
 } dtx_work_info;
 int dtx_work(const dtx_handle* h, dtx_work_info* out);
 /* Device time of the last synchronised window, per kernel class, in milliseconds (HIP events

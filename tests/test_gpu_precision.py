@@ -115,3 +115,47 @@ def test_symplectic_reduced_precision(precision):
     fc, nc, ec = run(w, precision, method="symplectic", steps=30)
     assert rel_err_f(fc, f64) <= F_TOL[precision]
     assert np.all(np.isfinite(nc)) and np.all(np.isfinite(ec))
+
+
+def _steps_with_stats(w, precision, correction, steps, verbose=3):
+    m = workloads.make_struct(w, precision=precision, use_monolithic=False,
+                              moment_correction=correction)
+    assert m.work().fused_correction == (1 if correction is True else 0)
+    t, stats = 0.0, []
+    for _ in range(steps):
+        dt, _, st = m(t, 1e300, w.dt, verbose, stats=True)
+        stats.append((st.finite, st.cr, st.cr_bin, st.cr_emi, st.cr_flux, st.sum_f))
+        t += dt
+    m.synchronize()
+    f = m.f
+    num, en = m.moments()
+    m.close()
+    return f, num, en, stats
+
+
+FUSED_CASES = {
+    # 2 species x 256 states, 128 cells: one pass of four segments per species
+    "transport_1d": lambda: workloads.transport_1d(steps=6),
+    # 320 states per species: a second pass with one live segment group of four
+    "two_passes": lambda: workloads.transport_1d(p_num=40, x_num=64, steps=6),
+    # 128 cells with emission blocks, a binary Domain and injection: the cell loaded into LDS
+    "emission_domain": lambda: workloads.tiny(p_num=16, u_num=8, x_num=128, monolithic=False,
+                                              emission=True, injection=True,
+                                              domain=list(range(1, 129, 3)), steps=6),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("case", list(FUSED_CASES))
+def test_fused_correction_equals_three_launches(precision, case):
+    """Corrected Forward Euler steps by the one-launch form (slot sums, coefficients and update
+    fused per (cell, species)) against the three-launch form: f, moments and the Verbose-3
+    diagnostics bit for bit."""
+    w = FUSED_CASES[case]()
+    print(f"[case] {case} {precision} N={w.ps.n_momentum} C={w.ps.n_space}", flush=True)
+    fa, na, ea, sa = _steps_with_stats(w, precision, True, w.steps)
+    fb, nb, eb, sb = _steps_with_stats(w, precision, 2, w.steps)
+    np.testing.assert_array_equal(fa.view(np.uint64), fb.view(np.uint64))
+    np.testing.assert_array_equal(na.view(np.uint64), nb.view(np.uint64))
+    np.testing.assert_array_equal(ea.view(np.uint64), eb.view(np.uint64))
+    assert sa == sb
