@@ -67,3 +67,17 @@ def test_every_ccall_is_declared():
     declared = set(re.findall(r"\b(dtx_\w+)\s*\(", header))
     called = set(re.findall(r"ccall\(\(:(\w+),", glue))
     assert called and called <= declared, called - declared
+
+
+def test_solve_hooks_move_the_state():
+    """The reference's unchanged Solve uploads f0 with `@. method.f = method.f_init` and copies each
+    saved state out with `copyto!(f, method.f)` (src/Solver.jl:22,40) before its own
+    SolutionFileSave writes the JLD2 file: the glue routes both through the C ABI."""
+    glue = GLUE.read_text()
+    down = re.search(r"function Base\.copyto!\(dest::AbstractVector\{Float64\}, src::DtxState\)"
+                     r"(.*?)\nend", glue, re.S)
+    assert down and "(:dtx_get_f, libdtx)" in down.group(1)
+    up = re.search(r"function Base\.copyto!\(dest::DtxState, src::AbstractVector\)(.*?)\nend",
+                   glue, re.S)
+    assert up and "(:dtx_set_f, libdtx)" in up.group(1)
+    assert re.search(r"Base\.copyto!\(dest::DtxState, bc::Base\.Broadcast\.Broadcasted\)", glue)
