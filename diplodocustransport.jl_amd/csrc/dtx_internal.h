@@ -204,6 +204,13 @@ hipError_t launch_binary_mixed(int precision, const BinWork* items, int nitems,
 // Contraction with (64 ns) x WIDE (rows x cells) gain items for C > 64: split-BF16 with ns = 2
 // or 4 k tiles per item, FP64 with ns = 2.  LOSS-layout items keep the 64 x 64 FP64 body.
 constexpr int WIDE = 128;   // cells per wide item
+// Groups of four waves in a wide FP64 workgroup: each group takes its own share of the item's
+// chunk range, and the groups' sums are added in LDS before the single partial is written, so
+// an item needs one slot per G chunk shares.  With G = 2 LOSS items take the wide shape too.
+#ifndef DTX_F64_WIDE_GROUPS
+#define DTX_F64_WIDE_GROUPS 2
+#endif
+constexpr int F64_WIDE_GROUPS = DTX_F64_WIDE_GROUPS;
 hipError_t launch_binary_wide(int precision, int ns, const BinWork* items, int nitems,
                               hipStream_t stream);
 hipError_t launch_pack_loss(const double* D, int N, const int32_t* pad_index, int Np, int row0,

@@ -966,33 +966,44 @@ __global__ __launch_bounds__(128 * NS) void bin_bf16x3_wide_kernel(
 
 
 // ------------------------------------------------------------------------- wide FP64 -----
-// FP64 gain items for C > 64: two packed 64-row slices (k tiles 2w and 2w+1) x 128 cells, four
-// waves of 64 x 64 (16 tiles of v_mfma_f64_16x16x4_f64 each), so every staged byte feeds twice
-// the MFMAs of a 64 x 64 item.  B' = f_i f_j is formed while staging, as in f64_item.  Both
-// tiles sit in LDS in MFMA operand order: one 64-double block per (k step of 4, 16 rows or 16
-// cells), element l of a block being what lane l supplies (A[row l % 16][k l / 16],
+// FP64 items for C > 64: two packed 64-row slices (k tiles 2w and 2w+1) x 128 cells, per group
+// of four waves 64 x 64 each (16 tiles of v_mfma_f64_16x16x4_f64), so every staged byte feeds
+// twice the MFMAs of a 64 x 64 item.  B' = f_i f_j is formed while staging, as in f64_item
+// (LOSS items stage f_j and multiply each row by its own f when written).  Both tiles sit in
+// LDS in MFMA operand order: one 64-double block per (k step of 4, 16 rows or 16 cells),
+// element l of a block being what lane l supplies (A[row l % 16][k l / 16],
 // B[k l / 16][cell l % 16]), so every LDS store and operand read of a wave is one linear
 // 512-byte block.
+//
+// With G groups the item's chunk range is cut into G consecutive shares, one per group, each
+// with its own double-buffered stages; the groups meet at the same barriers.  At the end group
+// g hands the other group the cell blocks it will not write, and adds the ones it receives:
+// the item's sum is share 0 + share 1, and one partial slot covers G shares.
+template <int G>
 struct alignas(16) SmemWide64 {
     static constexpr int HK = KC / 2;   // j per LDS stage
     union {
         struct {
-            double As[2][HK * 2 * TK];  // block ks * 8 + row block (0..7)
-            double Bs[2][HK * WIDE];    // block ks * 8 + cell block (0..7)
+            double As[G][2][HK * 2 * TK];  // block ks * 8 + row block (0..7)
+            double Bs[G][2][HK * WIDE];    // block ks * 8 + cell block (0..7)
         } s;
-        double T[4][16 * EPI_PITCH];    // epilogue: each wave's 16 cells x 64 rows
+        double X[4 * G][32 * 64];          // the exchanged half of each wave's tiles (G = 2)
+        double T[4 * G][16 * EPI_PITCH];   // epilogue: each wave's 16 cells x 64 rows
     };
     uint32_t ctab[MAX_RANGE];
 };
 
-__device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64& sm) {
-    constexpr int HK = SmemWide64::HK;
-    auto& As = sm.s.As;
-    auto& Bs = sm.s.Bs;
+template <int G>
+__device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm) {
+    constexpr int HK = SmemWide64<G>::HK;
     uint32_t* ctab = sm.ctab;
     const ItemView v = load_item(w, WIDE, 8);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x & 255, grp = threadIdx.x >> 8;
+    const int lane = tid & 63, wave = tid >> 6;
+    auto& As = sm.s.As[grp];
+    auto& Bs = sm.s.Bs[grp];
     const int wk = wave & 1, wc = wave >> 1;    // this wave: rows 64 wk.., cells 64 wc..
+    const bool loss = v.kind == CH_LOSS;
     // 16-byte pieces: elements q 512 + 2 tid and the next (q = 0..3) of a stage tile are in
     // block 8 q + tid / 32 = 8 ks + b (ks = q) at lanes l0 = 2 (tid % 32) and l0 + 1: the same k,
     // 4 ks + l0 / 16, and consecutive rows or cells.  A: row block b is slice b / 4, rows
@@ -1007,11 +1018,15 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64& sm) {
                            (int64_t)v.c_begin * KC * TK + 16 * (b8 & 3) + (l0 & 15);
     const int bcell = 16 * b8 + (l0 & 15);
     load_ctab(v, ctab);
+    // this group's share of the range: local chunks [cb, ce)
+    const int nch = v.c_end - v.c_begin, share = (nch + G - 1) / G;
+    const int cb = grp * share, ce_ = min(nch, cb + share);
+    const int nst = 2 * max(0, ce_ - cb), nst_all = 2 * share;   // stages: own, every group
 
     d2 ra[4], rb[4], rf;
-    // stage st = half (st & 1) of local chunk (st >> 1)
+    // stage st = half (st & 1) of share chunk (st >> 1)
     auto load_stage = [&](int st) {
-        const int ch = st >> 1, h = st & 1;
+        const int ch = cb + (st >> 1), h = st & 1;
         const uint32_t ce = ctab[ch];
         const int64_t ao = (int64_t)ch * KC * TK + (int64_t)(h * HK + kl) * TK;
 #pragma unroll
@@ -1020,13 +1035,13 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64& sm) {
                                (int64_t)((int)(ce >> 16) + h * HK + kl) * v.ldc + v.c0 + bcell;
 #pragma unroll
         for (int q = 0; q < 4; ++q) rb[q] = *(const GLB d2*)(bp + (int64_t)(4 * q) * v.ldc);
-        rf = *(const GLB d2*)(v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcell);
+        if (!loss) rf = *(const GLB d2*)(v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcell);
     };
     auto store_stage = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) *(d2*)&As[buf][q * 512 + 2 * tid] = ra[q];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) *(d2*)&Bs[buf][q * 512 + 2 * tid] = rf * rb[q];
+        for (int q = 0; q < 4; ++q) *(d2*)&Bs[buf][q * 512 + 2 * tid] = loss ? rb[q] : rf * rb[q];
     };
 
     d4 acc[4][4];
@@ -1034,70 +1049,113 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64& sm) {
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
-    const int nst = 2 * (v.c_end - v.c_begin);
     if (nst > 0) {
         load_stage(0);
         store_stage(0);
     }
     __syncthreads();
     int buf = 0;
-    for (int st = 0; st < nst; ++st) {
+    for (int st = 0; st < nst_all; ++st) {   // every group meets every barrier
+        const bool mine = st < nst;
         const bool more = st + 1 < nst;
         if (more) load_stage(st + 1);
+        if (mine) {
 #pragma unroll
-        for (int ks = 0; ks < HK / 4; ++ks) {
-            double a[4], b[4];
+            for (int ks = 0; ks < HK / 4; ++ks) {
+                double a[4], b[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = As[buf][(8 * ks + 4 * wk + r) * 64 + lane];
+                for (int r = 0; r < 4; ++r) a[r] = As[buf][(8 * ks + 4 * wk + r) * 64 + lane];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b[c] = Bs[buf][(8 * ks + 4 * wc + c) * 64 + lane];
+                for (int c = 0; c < 4; ++c) b[c] = Bs[buf][(8 * ks + 4 * wc + c) * 64 + lane];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+                    for (int c = 0; c < 4; ++c)
+                        acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+            }
         }
         if (more) store_stage(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
-    // write the item's partial sums: per group of 16 cells each wave puts its result tiles
-    // (FP64 layout D[(l / 16) + 4 e][l % 16]) into LDS cell-major, then lane l writes row l of
-    // one cell per store, 512 contiguous bytes instead of 16 pieces of 32 (the epilogue of all
-    // items runs at about the same time: measured -3% contraction time at transport_1d)
-    double* tw = sm.T[wave];
+    // the cell blocks c this group writes: all (G = 1), or 2 grp and 2 grp + 1 (G = 2); the
+    // block indices are compile-time constants (a run-time index would put acc in scratch)
+    constexpr int NC = 4 / G;
     const int krow = kt0 * TK + 64 * wk + lane;
+    double* tw = sm.T[threadIdx.x >> 6];
+    auto finish = [&](auto first) {
+        constexpr int c_first = decltype(first)::value;
+        if constexpr (G == 2) {
+            // hand over the other group's two cell blocks (lane-minor, 64 doubles a store), then
+            // add the two received: share 0 + share 1 for every tile (addition is commutative,
+            // so both groups form the same sums)
+            constexpr int give = 2 - c_first;
+            double* mine_x = sm.X[threadIdx.x >> 6];
+            const double* other_x = sm.X[(threadIdx.x >> 6) ^ 4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+            for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                tw[(lane & 15) * EPI_PITCH + 16 * r + (lane >> 4) + 4 * e] = acc[r][c][e];
-        __syncthreads();
-        for (int q = 0; q < 16; ++q) {
-            const int cell = v.c0 + 64 * wc + 16 * c + q;
-            if (cell < v.cells && krow < v.rows)
-                v.part[(int64_t)cell * v.N + v.row_base + krow] = tw[q * EPI_PITCH + lane];
+                    for (int e = 0; e < 4; ++e)
+                        mine_x[((cc * 4 + r) * 4 + e) * 64 + lane] = acc[r][give + cc][e];
+            __syncthreads();
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        acc[r][c_first + cc][e] =
+                            acc[r][c_first + cc][e] + other_x[((cc * 4 + r) * 4 + e) * 64 + lane];
+            __syncthreads();   // T aliases X
         }
-        __syncthreads();
-    }
+        // write the item's partial sums: per group of 16 cells each wave puts its result tiles
+        // (FP64 layout D[(l / 16) + 4 e][l % 16]) into LDS cell-major, then lane l writes row l
+        // of one cell per store, 512 contiguous bytes instead of 16 pieces of 32 (the epilogue
+        // of all items runs at about the same time: measured -3% contraction time at
+        // transport_1d); LOSS rows are multiplied by their own f
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+            const int c = c_first + cc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    tw[(lane & 15) * EPI_PITCH + 16 * r + (lane >> 4) + 4 * e] = acc[r][c][e];
+            __syncthreads();
+            for (int q = 0; q < 16; ++q) {
+                const int cell = v.c0 + 64 * wc + 16 * c + q;
+                if (cell < v.cells && krow < v.rows) {
+                    double x = tw[q * EPI_PITCH + lane];
+                    if (loss) x = x * v.fT[(int64_t)(v.row_pad_base + krow) * v.ldc + cell];
+                    v.part[(int64_t)cell * v.N + v.row_base + krow] = x;
+                }
+            }
+            __syncthreads();
+        }
+    };
+    if (grp == 0) finish(std::integral_constant<int, 0>{});
+    else if constexpr (G == 2) finish(std::integral_constant<int, 2>{});
 }
 
-__global__ __launch_bounds__(256) void bin_f64_wide_kernel(const BinWork* __restrict__ items) {
+// G = 1: LOSS items take the 64-cell FP64 body.  G = 2: every item is wide.
+__global__ __launch_bounds__(256 * F64_WIDE_GROUPS) void bin_f64_wide_kernel(
+    const BinWork* __restrict__ items) {
     __shared__ union {
         SmemF64<64> d;
-        SmemWide64 w;
+        SmemWide64<F64_WIDE_GROUPS> w;
     } sm;
     const BinWork& w = items[blockIdx.x];
-    if (w.kind >= CH_LOSS) f64_item<64>(w, sm.d);
-    else wide64_item(w, sm.w);
+    if (F64_WIDE_GROUPS == 1 && w.kind >= CH_LOSS) f64_item<64>(w, sm.d);
+    else wide64_item<F64_WIDE_GROUPS>(w, sm.w);
 }
 
 hipError_t launch_binary_wide(int precision, int ns, const BinWork* items, int nitems,
                               hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
-    if (precision == DTX_PREC_FP64 && ns == 2) bin_f64_wide_kernel<<<nitems, 256, 0, stream>>>(items);
+    if (precision == DTX_PREC_FP64 && ns == 2)
+        bin_f64_wide_kernel<<<nitems, 256 * F64_WIDE_GROUPS, 0, stream>>>(items);
     else if (precision != DTX_PREC_BF16X3) return hipErrorInvalidValue;
     else if (ns == 4) bin_bf16x3_wide_kernel<4><<<nitems, 512, 0, stream>>>(items);
     else if (ns == 2) bin_bf16x3_wide_kernel<2><<<nitems, 256, 0, stream>>>(items);

@@ -605,12 +605,17 @@ static int build_work(dtx_handle* h) {
     // was 5% faster)
     const int64_t loss_cap = h->wide ? 16 : 4;
     auto loss_range_of = [&](int64_t rg) { return std::min<int64_t>(rg, loss_cap); };
+    // wide FP64 workgroups of several groups: LOSS items take the wide shape as well, so every
+    // workgroup of the launch holds a CU alike
+    const bool wide_loss =
+        h->wide && h->precision == DTX_PREC_FP64 && F64_WIDE_GROUPS > 1;
     auto launch_items = [&](int64_t rg) {
         const int64_t lr = loss_range_of(rg);
         int64_t n = 0;
         for (auto& c : h->chans)
             n += c.kind == CH_LOSS
-                     ? (c.nchunks + lr - 1) / lr * c.nktiles * ctiles
+                     ? (c.nchunks + lr - 1) / lr *
+                           (wide_loss ? gain_ksteps(c) * wctiles : c.nktiles * ctiles)
                      : (c.nchunks + rg - 1) / rg * gain_ksteps(c) * (h->wide ? wctiles : ctiles);
         for (auto& c : h->mchans) n += (c.nchunks + lr - 1) / lr * c.nktiles * ctiles;
         return n;
@@ -620,13 +625,15 @@ static int build_work(dtx_handle* h) {
     // CU and runs fastest when all its items fit one round of them: the smallest range with at
     // most 2 x CUs items (at transport_1d, FP64 and split-BF16, a round and a bit more measured
     // 10-20% slower, and fewer, longer items slower too).  With four k tiles (one workgroup per
-    // CU) the items cannot fit one round; about three rounds of gain items measured best.
+    // CU) the items cannot fit one round; about three rounds of gain items measured best.  A
+    // wide FP64 workgroup of G groups holds a CU alone and covers G chunk shares: one round is
+    // CUs items.
     auto clamp_range = [](double r) {
         return std::min<int64_t>(MAX_RANGE, std::max<int64_t>(8, (int64_t)std::ceil(r)));
     };
     int64_t range = clamp_range(total / (h->wide == 4 ? 768 : h->wide ? 512 : 1024));
     if (h->wide == 2) {
-        const int64_t round = 2 * (int64_t)h->num_cu;
+        const int64_t round = (wide_loss ? 1 : 2) * (int64_t)h->num_cu;
         for (int64_t rg = 8; rg <= MAX_RANGE; ++rg)
             if (launch_items(rg) <= round) {
                 range = rg;
@@ -645,7 +652,7 @@ static int build_work(dtx_handle* h) {
     auto add_units = [&](const PackedChannel& c, int64_t rg, std::vector<Unit>& out,
                          const std::function<int()>& next) {
         const bool loss = c.kind == CH_LOSS;
-        const bool wide = h->wide && !loss;
+        const bool wide = h->wide && (!loss || wide_loss);
         const int prec = loss ? DTX_PREC_FP64 : h->precision;
         const int s_row = c.moment >= 0 ? c.s1 : c.s_out;  // owner of the output rows
         for (int64_t c0 = 0; c0 < c.nchunks; c0 += rg) {
