@@ -991,7 +991,22 @@ struct alignas(16) SmemWide64 {
         double T[4 * G][16 * EPI_PITCH];   // epilogue: each wave's 16 cells x 64 rows
     };
     uint32_t ctab[MAX_RANGE];
+    uint32_t arrived[G];   // per group: waves arrived at its stage barriers so far
 };
+
+// Stage barrier of one group of four waves (G > 1): the workgroup barrier would hold every
+// group at each stage, so that no group's MFMAs cover another's barrier.  Each wave waits for
+// its own LDS reads and writes, adds one to the group's counter and waits until all four have
+// arrived at this barrier (target = 4 x barriers so far).  The four waves of a group run the
+// same stages and every wave of the workgroup is resident, so every wait ends.
+__device__ __forceinline__ void group_barrier(uint32_t* arrived, uint32_t target) {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if ((threadIdx.x & 63) == 0)
+        __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (__hip_atomic_load(arrived, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target)
+        __builtin_amdgcn_s_sleep(1);
+    asm volatile("" ::: "memory");
+}
 
 template <int G>
 __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm) {
@@ -1017,7 +1032,8 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm)
                            (int64_t)(kt0 + ((b8 >> 2) && has1 ? 1 : 0)) * slice +
                            (int64_t)v.c_begin * KC * TK + 16 * (b8 & 3) + (l0 & 15);
     const int bcell = 16 * b8 + (l0 & 15);
-    load_ctab(v, ctab);
+    if (threadIdx.x < G) sm.arrived[threadIdx.x] = 0;
+    load_ctab(v, ctab);   // ends with a barrier
     // this group's share of the range: local chunks [cb, ce)
     const int nch = v.c_end - v.c_begin, share = (nch + G - 1) / G;
     const int cb = grp * share, ce_ = min(nch, cb + share);
@@ -1055,7 +1071,10 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm)
     }
     __syncthreads();
     int buf = 0;
-    for (int st = 0; st < nst_all; ++st) {   // every group meets every barrier
+    // G = 1: the workgroup barrier after each stage; G > 1: each group its own stages and
+    // barriers
+    const int nloop = G == 1 ? nst_all : nst;
+    for (int st = 0; st < nloop; ++st) {
         const bool mine = st < nst;
         const bool more = st + 1 < nst;
         if (more) load_stage(st + 1);
@@ -1075,9 +1094,11 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm)
             }
         }
         if (more) store_stage(buf ^ 1);
-        __syncthreads();
+        if constexpr (G == 1) __syncthreads();
+        else group_barrier(&sm.arrived[grp], 4u * (uint32_t)(st + 1));
         buf ^= 1;
     }
+    if constexpr (G > 1) __syncthreads();   // both groups' sums are complete
     // the cell blocks c this group writes: all (G = 1), or 2 grp and 2 grp + 1 (G = 2); the
     // block indices are compile-time constants (a run-time index would put acc in scratch)
     constexpr int NC = 4 / G;
