@@ -24,6 +24,7 @@
 // instructions, and as D[(l/16)+4*e][l%16] by v_mfma_f64_16x16x4_f64 (measured on gfx950).
 #include "bf16_util.h"
 #include "dtx_internal.h"
+#include "wave_ops.h"
 
 namespace dtx {
 
@@ -294,8 +295,8 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
                         cs[c] = cs[c] +
                                 acc[r][c][e] * v.fT[(int64_t)(v.row_pad_base + krow) * v.ldc + cell];
                 }
-            cs[c] = cs[c] + __shfl_xor(cs[c], 16, 64);
-            cs[c] = cs[c] + __shfl_xor(cs[c], 32, 64);
+            cs[c] = cs[c] + xor_lane<16>(cs[c]);
+            cs[c] = cs[c] + xor_lane<32>(cs[c]);
         }
         double* msum = &As[0][0];
         if (lane < 16)
@@ -1297,7 +1298,7 @@ __device__ __forceinline__ void gemv_item(const BinWork& w, SmemGemv& sm) {
     if (v.kind == CH_MOMENT) {
         double m = valid ? t * v.fT[(int64_t)(v.row_pad_base + krow) * v.ldc] : 0.0;
 #pragma unroll
-        for (int d = 32; d > 0; d >>= 1) m = m + __shfl_xor(m, d, 64);
+        for (int d = 32; d > 0; d >>= 1) m = m + xor_lane_rt(m, d);
         if (lane == 0) v.part[0] = m;
         return;
     }

@@ -16,6 +16,7 @@
 // order, so every reduction is deterministic.
 #include "bf16_util.h"
 #include "dtx_internal.h"
+#include "wave_ops.h"
 
 namespace dtx {
 
@@ -39,16 +40,17 @@ __device__ inline double cr_min(double m, double r) {
     return m;
 }
 
-// Wave reductions: a butterfly over the 64 lanes of each wave (__shfl_xor).
+// Wave reductions: a butterfly over the 64 lanes of each wave (xor_lane: DPP and permlane
+// exchanges, no LDS).
 __device__ inline double wave_sum(double v) {
 #pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v = v + __shfl_xor(v, m, 64);
+    for (int m = 32; m > 0; m >>= 1) v = v + xor_lane_rt(v, m);
     return v;
 }
 
 __device__ inline double wave_min(double v) {
 #pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v = cr_min(v, __shfl_xor(v, m, 64));
+    for (int m = 32; m > 0; m >>= 1) v = cr_min(v, xor_lane_rt(v, m));
     return v;
 }
 
@@ -142,8 +144,8 @@ __device__ inline double binary_term(const UpdateArgs& a, int c, int k, int q, b
                 if (s0 + 4 * u < ns) g = g + v[u];
         }
     }
-    g = g + __shfl_xor(g, 1, 64);
-    g = g + __shfl_xor(g, 2, 64);
+    g = g + xor_lane<1>(g);
+    g = g + xor_lane<2>(g);
     return g * (valid ? a.vol[c] : 0.0);
 }
 
@@ -187,7 +189,7 @@ __device__ inline void wave_reduce_batch(double* v, uint32_t summask) {
     for (int m = 32; m > 0; m >>= 1) {
         double x[WB];
 #pragma unroll
-        for (int i = 0; i < WB; ++i) x[i] = __shfl_xor(v[i], m, 64);
+        for (int i = 0; i < WB; ++i) x[i] = xor_lane_rt(v[i], m);
 #pragma unroll
         for (int i = 0; i < WB; ++i)
             v[i] = ((summask >> i) & 1u) ? v[i] + x[i] : cr_min(v[i], x[i]);
@@ -584,13 +586,12 @@ __device__ inline double ell_sum_x(const EllLane& L, const double* x, int s0, in
     double p[ELL_LANE];
 #pragma unroll
     for (int j = 0; j < ELL_LANE; ++j) p[j] = L.c[j] >= 0 ? L.v[j] * x[j] : 0.0;
-    const int lane = threadIdx.x & 63, grp = lane & ~3;
 #pragma unroll
     for (int j = 0; j < ELL_LANE; ++j)
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
-            const double pv = __shfl(p[j], grp | qq, 64);
-            const int cv = __shfl(L.c[j], grp | qq, 64);
+            const double pv = quad_lane_rt(p[j], qq);
+            const int cv = quad_lane_rt(L.c[j], qq);
             if (s0 + 4 * j + qq < width && cv >= 0) acc = acc + pv;
         }
     return acc;
@@ -650,8 +651,8 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
                 const double e = a.eweight ? a.eweight[k] : 0.0;
                 g = gs * (1.0 + (cf[0] + cf[1] * e)) + ls;
             }
-            g = g + __shfl_xor(g, 1, 64);
-            g = g + __shfl_xor(g, 2, 64);
+            g = g + xor_lane<1>(g);
+            g = g + xor_lane<2>(g);
             dB = g * ((mine && active) ? a.vol[c] : 0.0);
         } else {
             dB = binary_term(a, c, k, q, mine && active, coef);
@@ -990,8 +991,8 @@ __device__ inline void correct_sums_body(const CorrectArgs& a, int c, int s, int
     double g = slot_sum(a, idx, q, 0, gs, a.gain_f32 != 0);
     double l = slot_sum(a, idx, q, gs, nsl, false);
     for (int m = 1; m < a.tps; m <<= 1) {
-        g = g + __shfl_xor(g, m, 64);
-        l = l + __shfl_xor(l, m, 64);
+        g = g + xor_lane_rt(g, m);
+        l = l + xor_lane_rt(l, m);
     }
     const bool mine = valid && q == 0;
     if (mine) {
@@ -1148,8 +1149,8 @@ __global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(Updat
         }
 #pragma unroll
         for (int m = 1; m < 4; m <<= 1) {
-            g = g + __shfl_xor(g, m, 64);
-            l = l + __shfl_xor(l, m, 64);
+            g = g + xor_lane_rt(g, m);
+            l = l + xor_lane_rt(l, m);
         }
         gv[p] = g;
         lv[p] = l;
@@ -1201,8 +1202,8 @@ __global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(Updat
                 const double e = a.eweight ? a.eweight[k] : 0.0;
                 g = gv[p] * (1.0 + (cf0 + cf1 * e)) + lv[p];
             }
-            g = g + __shfl_xor(g, 1, 64);
-            g = g + __shfl_xor(g, 2, 64);
+            g = g + xor_lane<1>(g);
+            g = g + xor_lane<2>(g);
             dB = g * ((mine && active) ? a.vol[c] : 0.0);
         }
         double dF = ell_sum_x(ld.ell, ld.x, 0, a.ell_width, 0.0);

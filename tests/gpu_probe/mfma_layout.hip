@@ -1,10 +1,14 @@
-// mfma_layout.hip -- test helper: multiplies one 16x16xK tile with each MFMA builtin the
-// contraction kernels use, reading and writing the operands with the lane layout the kernels
-// assume (A[l%16][kgroup], B[kgroup][l%16]) and returns the raw accumulator of every lane
-// (D[l*4 + e]); the host locates each value in the plain matrix product.  Launched with one wave of 64 lanes; all indexes stay inside the
-// 16x16 / 16xK / Kx16 buffers.
+// mfma_layout.hip -- test helpers for the lane layouts the kernels rely on.
+// mfma_probe: multiplies one 16x16xK tile with each MFMA builtin the contraction kernels use,
+// reading and writing the operands with the lane layout the kernels assume (A[l%16][kgroup],
+// B[kgroup][l%16]) and returns the raw accumulator of every lane (D[l*4 + e]); the host locates
+// each value in the plain matrix product.
+// lane_probe: the DPP / permlane lane exchanges of wave_ops.h (xor distances 1..32, quad
+// broadcasts) applied to one value per lane, for the host to compare with the permutation.
+// Both launch one wave of 64 lanes; all indexes stay inside their buffers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "../../diplodocustransport.jl_amd/csrc/wave_ops.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -55,5 +59,43 @@ extern "C" int mfma_probe(int which, const void* hA, const void* hB, void* hD) {
     if (hipGetLastError() != hipSuccess) return -3;
     if (hipMemcpy(hD, dD, nd, hipMemcpyDeviceToHost)) return -4;
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dD);
+    return 0;
+}
+
+// out[r * 64 + l], r = 0..5: xor_lane<1, 2, 4, 8, 16, 32>(in[l]); r = 6..9: quad_lane<0..3>;
+// r = 10..15: the same xor distances on the low 32 bits as int (through xor_lane_rt);
+// r = 16..19: quad_lane<0..3> on int.
+__global__ void k_lanes(const double* in, double* out) {
+    const int l = threadIdx.x;
+    const double v = in[l];
+    out[0 * 64 + l] = dtx::xor_lane<1>(v);
+    out[1 * 64 + l] = dtx::xor_lane<2>(v);
+    out[2 * 64 + l] = dtx::xor_lane<4>(v);
+    out[3 * 64 + l] = dtx::xor_lane<8>(v);
+    out[4 * 64 + l] = dtx::xor_lane<16>(v);
+    out[5 * 64 + l] = dtx::xor_lane<32>(v);
+    out[6 * 64 + l] = dtx::quad_lane<0>(v);
+    out[7 * 64 + l] = dtx::quad_lane<1>(v);
+    out[8 * 64 + l] = dtx::quad_lane<2>(v);
+    out[9 * 64 + l] = dtx::quad_lane<3>(v);
+    const unsigned long long u = (unsigned long long)l * 0x9E3779B97F4A7C15ull;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        out[(10 + i) * 64 + l] = (double)(dtx::xor_lane_rt(u, 1 << i) >> 40);
+    const int iv = 1000 + 7 * l;
+    out[16 * 64 + l] = (double)dtx::quad_lane<0>(iv);
+    out[17 * 64 + l] = (double)dtx::quad_lane<1>(iv);
+    out[18 * 64 + l] = (double)dtx::quad_lane<2>(iv);
+    out[19 * 64 + l] = (double)dtx::quad_lane<3>(iv);
+}
+
+extern "C" int lane_probe(const double* h_in, double* h_out) {
+    double *d_in, *d_out;
+    if (hipMalloc(&d_in, 64 * 8) || hipMalloc(&d_out, 20 * 64 * 8)) return -1;
+    if (hipMemcpy(d_in, h_in, 64 * 8, hipMemcpyHostToDevice)) return -2;
+    k_lanes<<<1, 64>>>(d_in, d_out);
+    if (hipGetLastError() != hipSuccess) return -3;
+    if (hipMemcpy(h_out, d_out, 20 * 64 * 8, hipMemcpyDeviceToHost)) return -4;
+    (void)hipFree(d_in); (void)hipFree(d_out);
     return 0;
 }

@@ -62,6 +62,27 @@ def test_mfma_lane_layouts():
                                rtol=1e-5, atol=1e-4)
 
 
+def test_lane_exchanges():
+    """The DPP / permlane lane exchanges of csrc/wave_ops.h move exactly what __shfl_xor and
+    the quad broadcast would: lane l receives lane l ^ m (m = 1..32) or lane 4 (l // 4) + q,
+    bit for bit, on doubles, 64-bit integers and ints."""
+    lib = C.CDLL(str(Path(__file__).parent / "gpu_probe" / "libmfma_probe.so"))
+    lib.lane_probe.argtypes = [C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(64) * 10.0 ** rng.integers(-30, 30, 64)
+    out = np.zeros((20, 64))
+    assert lib.lane_probe(x.ctypes.data, out.ctypes.data) == 0
+    lanes = np.arange(64)
+    u = (lanes.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(40)
+    for i, m in enumerate((1, 2, 4, 8, 16, 32)):
+        assert np.array_equal(out[i].view(np.uint64), x[lanes ^ m].view(np.uint64)), m
+        assert np.array_equal(out[10 + i], u[lanes ^ m].astype(np.float64)), m
+    for q in range(4):
+        src = 4 * (lanes // 4) + q
+        assert np.array_equal(out[6 + q].view(np.uint64), x[src].view(np.uint64)), q
+        assert np.array_equal(out[16 + q], (1000 + 7 * src).astype(np.float64)), q
+
+
 def _solve_both(w, n_save=3, verbose=1, method="euler", mono=None, precision="fp64"):
     m = workloads.make_struct(w, precision=precision, method=method, use_monolithic=mono)
     out = Solve(m, w.dt, w.t_save(n_save), Verbose=verbose)
