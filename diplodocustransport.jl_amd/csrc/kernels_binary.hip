@@ -16,7 +16,7 @@
 // gives bit-identical results alone or grouped with others.  LOSS-layout items share the launch
 // with the gain items of any precision (they take the FP64 body).
 //
-// Item shapes: 64 rows x 16 or 64 cells (four waves = WK (rows) x WC (cells), each owning
+// Item shapes: 64 rows x 16, 32 or 64 cells (four waves = WK (rows) x WC (cells), each owning
 // MR x MC 16x16 MFMA tiles), and for more than 64 cells the wide shapes, 128 or 256 rows x 128
 // cells with 64 x 64 per wave (wide FP64 and split-BF16 sections below).
 // MFMA operand layout (16x16xK): lane l supplies A[l%16][kgroup(l/16)], B[kgroup(l/16)][l%16].
@@ -1321,10 +1321,13 @@ template <int P>
 __global__ __launch_bounds__(256) void bin_mixed_kernel(const BinWork* __restrict__ items) {
     __shared__ union {
         SmemF64<16> d16;
+        SmemF64<32> d32;
         SmemF64<64> d64;
         SmemF32<16> s16;
+        SmemF32<32> s32;
         SmemF32<64> s64;
         SmemBF16<16> b16;
+        SmemBF16<32> b32;
         SmemBF16<64> b64;
         SmemGemv g;
     } sm;
@@ -1336,6 +1339,10 @@ __global__ __launch_bounds__(256) void bin_mixed_kernel(const BinWork* __restric
         if (fp64_body) f64_item<16>(w, sm.d16);
         else if (P == DTX_PREC_FP32) f32_item<16>(w, sm.s16);
         else bf16x3_item<16>(w, sm.b16);
+    } else if (w.tc == 32) {
+        if (fp64_body) f64_item<32>(w, sm.d32);
+        else if (P == DTX_PREC_FP32) f32_item<32>(w, sm.s32);
+        else bf16x3_item<32>(w, sm.b32);
     } else {
         if (fp64_body) f64_item<64>(w, sm.d64);
         else if (P == DTX_PREC_FP32) f32_item<64>(w, sm.s64);
@@ -1362,12 +1369,15 @@ hipError_t launch_binary(int precision, int tc, const BinWork* items, int nitems
         else bin_gemv_kernel<DTX_PREC_BF16X3><<<grid, block, 0, stream>>>(items);
     } else if (precision == DTX_PREC_FP64) {
         if (tc == 16) bin_f64_kernel<16><<<grid, block, 0, stream>>>(items);
+        else if (tc == 32) bin_f64_kernel<32><<<grid, block, 0, stream>>>(items);
         else bin_f64_kernel<64><<<grid, block, 0, stream>>>(items);
     } else if (precision == DTX_PREC_FP32) {
         if (tc == 16) bin_f32_kernel<16><<<grid, block, 0, stream>>>(items);
+        else if (tc == 32) bin_f32_kernel<32><<<grid, block, 0, stream>>>(items);
         else bin_f32_kernel<64><<<grid, block, 0, stream>>>(items);
     } else {
         if (tc == 16) bin_bf16x3_kernel<16><<<grid, block, 0, stream>>>(items);
+        else if (tc == 32) bin_bf16x3_kernel<32><<<grid, block, 0, stream>>>(items);
         else bin_bf16x3_kernel<64><<<grid, block, 0, stream>>>(items);
     }
     return hipGetLastError();

@@ -1540,7 +1540,8 @@ int dtx_create(const dtx_config* cfg, dtx_handle** out) {
     h->precision = cfg->precision;
     h->method = cfg->method;
     // one cell: GEMV items that stream the operator (MFMA tiles would be 15/16 padding)
-    h->tc = h->C == 1 ? 1 : (h->C <= 16) ? 16 : 64;
+    // 17-32 cells take 32-cell tiles: a 64-cell tile would spend half its MFMAs on padding
+    h->tc = h->C == 1 ? 1 : (h->C <= 16) ? 16 : (h->C <= 32) ? 32 : 64;
     // wide gain items above 64 cells: split-BF16 with four k tiles per item from 512 cells on,
     // where the launch still has enough items for one eight-wave workgroup per CU.  FP64 with
     // two k tiles up to 256 cells: with many cells the 64-cell items, four workgroups per CU,

@@ -169,6 +169,16 @@ def test_many_cells_parity_fp64(shape):
         assert rel_linf(out.f[i], fo[i]) < TOL64, i
 
 
+@pytest.mark.parametrize("x_num", [20, 32])
+def test_32_cell_tiles_parity_fp64(x_num):
+    # 17-32 cells: 32-cell tiles (20: a partial tile), 160 states per species in channel form
+    # (three 64-row k tiles, the last partial), with flux
+    w = workloads.transport_1d(p_num=20, u_num=8, x_num=x_num, steps=12, monolithic=False)
+    m, out, fo, to, _ = _solve_both(w, n_save=2, mono=False)
+    for i in range(len(out.f)):
+        assert rel_linf(out.f[i], fo[i]) < TOL64, (x_num, i)
+
+
 @pytest.mark.parametrize("kw", [dict(p_num=20, u_num=8, monolithic=False),
                                 dict(num_species=3, emission=True, injection=True)],
                          ids=["3ktiles", "3species"])

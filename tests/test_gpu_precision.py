@@ -65,13 +65,19 @@ def wide_slices():
     return workloads.tiny(p_num=20, u_num=8, x_num=520, with_flux=False, monolithic=False)
 
 
+def cells_32():
+    """24 cells: 32-cell tiles (one partial tile) and 160 states per species."""
+    return workloads.tiny(p_num=20, u_num=8, x_num=24, with_flux=False, monolithic=False)
+
+
 def single_cell():
     """One cell, 160 states per species: GEMV items over three k tiles."""
     return workloads.tiny(p_num=20, u_num=8, x_num=1, with_flux=False, monolithic=False)
 
 
 CASES = {"host_channels": homogeneous_channels, "device_channels": synthetic_small,
-         "many_cells": many_cells, "wide_slices": wide_slices, "single_cell": single_cell}
+         "many_cells": many_cells, "wide_slices": wide_slices, "single_cell": single_cell,
+         "cells_32": cells_32}
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
