@@ -288,20 +288,37 @@ __global__ __launch_bounds__(UB) void correct_coef_kernel(UpdateArgs a) {
     correction_coef(a, (int)(p / a.S), (int)(p % a.S), a.coef + p * 2);
 }
 
+// The species and energy weight of state k (eweight absent: 0), for finish_block and the gain
+// correction.
+struct StateMeta {
+    int sp;
+    double ew;
+};
+__device__ inline StateMeta state_meta(const UpdateArgs& a, int k, bool mine) {
+    StateMeta m{-1, 0.0};
+    if (mine) {
+        m.sp = a.state_species[k];
+        m.ew = a.eweight ? a.eweight[k] : 0.0;
+    }
+    return m;
+}
+
 // Per-species moment partials of this block's new states and the per-block partials (sums and
 // Cr minima), in one fused reduction.  Thread (tid & 3) == 0 of each state holds its values;
 // the finalize kernels fold the partials in a fixed order.
 // t: the thread's index in its group of UB threads (red: the group's array); write: the group
-// owns a segment (every group of the workgroup takes part in the barrier).
-__device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, bool mine, int k,
-                             double fnew, double* red, double sumdf, double sumf, double cr,
-                             double crb, double cre, double crf, double crm, double crs,
-                             int t = threadIdx.x, bool write = true) {
+// owns a segment (every group of the workgroup takes part in the barrier).  sp and ew: the
+// state's species and energy weight (state_meta), loaded by the caller with the state's other
+// values so that no load waits at the end of the kernel.
+__device__ void finish_block(const UpdateArgs& a, int c, int seg, int nseg, bool mine, int sp_in,
+                             double ew_in, double fnew, double* red, double sumdf, double sumf,
+                             double cr, double crb, double cre, double crf, double crm,
+                             double crs, int t = threadIdx.x, bool write = true) {
     const int tid = t;
     const int64_t b = (int64_t)c * nseg + seg;
     const int S = a.S;
-    const int sp = mine ? a.state_species[k] : -1;
-    const double ew = (mine && a.eweight) ? a.eweight[k] : 0.0;
+    const int sp = mine ? sp_in : -1;
+    const double ew = mine ? ew_in : 0.0;
     for (int s = 0; s < S; ++s) {
         wave_put(s == sp ? fnew : 0.0, 2 * s, true, red, t);
         wave_put((s == sp && a.eweight) ? fnew * ew : 0.0, 2 * s + 1, true, red, t);
@@ -622,6 +639,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
         iap = a.inv_ap[r];
         dinj = a.df_inj[r];
     }
+    const StateMeta meta = state_meta(a, k, mine && q == 0);
     EllLane ell;
     ell_issue(a, r, q, mine, 0, ell);
     double gs = 0.0, ls = 0.0;
@@ -647,9 +665,8 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
             // loss sum (binary_term's corrected form, on the sums loaded above)
             double g = 0.0;
             if (mine && active && q == 0) {
-                const double* cf = coef + 2 * a.state_species[k];
-                const double e = a.eweight ? a.eweight[k] : 0.0;
-                g = gs * (1.0 + (cf[0] + cf[1] * e)) + ls;
+                const double* cf = coef + 2 * meta.sp;
+                g = gs * (1.0 + (cf[0] + cf[1] * meta.ew)) + ls;
             }
             g = g + xor_lane<1>(g);
             g = g + xor_lane<2>(g);
@@ -690,8 +707,8 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
         fnew = x;
         publish(a, c, k, x);
     }
-    finish_block(a, c, seg, nseg, mine && q == 0, k, fnew, red, sumdf, sumf, cr, crb, cre, crf,
-                 INFINITY, INFINITY);
+    finish_block(a, c, seg, nseg, mine && q == 0, meta.sp, meta.ew, fnew, red, sumdf, sumf, cr,
+                 crb, cre, crf, INFINITY, INFINITY);
 }
 
 // Grid (C, nseg).  With batched finalisation, workgroup (0, 0) also records the step's scalars.
@@ -892,6 +909,7 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
     double crm = INFINITY, crs = INFINITY, fnew = 0.0;
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const int k = k0 + (tid >> 2);
+    const StateMeta meta = state_meta(a, k, k < k1 && (tid & 3) == 0);
     if (k < k1 && (tid & 3) == 0) {
         const int64_t r = base + k;
         const double f0 = a.f[r];
@@ -923,9 +941,8 @@ __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
         fnew = x;
         publish(a, c, k, x);
     }
-    finish_block(a, c, blockIdx.y, gridDim.y, k < k1 && (tid & 3) == 0, k, fnew, red, 0.0, sumf,
-                 cr, crb, cre, crf, crm,
-                 crs);
+    finish_block(a, c, blockIdx.y, gridDim.y, k < k1 && (tid & 3) == 0, meta.sp, meta.ew, fnew,
+                 red, 0.0, sumf, cr, crb, cre, crf, crm, crs);
 }
 
 // --------------------------------------------------------- FP64 moment-rate correction -----
@@ -1239,8 +1256,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(Updat
             fnew = x;
             publish(a, c, k, x);
         }
-        finish_block(a, c, seg0 + (live ? z : 0), nseg, mine && q == 0, k, fnew, rg, sumdf, sumf,
-                     cr, crb, cre, crf, INFINITY, INFINITY, t, live);
+        const StateMeta meta = state_meta(a, k, mine && q == 0);
+        finish_block(a, c, seg0 + (live ? z : 0), nseg, mine && q == 0, meta.sp, meta.ew, fnew, rg,
+                     sumdf, sumf, cr, crb, cre, crf, INFINITY, INFINITY, t, live);
         __syncthreads();  // rg is reused by the next pass
     }
 }
