@@ -435,12 +435,23 @@ def main():
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (5 times), the "
                          "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving capacity run (--serving-requests requests queued at "
+                         "t = 0, FP32 grouped kernels): one JSON line with solves/s")
     ap.add_argument("--flux-setup", action="store_true",
                     help="only compare the flux set-up of the workload: host assembly + CSC "
                          "upload against the stencils built on the device (dtx_synth_flux)")
     args = ap.parse_args()
     if args.flux_setup:
         run_flux_setup(args.workload)
+        return
+    if args.serving_only:
+        from dtx import serving
+        specs = serving.request_mix(count=args.serving_requests, rate_per_s=1e9)
+        _, wall, _ = serving.run(specs, precision="fp32", realtime=False)
+        print(json.dumps({"serving_requests": args.serving_requests, "precision": "fp32",
+                          "capacity_solves_per_s": round(args.serving_requests / wall, 3),
+                          "request_steps": sum(sp.steps for sp in specs)}))
         return
     if args.copy_kernel_mib:
         import torch
