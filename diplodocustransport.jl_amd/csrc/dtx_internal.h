@@ -52,7 +52,7 @@ struct BinWork {
     int32_t ldb;              // 16-cell blocks of the BF16 B arrays (ldc / 16)
     int32_t N;                // states per cell (partial buffer row pitch)
     int32_t kind;             // ChannelKind
-    int32_t tc;               // cells per tile of the item (1: GEMV, 16, 64, or WIDE)
+    int32_t tc;               // cells per tile of the item (1: GEMV, 16, 32, 64, or WIDE)
 };
 
 // Packed operator of one channel on the device.

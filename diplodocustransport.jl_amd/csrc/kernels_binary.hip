@@ -1314,7 +1314,7 @@ __global__ __launch_bounds__(256) void bin_gemv_kernel(const BinWork* __restrict
 }
 
 
-// One launch over items of the narrow shapes (single-cell GEMV, 16 and 64 cells): each
+// One launch over items of the narrow shapes (single-cell GEMV, 16, 32 and 64 cells): each
 // workgroup takes the body of its item's shape (the same device code as the single-shape kernels, so a result does not depend
 // on the launch it ran in).
 template <int P>

@@ -1409,9 +1409,9 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
     }
     g->n_corr = (int)ctable.size();
     // The narrow shapes go into one launch (list 0): GEMV items of single-cell requests and
-    // 16-cell items stream their operator (HBM-bound) while 64-cell items are MFMA-bound, so the
-    // two lists are interleaved in proportion to run side by side.  Each member's items keep
-    // their order.
+    // 16-cell items stream their operator (HBM-bound) while 32- and 64-cell items are
+    // MFMA-bound, so the two lists are interleaved in proportion to run side by side.  Each
+    // member's items keep their order.
     {
         std::vector<BinWork> mixed;
         mixed.reserve(lists[0].size() + lists[1].size());
