@@ -12,8 +12,9 @@ records in it; the kernel durations of the roofline come from HIP events the lib
 its own stream around every step's kernels in a separate, untimed pass.  Beside it: the FP32-MFMA
 and split-BF16 paths (speed, and their relative error and number / energy drift against FP64
 over the configuration's full step count), the same comparison at the largest configuration
-(transport_2d, where the split-BF16 contraction is MFMA-bound), the serving loop, the
-collision-matrix ingest, and the FP64 CPU oracle timed on a bounded sample on the host cores.
+(transport_2d, where the split-BF16 contraction is MFMA-bound), the emission GEMV's HBM rate
+(transport_1d with dense per-cell emission blocks), the serving loop, the collision-matrix
+ingest, and the FP64 CPU oracle timed on a bounded sample on the host cores.
 """
 import argparse
 import json
@@ -115,6 +116,8 @@ def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True, accu
     res = dict(ms_per_step=(t1 - t0) * 1e3 / steps, steps_per_s=steps / (t1 - t0),
                bin_ms=kt.bin_ms / max(1, kt.bin_launches),
                update_ms=kt.update_ms / max(1, kt.update_launches),
+               emission_ms=kt.emission_ms / max(1, kt.emission_launches),
+               emission_bytes=work.emission_bytes,
                bin_flops=work.bin_flops, gain_bytes=work.gain_bytes, bin_bytes=work.bin_bytes,
                update_bytes=work.update_bytes, items=work.bin_launch_items,
                device_gb=m.device_bytes() / 1e9)
@@ -164,6 +167,26 @@ def run_largest(steps, warmup):
                          speedup_vs_fp64=round(r["steps_per_s"] / base["steps_per_s"], 3),
                          binary_kernel_us=round(r["bin_ms"] * 1e3, 1), **accuracy(r, base))
     return out
+
+
+def run_emission(steps, warmup):
+    """transport_1d with dense per-cell emission blocks (128 x 512^2 FP64, 268 MB streamed per
+    step): the FP64 step rate and the emission GEMV's HBM rate against the 8 TB/s peak."""
+    from dtx import workloads
+    w = workloads.transport_1d(emission=True)
+    log(f"[bench] transport_1d + emission: {warmup} warmup + {steps} timed steps")
+    r = run_gpu(w, "fp64", steps, warmup)
+    gbs = r["emission_bytes"] / (r["emission_ms"] * 1e-3) / 1e9
+    log(f"[bench] emission GEMV {r['emission_ms'] * 1e3:.1f} us, {gbs:.0f} GB/s")
+    return dict(workload="transport_1d with dense per-cell emission blocks (128 cells x "
+                         "512 x 512 FP64)",
+                steps_per_s=round(r["steps_per_s"], 2),
+                kernel="emission_gemv_kernel<2>",
+                kernel_avg_us=round(r["emission_ms"] * 1e3, 2),
+                bytes_per_launch=r["emission_bytes"],
+                achieved_gbs=round(gbs, 1), peak_gbs=PEAK_HBM_GBS,
+                frac=round(gbs / PEAK_HBM_GBS, 4),
+                update_kernel_us=round(r["update_ms"] * 1e3, 2))
 
 
 def fp64_kernels_per_step(w):
@@ -430,6 +453,8 @@ def main():
                     help="skip the full-length accuracy runs of the precision modes")
     ap.add_argument("--no-largest", action="store_true",
                     help="skip the transport_2d FP64 / FP32 / split-BF16 comparison")
+    ap.add_argument("--no-emission", action="store_true",
+                    help="skip transport_1d with emission blocks (the emission GEMV's HBM rate)")
     ap.add_argument("--largest-steps", type=int, default=50,
                     help="timed steps per precision of the transport_2d comparison")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
@@ -530,6 +555,9 @@ def main():
                     traffic=TRAFFIC.get((w.name, kernel)), kernel=kernel,
                     flops_per_launch=base["bin_flops"], bytes_per_launch=base["bin_bytes"],
                     kernel_avg_us=round(base["bin_ms"] * 1e3, 2))
+    emission = None
+    if not args.no_emission and args.workload == "transport_1d":
+        emission = run_emission(200, 10)
     largest = None
     if not args.no_largest and args.workload == "transport_1d":
         largest = run_largest(args.largest_steps, 5)
@@ -569,6 +597,7 @@ def main():
         "update_kernel_us": round(base["update_ms"] * 1e3, 2),
         "roofline": roof,
         "largest_solve": largest,
+        "emission": emission,
         "serving": serving_line,
         "collision_ingest": ingest,
         "cpu_baseline": cpu,

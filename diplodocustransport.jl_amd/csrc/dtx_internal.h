@@ -110,6 +110,8 @@ struct UpdateArgs {
     const int32_t* nslot32; // leading slots of each state holding FP32 sums (N), or null
     int64_t part_stride;    // doubles between slots
     const double* emi;      // emission blocks [cell][N x N] or null
+    double* emi_part;       // emission GEMV part sums [emi_q][cell][N] when emi_q > 1
+    int32_t emi_q;          // workgroups per (cell, row tile) of the emission GEMV
     const uint8_t* active;  // binary domain per cell (1 = evaluated) or null = all
     const int32_t* ell_col; const double* ell_val; int32_t ell_width;
     const double* eweight;  // per-state energy weight (N) or null
@@ -125,7 +127,8 @@ struct UpdateArgs {
     double* df_mom;         // df_Momentum (scaled)
     double* dpflux;         // df_PFlux (unscaled) kept for Cr_PFlux
     double* dbin;           // df_Bin (unscaled) kept for Cr_Bin
-    double* demi;           // df_Emi (unscaled) kept for Cr_Emi
+    double* demi;           // df_Emi (unscaled) of the step's emission GEMV (n), read by the
+                            // update kernels and by the space half's Cr_Emi
     int64_t n;
     int32_t N, C, S, ldc;
     int32_t binary, emission, verbose;
@@ -144,6 +147,10 @@ struct UpdateArgs {
 constexpr int FIN_BATCH = 64;
 constexpr int FIN_WORDS = 8;
 
+constexpr int EMI_MAX_N = 16384;   // states per cell with emission blocks (f of a cell in LDS)
+// Workgroups of the emission GEMV per (cell, row tile): enough for two per CU over the whole
+// grid, each keeping at least 64 columns
+int emission_parts(int N, int C, int num_cu);
 constexpr int UPDATE_STATES = 64;  // states per update workgroup (four threads each)
 constexpr int UPDATE_THREADS = 256;  // threads per update and finalize workgroup
 constexpr int STATUS_RING = 4096;
@@ -227,9 +234,11 @@ hipError_t launch_finalize_batch(const UpdateArgs& base, int64_t bp_words, int f
 bool finalize_is_single(const UpdateArgs& a);
 hipError_t launch_finalize(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream);
+// df_Emi = M_Emi f over the per-cell blocks into a.demi (nothing without emission)
+hipError_t launch_emission(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream);
 hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* block_start,
-                                    int count, int total_blocks, int max_N, hipStream_t stream);
+                                    int count, int total_blocks, hipStream_t stream);
 hipError_t launch_fill_zero(double* p, int64_t count, hipStream_t stream);
 hipError_t launch_publish_state(const double* f, int N, int C, int ldc, const int32_t* pad_index,
                                 double* fT, float* fT32, uint16_t* fhi, uint16_t* flo,

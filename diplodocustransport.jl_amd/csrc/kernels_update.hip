@@ -165,11 +165,158 @@ __device__ inline void publish(const UpdateArgs& a, int c, int k, double x) {
     }
 }
 
-__device__ inline double emission_term(const UpdateArgs& a, int c, int k, const double* fc) {
-    const double* E = a.emi + (int64_t)c * a.N * a.N;
-    double acc = 0.0;
-    for (int j = 0; j < a.N; ++j) acc = acc + E[k + (int64_t)a.N * j] * fc[j];
-    return acc;
+// ------------------------------------------------------------------ emission GEMV ---------
+// df_Emi = M_Emi f (src/SteppingMethods.jl:35-38) over the per-cell N x N blocks, into demi
+// before the update reads it.  The blocks are column-major: a wave reads 64 R consecutive rows
+// of one column per load (R = 2, 16-byte loads of two rows, when N is even) and EMI_ROWS = 2
+// such row groups per column, so each column is read in runs of 2 KB.  Workgroup (cell, column
+// range q of Q, tile of 128 R rows): EMI_WAVES waves, wave p summing its share of the range in
+// ascending column order.  The loads run EMI_BATCH columns at a time in two alternating register
+// sets, so the next batch is in flight while the current one is added; the cell's f sits in LDS
+// (every lane reads the same f_j).  The wave sums are added pairwise (((s0 + s1) + (s2 + s3)) +
+// ...).  Q keeps the launch at about one workgroup per CU when there are few cells; with Q > 1
+// the range sums go to emi_part and emission_sum_kernel adds them in range order.  Measured at
+// 128 cells x 512 states (268 MB): 1 KB runs (one row group) 60 us, 2 KB 50 us, 4 KB 55 us; two
+// workgroups per CU (Q = 2) 67 us; 8 to 16 waves and batches of 2 to 8 columns alike.
+constexpr int EMI_WAVES = 8;
+constexpr int EMI_BATCH = 4;
+constexpr int EMI_ROWS = 2;   // loads of 64 R rows per lane and column
+constexpr int EMI_WG_PER_CU = 1;
+
+int emission_parts(int N, int C, int num_cu) {
+    const int64_t tiles = (int64_t)C * ((N + 128 * EMI_ROWS - 1) / (128 * EMI_ROWS));
+    const int64_t want = (EMI_WG_PER_CU * (int64_t)num_cu + tiles - 1) / tiles;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, N / 64));
+}
+static_assert((EMI_WAVES & (EMI_WAVES - 1)) == 0, "part sums are added pairwise");
+
+template <int R, int T>
+__global__ __launch_bounds__(64 * EMI_WAVES) void emission_gemv_kernel(UpdateArgs a) {
+    extern __shared__ double fl[];   // f of the cell
+    __shared__ double part[EMI_WAVES][64 * R * T];
+    constexpr int B = EMI_BATCH;
+    constexpr int RW = 64 * R;         // rows of one load of the wave
+    const int N = a.N, Q = a.emi_q, c = blockIdx.x / Q, q = blockIdx.x % Q;
+    const int lane = threadIdx.x & 63;
+    const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int kt = blockIdx.y * RW * T + lane * R;   // rows kt + RW t (+ 1 when R = 2) of the lane
+    const int q0 = (int)((int64_t)N * q / Q), q1 = (int)((int64_t)N * (q + 1) / Q);
+    const int j0 = q0 + (int)((int64_t)(q1 - q0) * p / EMI_WAVES);
+    const int j1 = q0 + (int)((int64_t)(q1 - q0) * (p + 1) / EMI_WAVES);
+    const double* E = a.emi + (int64_t)c * N * N;
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    double ea[B][T][R], eb[B][T][R];
+    // columns jc .. jc + B - 1 into e; past j1 the last column is read again and not added, and
+    // rows past N read row 0 and are not written, so the loads carry no branch and the compiler
+    // counts them exactly
+    auto issue = [&](int jc, double (&e)[B][T][R]) {
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int k = kt + RW * t;
+                const double* col = E + (int64_t)min(jc + u, j1 - 1) * N + (k < N ? k : 0);
+                if constexpr (R == 2) {
+                    const d2v x = *(const d2v*)col;
+                    e[u][t][0] = x[0];
+                    e[u][t][1] = x[1];
+                } else {
+                    e[u][t][0] = col[0];
+                }
+            }
+    };
+    double acc[T][R];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[t][r] = 0.0;
+    auto consume = [&](int jc, const double (&e)[B][T][R]) {
+        double fj[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) fj[u] = fl[min(jc + u, j1 - 1)];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const bool in = jc + u < j1;
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double x = acc[t][r] + e[u][t][r] * fj[u];
+                    acc[t][r] = in ? x : acc[t][r];
+                }
+        }
+    };
+    if (j0 < j1) issue(j0, ea);
+    const double* fc = a.f + (int64_t)c * N;
+    for (int j = threadIdx.x; j < N; j += 64 * EMI_WAVES) fl[j] = fc[j];
+    __syncthreads();
+    // the next batch is issued before the current one is added, also past j1 (a few repeated
+    // loads of the last column at the end), so every wait is for one batch exactly; the
+    // scheduling barriers keep the compiler from sinking the loads below the adds
+    for (int jc = j0; jc < j1;) {
+        issue(jc + B, eb);
+        __builtin_amdgcn_sched_barrier(0);
+        consume(jc, ea);
+        jc += B;
+        if (jc >= j1) break;
+        issue(jc + B, ea);
+        __builtin_amdgcn_sched_barrier(0);
+        consume(jc, eb);
+        jc += B;
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int r = 0; r < R; ++r) part[p][RW * t + lane * R + r] = acc[t][r];
+    __syncthreads();
+    if (p == 0)
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = RW * t + lane * R + r, k = kt + RW * t + r;
+                double v[EMI_WAVES];
+#pragma unroll
+                for (int w = 0; w < EMI_WAVES; ++w) v[w] = part[w][i];
+#pragma unroll
+                for (int h = EMI_WAVES / 2; h >= 1; h /= 2)
+#pragma unroll
+                    for (int w = 0; w < h; ++w) v[w] = v[2 * w] + v[2 * w + 1];
+                if (kt + RW * t < N) {
+                    if (Q == 1) a.demi[(int64_t)c * N + k] = v[0];
+                    else a.emi_part[((int64_t)q * a.C + c) * N + k] = v[0];
+                }
+            }
+}
+
+// demi = the Q range sums of each state added in range order
+__global__ void emission_sum_kernel(UpdateArgs a) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double s = a.emi_part[i];
+        for (int q = 1; q < a.emi_q; ++q) s = s + a.emi_part[(int64_t)q * a.n + i];
+        a.demi[i] = s;
+    }
+}
+
+hipError_t launch_emission(const UpdateArgs& a, hipStream_t stream) {
+    if (!a.emission) return hipSuccess;
+    if (a.N > EMI_MAX_N || a.emi_q < 1 || (a.emi_q > 1 && !a.emi_part))
+        return hipErrorInvalidValue;
+    constexpr int T = EMI_ROWS;
+    const size_t lds = (size_t)a.N * sizeof(double);
+    if (a.N % 2 == 0) {
+        const dim3 grid(a.C * a.emi_q, (a.N + 128 * T - 1) / (128 * T));
+        emission_gemv_kernel<2, T><<<grid, 64 * EMI_WAVES, lds, stream>>>(a);
+    } else {
+        const dim3 grid(a.C * a.emi_q, (a.N + 64 * T - 1) / (64 * T));
+        emission_gemv_kernel<1, T><<<grid, 64 * EMI_WAVES, lds, stream>>>(a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.emi_q == 1) return e;
+    const int64_t blocks = std::min<int64_t>((a.n + 255) / 256, 4096);
+    emission_sum_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a);
+    return hipGetLastError();
 }
 
 // Values reduced together by one workgroup: per-species number and energy sums, then the block
@@ -553,18 +700,13 @@ __device__ void finalize_single(const UpdateArgs& a, int nseg, double* red,
     }
 }
 
-// Load the whole cell into LDS (needed by the emission GEMV and the Domain all-zero test) and
-// return whether any entry is non-zero.
+// Whether any entry of cell c's f is non-zero (the Domain all-zero test), over the workgroup.
 template <int NT = UB>
-__device__ int load_cell(const UpdateArgs& a, int c, double* fc, int* ired) {
+__device__ int cell_nonzero(const UpdateArgs& a, int c, int* ired) {
     const int64_t base = (int64_t)c * a.N;
     int nz = 0;
-    for (int k = threadIdx.x; k < a.N; k += NT) {
-        const double v = a.f[base + k];
-        fc[k] = v;
-        nz |= (v != 0.0);
-    }
-    return block_or<NT>(nz, ired);  // also orders the fc writes before use
+    for (int k = threadIdx.x; k < a.N; k += NT) nz |= (a.f[base + k] != 0.0);
+    return block_or<NT>(nz, ired);
 }
 
 // ----------------------------------------------------------------------- ForwardEuler ------
@@ -620,9 +762,8 @@ __device__ inline double ell_sum(const UpdateArgs& a, const EllLane& L, int s0, 
     return ell_sum_x(L, x, s0, a.ell_width, acc);
 }
 
-__device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg, double* smem,
-                                  double* red, int* ired) {
-    double* fc = smem + UPDATE_STATES;    // f of the whole cell (emission / Domain only)
+__device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg, double* red,
+                                  int* ired) {
     const int tid = threadIdx.x, N = a.N;
     const int k0 = seg * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
@@ -630,14 +771,16 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
     const bool mine = k < k1;
     const int64_t r = base + k;
     // Every load that does not depend on another is issued before the first wait: the state's
-    // own values, its flux row's columns and values (over the four lanes of the state), and,
-    // with the moment-rate correction, its gain and loss sums; the workgroup barriers below
-    // (cell load, coefficients) wait only for LDS, so these stay in flight across them.
-    double fk = 0.0, iap = 0.0, dinj = 0.0;
+    // own values (with its emission term), its flux row's columns and values (over the four
+    // lanes of the state), and, with the moment-rate correction, its gain and loss sums; the
+    // workgroup barriers below (Domain test, coefficients) wait only for LDS, so these stay in
+    // flight across them.
+    double fk = 0.0, iap = 0.0, dinj = 0.0, dE = 0.0;
     if (mine && q == 0) {
         fk = a.f[r];
         iap = a.inv_ap[r];
         dinj = a.df_inj[r];
+        if (a.emission) dE = a.demi[r];
     }
     const StateMeta meta = state_meta(a, k, mine && q == 0);
     EllLane ell;
@@ -647,8 +790,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
         gs = a.gsum[r];
         ls = a.part[r];
     }
-    const bool need_cell = a.emission || (a.binary && a.active);
-    const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
+    const int any_nz = (a.binary && a.active) ? cell_nonzero(a, c, ired) : 1;
     // update_Big_Bin! evaluates a cell when Domain is nothing, the cell is in Domain, or its f
     // is all zero (src/SteppingMethods.jl:369); otherwise df_Bin keeps its zero initial value.
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
@@ -684,11 +826,7 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
     if (mine && q == 0) {
         sumf = fk;
         double df = -dF;
-        double dE = 0.0;
-        if (a.emission) {
-            dE = emission_term(a, c, k, fc);
-            df = df + dE;
-        }
+        if (a.emission) df = df + dE;
         if (a.binary) df = df + dB;
         df = df * (iap * a.s);
         sumdf = df;
@@ -713,7 +851,6 @@ __device__ void euler_update_body(const UpdateArgs& a, int c, int seg, int nseg,
 
 // Grid (C, nseg).  With batched finalisation, workgroup (0, 0) also records the step's scalars.
 __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
-    extern __shared__ double smem[];
     __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
     if (a.fin_rec && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
@@ -724,7 +861,7 @@ __global__ __launch_bounds__(UB) void euler_update_kernel(UpdateArgs a) {
         rec[3] = (double)a.save;
         rec[4] = (double)a.verbose;
     }
-    euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, smem, red, ired);
+    euler_update_body(a, blockIdx.x, blockIdx.y, gridDim.y, red, ired);
 }
 
 // Batched finalisation: workgroup i finalises the pending step in ring slot (first + i) mod
@@ -835,7 +972,6 @@ __device__ inline int group_member(const int32_t* start, int count, int b) {
 __global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs* table,
                                                                 const int32_t* block_start,
                                                                 int count) {
-    extern __shared__ double smem[];
     __shared__ double red[(UB / 64) * RED_VALUES];
     __shared__ int ired[UB / 64];
     const int b = blockIdx.x;
@@ -843,7 +979,7 @@ __global__ __launch_bounds__(UB) void euler_update_group_kernel(const UpdateArgs
     const UpdateArgs a = table[lo];
     const int nseg = (a.N + UPDATE_STATES - 1) / UPDATE_STATES;
     const int local = b - block_start[lo];
-    euler_update_body(a, local / nseg, local % nseg, nseg, smem, red, ired);
+    euler_update_body(a, local / nseg, local % nseg, nseg, red, ired);
 }
 
 // One workgroup per simulation, with the arithmetic the simulation gets when stepped alone
@@ -859,17 +995,14 @@ __global__ __launch_bounds__(UB) void finalize_group_kernel(const UpdateArgs* ta
 }
 
 // ------------------------------------------------------------- ForwardSymplecticEuler ------
-// Momentum half (src/SteppingMethods.jl:137-160): P flux, emission and binary terms from f,
-// f_Momentum = f + df_Momentum clamped at 1f-10.
+// Momentum half (src/SteppingMethods.jl:137-160): P flux, emission (demi, from the emission
+// GEMV) and binary terms from f, f_Momentum = f + df_Momentum clamped at 1f-10.
 __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
-    extern __shared__ double smem[];
-    double* fc = smem + UPDATE_STATES;
     __shared__ int ired[UB / 64];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
     const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
     const int64_t base = (int64_t)c * N;
-    const bool need_cell = a.emission || (a.binary && a.active);
-    const int any_nz = need_cell ? load_cell(a, c, fc, ired) : 1;
+    const int any_nz = (a.binary && a.active) ? cell_nonzero(a, c, ired) : 1;
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
     const int k = k0 + (tid >> 2), q = tid & 3;
     const bool mine = k < k1;
@@ -881,11 +1014,7 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
         const double dP = a.ell_width ? ell_row(a.ell_col, a.ell_val, a.ell_width, a.n, r, a.f)
                                       : 0.0;
         double dm = -dP;
-        if (a.emission) {
-            const double dE = emission_term(a, c, k, fc);
-            dm = dm + dE;
-            a.demi[r] = dE;
-        }
+        if (a.emission) dm = dm + a.demi[r];
         if (a.binary) {
             dm = dm + dB;
             a.dbin[r] = dB;
@@ -900,7 +1029,6 @@ __global__ __launch_bounds__(UB) void symplectic_momentum_kernel(UpdateArgs a) {
 
 // Space half and final update (src/SteppingMethods.jl:162-237).  The ELL fields hold X_Flux.
 __global__ __launch_bounds__(UB) void symplectic_space_kernel(UpdateArgs a) {
-    extern __shared__ double smem[];
     __shared__ double red[(UB / 64) * RED_VALUES];
     const int c = blockIdx.x, tid = threadIdx.x, N = a.N;
     const int k0 = blockIdx.y * UPDATE_STATES, k1 = min(N, k0 + UPDATE_STATES);
@@ -1100,25 +1228,25 @@ constexpr int FUSED_PASSES = 4;
 constexpr int FUSED_SEGMENTS = FUSED_GROUPS * FUSED_PASSES;  // 1024 states per species
 
 struct StateLoads {
-    double fk, iap, dinj;
+    double fk, iap, dinj, dE;
     EllLane ell;
     double x[ELL_LANE];  // gathered f of the row's first slots (ell_gather)
 };
 
 __device__ inline void state_loads(const UpdateArgs& a, int64_t r, int q, bool mine,
                                    StateLoads& L) {
-    L.fk = L.iap = L.dinj = 0.0;
+    L.fk = L.iap = L.dinj = L.dE = 0.0;
     if (mine && q == 0) {
         L.fk = a.f[r];
         L.iap = a.inv_ap[r];
         L.dinj = a.df_inj[r];
+        if (a.emission) L.dE = a.demi[r];
     }
     ell_issue(a, r, q, mine, 0, L.ell);
 }
 
 __global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(UpdateArgs a,
                                                                            CorrectArgs ca) {
-    extern __shared__ double fc[];  // f of the whole cell (emission / Domain only)
     __shared__ double red[(FUSED_THREADS / 64) * RED_VALUES];
     __shared__ double cpl[FUSED_SEGMENTS * 3];
     __shared__ double coefl[2];
@@ -1190,9 +1318,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(Updat
         cpart_sums(cpl, nz, tid, cv);
         coef_solve(a, cv, tid, coefl);
     }
-    const bool need_cell = a.emission || (a.binary && a.active);
-    const int any_nz = need_cell ? load_cell<FUSED_THREADS>(a, c, fc, ired) : 1;
-    if (!need_cell) __syncthreads();  // coefl (load_cell ends with barriers)
+    const bool need_cell = a.binary && a.active;
+    const int any_nz = need_cell ? cell_nonzero<FUSED_THREADS>(a, c, ired) : 1;
+    if (!need_cell) __syncthreads();  // coefl (cell_nonzero ends with barriers)
     const bool active = a.binary && (a.active == nullptr || a.active[c] || !any_nz);
     const bool want_cr = a.verbose >= 1 && a.verbose <= 3;
     const double cf0 = coefl[0], cf1 = coefl[1];
@@ -1232,12 +1360,9 @@ __global__ __launch_bounds__(FUSED_THREADS) void euler_update_fused_kernel(Updat
         if (mine && q == 0) {
             const double fk = ld.fk;
             sumf = fk;
+            const double dE = ld.dE;
             double df = -dF;
-            double dE = 0.0;
-            if (a.emission) {
-                dE = emission_term(a, c, k, fc);
-                df = df + dE;
-            }
+            if (a.emission) df = df + dE;
             if (a.binary) df = df + dB;
             df = df * (ld.iap * a.s);
             sumdf = df;
@@ -1275,9 +1400,7 @@ bool fused_update_fits(int N, int S, const int32_t* species_off, int tps) {
 
 hipError_t launch_euler_update_fused(const UpdateArgs& a, const CorrectArgs& ca, bool finalize,
                                      hipStream_t stream) {
-    const bool need_cell = a.emission || (a.binary && a.active);
-    const size_t smem = need_cell ? (size_t)a.N * sizeof(double) : 0;
-    euler_update_fused_kernel<<<dim3(a.C, a.S), FUSED_THREADS, smem, stream>>>(a, ca);
+    euler_update_fused_kernel<<<dim3(a.C, a.S), FUSED_THREADS, 0, stream>>>(a, ca);
     hipError_t e = hipGetLastError();
     return (e != hipSuccess || !finalize) ? e : launch_finalize(a, stream);
 }
@@ -1315,10 +1438,6 @@ __global__ void fill_zero_kernel(double* p, int64_t count) {
         p[i] = 0.0;
 }
 
-static size_t update_smem(const UpdateArgs& a) {
-    return (size_t)(UPDATE_STATES + a.N) * sizeof(double);
-}
-
 static dim3 update_grid(const UpdateArgs& a) {
     return dim3(a.C, (a.N + UPDATE_STATES - 1) / UPDATE_STATES);
 }
@@ -1350,7 +1469,7 @@ static hipError_t launch_coef(const UpdateArgs& a, hipStream_t stream) {
 hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
     hipError_t e0 = launch_coef(a, stream);
     if (e0 != hipSuccess) return e0;
-    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
+    euler_update_kernel<<<update_grid(a), UB, 0, stream>>>(a);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize(a, stream);
 }
@@ -1358,7 +1477,7 @@ hipError_t launch_euler_update(const UpdateArgs& a, hipStream_t stream) {
 hipError_t launch_euler_update_only(const UpdateArgs& a, hipStream_t stream) {
     hipError_t e0 = launch_coef(a, stream);
     if (e0 != hipSuccess) return e0;
-    euler_update_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
+    euler_update_kernel<<<update_grid(a), UB, 0, stream>>>(a);
     return hipGetLastError();
 }
 
@@ -1373,21 +1492,20 @@ hipError_t launch_finalize_batch(const UpdateArgs& base, int64_t bp_words, int f
 hipError_t launch_symplectic_momentum(const UpdateArgs& a, hipStream_t stream) {
     hipError_t e0 = launch_coef(a, stream);
     if (e0 != hipSuccess) return e0;
-    symplectic_momentum_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
+    symplectic_momentum_kernel<<<update_grid(a), UB, 0, stream>>>(a);
     return hipGetLastError();
 }
 
 hipError_t launch_symplectic_space(const UpdateArgs& a, hipStream_t stream) {
-    symplectic_space_kernel<<<update_grid(a), UB, update_smem(a), stream>>>(a);
+    symplectic_space_kernel<<<update_grid(a), UB, 0, stream>>>(a);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize(a, stream);
 }
 
 hipError_t launch_euler_update_group(const UpdateArgs* table, const int32_t* block_start,
-                                    int count, int total_blocks, int max_N, hipStream_t stream) {
+                                    int count, int total_blocks, hipStream_t stream) {
     if (count <= 0) return hipSuccess;
-    euler_update_group_kernel<<<total_blocks, UB, (size_t)(UPDATE_STATES + max_N) * 8, stream>>>(
-        table, block_start, count);
+    euler_update_group_kernel<<<total_blocks, UB, 0, stream>>>(table, block_start, count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     finalize_group_kernel<<<count, UB, 0, stream>>>(table);

@@ -41,7 +41,7 @@ struct HostChannel {
 
 struct TimedEvent {
     hipEvent_t start = nullptr, stop = nullptr;
-    int cls = 0;  // 0 binary, 1 update
+    int cls = 0;  // 0 binary, 1 update, 2 emission GEMV
 };
 
 }  // namespace
@@ -141,6 +141,8 @@ struct dtx_handle {
     uint8_t* d_active = nullptr;
     EllMatrix ell[3];
     double* d_emi = nullptr;
+    double* d_emi_part = nullptr;  // emission GEMV part sums (emi_q > 1)
+    int32_t emi_q = 1;
     double *d_fmom = nullptr, *d_dfmom = nullptr, *d_dpflux = nullptr, *d_dbin = nullptr,
            *d_demi = nullptr;
     bool vectors_set = false;
@@ -988,6 +990,8 @@ static UpdateArgs make_args(dtx_handle* h, double s, int verbose, double t_start
         else a.mpart = h->scratch + h->part_stride * h->part_slots;
     }
     a.emi = h->d_emi;
+    a.emi_part = h->d_emi_part;
+    a.emi_q = h->emi_q;
     a.active = h->d_active;
     const EllMatrix& E = h->ell[which_ell];
     a.ell_col = E.col;
@@ -1076,6 +1080,15 @@ static int enqueue_step(dtx_handle* h, double t_start, double t_stop, double dt,
         if (use_scratch) fca.part = h->scratch;
     }
     TimedEvent ev{};
+    if (h->emission) {
+        // df_Emi of the state the step starts from, read by the update kernels
+        int rc = timing ? timed_begin(h, 2, &ev) : DTX_OK;
+        if (rc) return rc;
+        const UpdateArgs ea = make_args(h, s, verbose, t_start, dt, save, DTX_FLUX_F, use_scratch);
+        HIP_CHECK(h, launch_emission(ea, h->stream));
+        rc = timing ? timed_end(h, &ev) : DTX_OK;
+        if (rc) return rc;
+    }
     if (bin) {
         int rc = timing ? timed_begin(h, 0, &ev) : DTX_OK;
         if (rc) return rc;
@@ -1225,7 +1238,7 @@ struct Group {
     BinWork* d_items[4] = {nullptr, nullptr, nullptr, nullptr};
     int n_items[4] = {0, 0, 0, 0};
     int32_t* d_block_start = nullptr;
-    int total_blocks = 0, max_N = 0;
+    int total_blocks = 0;
     CorrectArgs* d_ctable = nullptr;
     int32_t* d_cstart = nullptr;
     int n_corr = 0, corr_blocks = 0;
@@ -1374,7 +1387,6 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
     std::vector<int32_t> starts, cstart, coef_start;
     std::vector<CorrectArgs> ctable;
     g->total_blocks = 0;
-    g->max_N = 0;
     g->corr_blocks = 0;
     g->coef_blocks = 0;
     for (int m = 0; m < count; ++m) {
@@ -1396,7 +1408,6 @@ int group_set(Group* g, dtx_handle* const* hs, int count) {
         if (h->binary) lists[q].insert(lists[q].end(), h->items_host.begin(), h->items_host.end());
         starts.push_back(g->total_blocks);
         g->total_blocks += h->C * ((h->N + UPDATE_STATES - 1) / UPDATE_STATES);
-        g->max_N = std::max(g->max_N, h->N);
         if (h->binary && h->nitems > 0 && h->correct_active) {
             ctable.push_back(make_correct_args(h));
             cstart.push_back(g->corr_blocks);
@@ -1474,9 +1485,12 @@ int group_step(Group* g, const double* dt, int verbose) {
                                       g->stream);
     if (e == hipSuccess)
         e = launch_coef_group(g->d_table, g->d_coef_start, count, g->coef_blocks, g->stream);
+    // members with emission blocks: their GEMV, one launch each
+    for (int m = 0; m < count && e == hipSuccess; ++m)
+        e = launch_emission(table[(size_t)m], g->stream);
     if (e == hipSuccess)
         e = launch_euler_update_group(g->d_table, g->d_block_start, count, g->total_blocks,
-                                      g->max_N, g->stream);
+                                      g->stream);
     if (e != hipSuccess) {
         g->err = std::string("group step: ") + hipGetErrorString(e);
         return DTX_ERR_HIP;
@@ -1988,9 +2002,21 @@ int dtx_set_binary_domain(dtx_handle* h, int64_t count, const int64_t* domain) {
 int dtx_set_emission_blocks(dtx_handle* h, const double* blocks) {
     if (!h) return DTX_ERR_INVALID;
     if (!blocks) return fail(h, DTX_ERR_INVALID, "null emission blocks");
+    if (h->N > EMI_MAX_N)
+        return fail(h, DTX_ERR_UNSUPPORTED, "emission blocks need at most %d states per cell",
+                    EMI_MAX_N);
     const int64_t count = (int64_t)h->C * h->N * h->N;
     if (!h->d_emi) {
         int rc = dalloc(h, &h->d_emi, count);
+        if (rc) return rc;
+    }
+    if (!h->d_demi) {  // the GEMV's output (symplectic handles hold it already)
+        int rc = dalloc(h, &h->d_demi, h->n);
+        if (rc) return rc;
+    }
+    h->emi_q = emission_parts(h->N, h->C, h->num_cu);
+    if (h->emi_q > 1 && !h->d_emi_part) {  // the part sums of column ranges
+        int rc = dalloc(h, &h->d_emi_part, (int64_t)h->emi_q * h->n);
         if (rc) return rc;
     }
     int rc = upload(h, h->d_emi, blocks, count);
@@ -2376,6 +2402,7 @@ int dtx_work(const dtx_handle* hc, dtx_work_info* out) {
     double ell_bytes = 0;
     for (int w = 0; w < 3; ++w) ell_bytes += (double)h->ell[w].width * h->ell[w].n * 12.0;
     out->update_bytes = h->n * 8.0 * 5 + ell_bytes + (double)h->max_slots * C * N * 8.0;
+    out->emission_bytes = h->emission ? (double)C * N * N * 8.0 + h->n * 16.0 : 0.0;
     return DTX_OK;
 }
 
@@ -2388,6 +2415,9 @@ int dtx_kernel_timing(dtx_handle* h, int32_t enable, int32_t reset, dtx_kernel_t
         if (ev.cls == 0) {
             h->times.bin_ms += ms;
             h->times.bin_launches++;
+        } else if (ev.cls == 2) {
+            h->times.emission_ms += ms;
+            h->times.emission_launches++;
         } else {
             h->times.update_ms += ms;
             h->times.update_launches++;

@@ -48,12 +48,13 @@ class StepStats(C.Structure):
 
 class WorkInfo(C.Structure):
     _fields_ = [("bin_flops", D), ("bin_bytes", D), ("update_bytes", D), ("gain_bytes", D),
-                ("pairs", I64), ("bin_launch_items", I32), ("fused_correction", I32)]
+                ("pairs", I64), ("bin_launch_items", I32), ("fused_correction", I32),
+                ("emission_bytes", D)]
 
 
 class KernelTimes(C.Structure):
     _fields_ = [("bin_ms", D), ("update_ms", D), ("bin_launches", I64),
-                ("update_launches", I64)]
+                ("update_launches", I64), ("emission_ms", D), ("emission_launches", I64)]
 
 
 class ServerConfig(C.Structure):

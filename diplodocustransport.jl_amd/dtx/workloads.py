@@ -138,8 +138,22 @@ def homogeneous_box(p_num=32, u_num=8, h_num=1, steps=1000, seed=None, monolithi
                     seed, gains, losses, M)
 
 
+def emission_blocks(ps, seed, dt, Ap=None):
+    """Dense per-cell emission blocks [cell][column j][row i] (each block column-major), entries
+    uniform in (-0.1, 0.1) / (dt N), and with Ap given each row i times Ap_i, so that the term
+    changes f_i by at most a tenth of the mean f per step whatever the cell volume."""
+    N = ps.n_momentum
+    key = synth.stream_key(seed, 77)
+    raw = synth.uniform(np.arange(ps.n_space * N * N, dtype=np.uint64), key)
+    blocks = (raw.reshape(ps.n_space, N, N) - 0.5) * (0.2 / dt / N)   # [cell][row][column]
+    if Ap is not None:
+        blocks *= np.asarray(Ap, dtype=np.float64).reshape(ps.n_space, N, 1)
+    return np.ascontiguousarray(blocks.transpose(0, 2, 1))
+
+
 def transport_1d(p_num=32, u_num=8, h_num=1, x_num=128, steps=2000, seed=None,
-                 monolithic=False, with_sync=True):
+                 monolithic=False, with_sync=True, emission=False):
+    """emission: dense per-cell emission blocks as well (emission_blocks)."""
     seed = synth.config_seed(1) if seed is None else seed
     ps = PhaseSpace(species_pair(p_num, u_num, h_num), x_num=x_num)
     f0 = synth.initial_state(ps, theta=1.0, seed=seed)
@@ -149,8 +163,10 @@ def transport_1d(p_num=32, u_num=8, h_num=1, x_num=128, steps=2000, seed=None,
     K = _sync_K(ps, dt) if with_sync else 0.0
     X, P = synth.flux_matrices(ps, sync_K=K, axes="x")
     M = synth.assemble_M_Bin(ps, gains, losses) if monolithic else None
+    blocks = emission_blocks(ps, seed, dt, Ap) if emission else None
     return Workload("transport_1d", ps, f0, np.zeros(ps.n), Ap, Vol, X, P, dt, steps, seed,
-                    gains, losses, M, flux_spec=dict(sync_K=K, axes="x", periodic=True))
+                    gains, losses, M, emission_blocks=blocks,
+                    flux_spec=dict(sync_K=K, axes="x", periodic=True))
 
 
 # Interactions of the collision-ingest case: like-particle, Compton-like and photon pair
@@ -175,7 +191,9 @@ def collision_ingest(p_num=32, u_num=8, h_num=1, seed=None, theta=30.0):
     return ps, blist, arrays, f0, Ap, Vol
 
 
-def precision_box(p_num=64, u_num=8, h_num=2, num_species=3, steps=1000, seed=None):
+def precision_box(p_num=64, u_num=8, h_num=2, num_species=3, steps=1000, seed=None,
+                  emission=False):
+    """emission: a dense emission block as well (emission_blocks)."""
     seed = synth.config_seed(2) if seed is None else seed
     sp = [Species("Ele", 1.0, -1.0, p_num, u_num, h_num),
           Species("Pos", 1.0, 1.0, p_num, u_num, h_num),
@@ -184,8 +202,9 @@ def precision_box(p_num=64, u_num=8, h_num=2, num_species=3, steps=1000, seed=No
     f0 = synth.initial_state(ps, theta=1.0, seed=seed)
     Ap, Vol = synth.ap_vol(ps)
     dt = _dt_expected(ps, f0, Ap, Vol)
+    blocks = emission_blocks(ps, seed, dt, Ap) if emission else None
     return Workload("precision_box", ps, f0, np.zeros(ps.n), Ap, Vol, None, None, dt, steps,
-                    seed, synth_seed=seed)
+                    seed, emission_blocks=blocks, synth_seed=seed)
 
 
 def transport_2d(p_num=32, u_num=8, h_num=2, x_num=32, y_num=32, steps=500, seed=None):
@@ -223,13 +242,7 @@ def tiny(p_num=4, u_num=2, h_num=1, x_num=3, seed=7, steps=20, monolithic=True,
     inj = np.zeros(ps.n)
     if injection:
         inj = 0.01 * synth.initial_state(ps, theta=0.5, seed=seed + 1, noise=0.0)
-    blocks = None
-    if emission:
-        N = ps.n_momentum
-        key = synth.stream_key(seed, 77)
-        raw = synth.uniform(np.arange(ps.n_space * N * N, dtype=np.uint64), key)
-        blocks = (raw.reshape(ps.n_space, N, N) - 0.5) * (0.2 / dt / N)
-        blocks = np.ascontiguousarray(blocks.transpose(0, 2, 1))  # column-major per cell
+    blocks = emission_blocks(ps, seed, dt) if emission else None
     return Workload("tiny", ps, f0, inj, Ap, Vol, X, P, dt, steps, seed, gains, losses, M,
                     emission_blocks=blocks, domain=domain, flux_spec=spec)
 

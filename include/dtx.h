@@ -301,6 +301,7 @@ typedef struct {
     int32_t bin_launch_items;
     int32_t fused_correction;  /* 1: corrected Forward Euler steps sum the slots, solve the
                                   coefficients and update in one launch                    */
+    double emission_bytes; /* emission GEMV: the per-cell blocks + f + df_Emi (0: none)    */
 } dtx_work_info;
 int dtx_work(const dtx_handle* h, dtx_work_info* out);
 /* Device time of the last synchronised window, per kernel class, in milliseconds (HIP events
@@ -310,6 +311,8 @@ int dtx_work(const dtx_handle* h, dtx_work_info* out);
 typedef struct {
     double bin_ms, update_ms;
     int64_t bin_launches, update_launches;
+    double emission_ms;          /* the emission GEMV (handles with emission blocks) */
+    int64_t emission_launches;
 } dtx_kernel_times;
 int dtx_kernel_timing(dtx_handle* h, int32_t enable, int32_t reset, dtx_kernel_times* out);
 

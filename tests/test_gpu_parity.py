@@ -191,6 +191,32 @@ def test_single_cell_parity_fp64(kw):
         assert rel_linf(out.f[i], fo[i]) < TOL64, (kw, i)
 
 
+@pytest.mark.parametrize("case", ["odd_N", "transport_1d_grid", "single_cell", "symplectic"])
+def test_emission_gemv_parity(case):
+    """The emission GEMV (16-byte loads of two rows for even N, 8-byte loads for odd N; with
+    few cells the columns split over several workgroups whose sums are added by a second
+    launch) against the oracle's block GEMV, in both step methods."""
+    mono, method = None, "euler"
+    if case == "odd_N":  # 3 species x 3 states: N = 9
+        w = workloads.tiny(num_species=3, p_num=3, u_num=1, x_num=3, steps=10, emission=True,
+                           injection=True)
+    elif case == "transport_1d_grid":  # N = 512 in 4 cells: column ranges over workgroups
+        w = workloads.transport_1d(x_num=4, steps=6, emission=True)
+        mono = False
+    elif case == "single_cell":  # N = 256, one cell
+        w = workloads.tiny(p_num=16, u_num=8, x_num=1, steps=10, emission=True,
+                           monolithic=False)
+        mono = False
+    else:
+        w = workloads.tiny(steps=20, emission=True, injection=True)
+        method = "symplectic"
+    assert (w.ps.n_momentum % 2 == 1) == (case == "odd_N")
+    m, out, fo, to, _ = _solve_both(w, n_save=2, method=method, mono=mono)
+    for i in range(len(out.f)):
+        assert np.all(np.isfinite(out.f[i]))
+        assert rel_linf(out.f[i], fo[i]) < TOL64, (case, i)
+
+
 @pytest.mark.parametrize("path", __import__("fixture_util").fixtures(), ids=lambda p: p.stem)
 def test_golden_fixtures_fp64(path):
     """The HIP path against the committed golden outputs (oracle + brute force agreed on them)."""

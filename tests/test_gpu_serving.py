@@ -44,11 +44,13 @@ def test_served_equals_single_handle_steps(precision):
         h.close()
 
 
-def test_caller_built_simulation_matches_oracle():
+@pytest.mark.parametrize("emission", [False, True], ids=["no_emission", "emission"])
+def test_caller_built_simulation_matches_oracle(emission):
     """A simulation built by the caller (host flux CSC, host gain / loss channels, f0 with 1%
-    noise) is served next to synthetic requests; its final state agrees with the oracle at
-    1e-12 and equals the same simulation stepped alone bit for bit."""
-    w = workloads.transport_1d(p_num=8, u_num=4, x_num=24, steps=40)
+    noise, and per-cell emission blocks in the second case) is served next to synthetic
+    requests; its final state agrees with the oracle at 1e-12 and equals the same simulation
+    stepped alone bit for bit."""
+    w = workloads.transport_1d(p_num=8, u_num=4, x_num=24, steps=40, emission=emission)
     m = workloads.make_struct(w, precision="fp64", use_monolithic=False)
     alone = workloads.make_struct(w, precision="fp64", use_monolithic=False)
     t = 0.0
