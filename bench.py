@@ -181,7 +181,7 @@ def run_emission(steps, warmup):
     return dict(workload="transport_1d with dense per-cell emission blocks (128 cells x "
                          "512 x 512 FP64)",
                 steps_per_s=round(r["steps_per_s"], 2),
-                kernel="emission_gemv_kernel<2>",
+                kernel="emission_gemv_kernel<2, 2>",
                 kernel_avg_us=round(r["emission_ms"] * 1e3, 2),
                 bytes_per_launch=r["emission_bytes"],
                 achieved_gbs=round(gbs, 1), peak_gbs=PEAK_HBM_GBS,
