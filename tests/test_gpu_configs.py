@@ -4,7 +4,8 @@
   Solve in FP64 against the oracle at 1e-12, every save and every step's dt / save / Cr.
 * transport_1d (2 x 256 states x 128 cells): the full 2000-step FP64 Solve on the GPU; the
   oracle checks the first 30 steps from f0 and the last 30 steps from the GPU's own state at
-  step 1970 (same dt sequence), both at 1e-12; total number per species is conserved.
+  step 1970 (same dt sequence), both at 1e-12; total number per species is conserved.  With
+  dense per-cell emission blocks as well: 10 steps against the oracle at 1e-12.
 * precision_box (3 x 1024 states) and transport_2d (2 x 512 states x 32 x 32 cells): FP32 and
   split-BF16 against the FP64 GPU path over the full 1000 / 500 steps, at the accuracy bars of
   the reduced-precision modes (< 1e-6 relative in f, < 1e-9 number and energy drift); FP64
@@ -124,6 +125,26 @@ def test_transport_1d_full_solve_windows_vs_oracle():
         t1 = species_totals(w.ps, f)
         assert np.all(np.abs(t1 - t0) <= 1e-12 * t0), (t0, t1)
     m.close()
+
+
+def test_transport_1d_emission_vs_oracle():
+    """transport_1d at full size with dense per-cell emission blocks (128 x 512^2, 269 MB): the
+    emission GEMV runs with one column range per row tile; 10 FP64 steps against the oracle
+    (block-CSC emission) at 1e-12, with every step's Cr."""
+    w = workloads.transport_1d(emission=True, steps=10)
+    ts = t_save_marks(w, [0, w.steps])
+    m = workloads.make_struct(w, precision="fp64", use_monolithic=False)
+    out = Solve(m, w.dt, ts, Verbose=1)
+    gstats = m.recent_stats(w.steps)
+    m.close()
+    prob = O.problem_from_workload(w, use_monolithic=False, batched=True, sparse_emission=True)
+    fo, _, ostats, n = prob.solve(w.f0, w.dt, ts, verbose=1)
+    assert n == w.steps
+    assert np.all(np.isfinite(out.f[1]))
+    assert rel_linf(out.f[1], fo[1]) < TOL64
+    for g, o in zip(gstats, ostats):
+        assert g.dt == o.dt and g.finite == o.finite == 1
+        assert cr_close(g.cr, o.cr), (g.cr, o.cr)
 
 
 def _run_modes(w, steps):
