@@ -144,7 +144,8 @@ FUSED_CASES = {
     "transport_1d": lambda: workloads.transport_1d(steps=6),
     # 320 states per species: a second pass with one live segment group of four
     "two_passes": lambda: workloads.transport_1d(p_num=40, x_num=64, steps=6),
-    # 128 cells with emission blocks, a binary Domain and injection: the cell loaded into LDS
+    # 128 cells with emission blocks (the GEMV's df_Emi read by the update), a binary Domain
+    # (the workgroup's all-zero test of the cell) and injection
     "emission_domain": lambda: workloads.tiny(p_num=16, u_num=8, x_num=128, monolithic=False,
                                               emission=True, injection=True,
                                               domain=list(range(1, 129, 3)), steps=6),
