@@ -5,7 +5,8 @@
 * transport_1d (2 x 256 states x 128 cells): the full 2000-step FP64 Solve on the GPU; the
   oracle checks the first 30 steps from f0 and the last 30 steps from the GPU's own state at
   step 1970 (same dt sequence), both at 1e-12; total number per species is conserved.  With
-  dense per-cell emission blocks as well: 10 steps against the oracle at 1e-12.
+  dense per-cell emission blocks as well: 10 steps against the oracle at 1e-12; and 10
+  ForwardSymplecticEuler steps with their Verbose-3 diagnostics.
 * precision_box (3 x 1024 states) and transport_2d (2 x 512 states x 32 x 32 cells): FP32 and
   split-BF16 against the FP64 GPU path over the full 1000 / 500 steps, at the accuracy bars of
   the reduced-precision modes (< 1e-6 relative in f, < 1e-9 number and energy drift); FP64
@@ -145,6 +146,27 @@ def test_transport_1d_emission_vs_oracle():
     for g, o in zip(gstats, ostats):
         assert g.dt == o.dt and g.finite == o.finite == 1
         assert cr_close(g.cr, o.cr), (g.cr, o.cr)
+
+
+def test_transport_1d_symplectic_vs_oracle():
+    """ForwardSymplecticEuler at transport_1d's full size (P flux with the momentum half, X flux
+    with the space half): 10 FP64 steps at Verbose 3 against the oracle, the state at 1e-12 and
+    every step's Cr, Cr_Bin, Cr_Momentum and Cr_Space."""
+    w = workloads.transport_1d(steps=10)
+    ts = t_save_marks(w, [0, w.steps])
+    m = workloads.make_struct(w, precision="fp64", method="symplectic", use_monolithic=False)
+    out = Solve(m, w.dt, ts, Verbose=3)
+    gstats = m.recent_stats(w.steps)
+    m.close()
+    prob = O.problem_from_workload(w, use_monolithic=False, batched=True)
+    fo, _, ostats, n = prob.solve(w.f0, w.dt, ts, verbose=3, method=1)
+    assert n == w.steps == len(gstats)
+    assert rel_linf(out.f[1], fo[1]) < TOL64
+    for g, o in zip(gstats, ostats):
+        assert g.dt == o.dt and g.finite == o.finite
+        for name in ("cr", "cr_bin", "cr_momentum", "cr_space"):
+            assert cr_close(getattr(g, name), getattr(o, name)), (name, getattr(g, name),
+                                                                 getattr(o, name))
 
 
 def _run_modes(w, steps):
