@@ -305,7 +305,8 @@ def run_serving(count):
     1-2 species, 200-2000 steps).  Capacity = solves/s with every request queued at t = 0 (FP32
     grouped kernels with the FP64 moment-rate correction); then Poisson arrivals at 80% of that
     capacity give the time-to-solution percentiles; then every request one at a time in plain
-    FP64 (a server with one slot), whose results give the served FP32 number / energy drift."""
+    FP64 (a server with one slot), whose results give the served FP32 number / energy drift;
+    then the capacity with split-BF16 grouped kernels (corrected) and its drift."""
     import numpy as np
     from dtx import serving
     specs = serving.request_mix(count=count, rate_per_s=1e9)
@@ -326,6 +327,12 @@ def run_serving(count):
              for a, b in zip(res_cap, res64))
     de = max(abs(a.energy_total - b.energy_total) / abs(b.energy_total)
              for a, b in zip(res_cap, res64))
+    log(f"[bench] serving capacity with split-BF16 grouped kernels: {count} requests at t=0")
+    res_bf, wall_bf, _ = serving.run(specs, precision="bf16x3", realtime=False)
+    dn_bf = max(abs(a.number_total - b.number_total) / abs(b.number_total)
+                for a, b in zip(res_bf, res64))
+    de_bf = max(abs(a.energy_total - b.energy_total) / abs(b.energy_total)
+                for a, b in zip(res_bf, res64))
     tf = flops / wall / 1e12
     gbs = gbytes / wall / 1e9
     return dict(requests=count, precision="fp32", moment_correction=True, hbm_budget_gb=200,
@@ -336,6 +343,9 @@ def run_serving(count):
                 serial_fp64_solves_per_s=round(serial, 3), serial_fp64_requests=count,
                 capacity_vs_serial_fp64=round(capacity / serial, 2),
                 number_drift_vs_fp64=dn, energy_drift_vs_fp64=de,
+                bf16x3=dict(capacity_solves_per_s=round(count / wall_bf, 3),
+                            capacity_vs_serial_fp64=round(count / wall_bf / serial, 2),
+                            number_drift_vs_fp64=dn_bf, energy_drift_vs_fp64=de_bf),
                 roofline=dict(achieved_tflops=round(tf, 2),
                               mfma_frac_fp32=round(tf / PEAK_TFLOPS["fp32"], 4),
                               operator_gbs=round(gbs, 1),
@@ -462,7 +472,10 @@ def main():
                          "known-size kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving capacity run (--serving-requests requests queued at "
-                         "t = 0, FP32 grouped kernels): one JSON line with solves/s")
+                         "t = 0, grouped kernels of --serving-precision): one JSON line with "
+                         "solves/s")
+    ap.add_argument("--serving-precision", default="fp32", choices=["fp64", "fp32", "bf16x3"],
+                    help="contraction precision of the --serving-only run")
     ap.add_argument("--flux-setup", action="store_true",
                     help="only compare the flux set-up of the workload: host assembly + CSC "
                          "upload against the stencils built on the device (dtx_synth_flux)")
@@ -473,8 +486,9 @@ def main():
     if args.serving_only:
         from dtx import serving
         specs = serving.request_mix(count=args.serving_requests, rate_per_s=1e9)
-        _, wall, _ = serving.run(specs, precision="fp32", realtime=False)
-        print(json.dumps({"serving_requests": args.serving_requests, "precision": "fp32",
+        _, wall, _ = serving.run(specs, precision=args.serving_precision, realtime=False)
+        print(json.dumps({"serving_requests": args.serving_requests,
+                          "precision": args.serving_precision,
                           "capacity_solves_per_s": round(args.serving_requests / wall, 3),
                           "request_steps": sum(sp.steps for sp in specs)}))
         return
