@@ -88,14 +88,23 @@ def species_totals(w, f, weights=None):
 
 def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True, accuracy_steps=0,
             device_flux=False):
-    """Timed window (no event records), then an untimed pass with HIP events around every
-    step's kernels (their average durations), then, when accuracy_steps > 0, a Solve of that
-    many steps from f0 whose final state the accuracy comparison uses."""
+    """When accuracy_steps > 0, first a Solve of that many steps from f0 whose final state the
+    accuracy comparison uses (it also brings the GPU out of its idle clocks: after 0.2 s idle
+    the first 5 ms of steps run about 20% slower); then from f0 the warmup steps, the timed
+    window (no event records), and an untimed pass with HIP events around every step's kernels
+    (their average durations)."""
     from dtx import workloads
     m = workloads.make_struct(w, precision=precision, use_monolithic=False,
                               moment_correction=moment_correction,
                               device_flux=device_flux and w.flux_spec is not None)
     work = m.work()
+    acc = {}
+    if accuracy_steps:
+        out, n = m.solve_native(w.dt, t_save_marks(w, [0, accuracy_steps]), Verbose=0)
+        assert n == accuracy_steps
+        f = out.f[-1]
+        acc = dict(f=f, number=species_totals(w, f),
+                   energy=species_totals(w, f, w.ps.energy_weights()))
     m.reset()
     t, big = 0.0, 1e300
     for _ in range(warmup):
@@ -120,13 +129,7 @@ def run_gpu(w, precision, steps, warmup, verbose=1, moment_correction=True, accu
                emission_bytes=work.emission_bytes,
                bin_flops=work.bin_flops, gain_bytes=work.gain_bytes, bin_bytes=work.bin_bytes,
                update_bytes=work.update_bytes, items=work.bin_launch_items,
-               device_gb=m.device_bytes() / 1e9)
-    if accuracy_steps:
-        out, n = m.solve_native(w.dt, t_save_marks(w, [0, accuracy_steps]), Verbose=0)
-        assert n == accuracy_steps
-        f = out.f[-1]
-        res.update(f=f, number=species_totals(w, f),
-                   energy=species_totals(w, f, w.ps.energy_weights()))
+               device_gb=m.device_bytes() / 1e9, **acc)
     m.close()
     return res
 
