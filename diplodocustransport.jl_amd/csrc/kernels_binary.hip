@@ -1040,23 +1040,26 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm)
     const int cb = grp * share, ce_ = min(nch, cb + share);
     const int nst = 2 * max(0, ce_ - cb), nst_all = 2 * share;   // stages: own, every group
 
-    d2 ra[4], rb[4], rf;
-    // stage st = half (st & 1) of share chunk (st >> 1)
-    auto load_stage = [&](int st) {
+    d2 rb[4], rf;
+    // stage st = half (st & 1) of share chunk (st >> 1), into buffer buf: the operator straight
+    // into LDS (global_load_lds_dwordx4: lane l of wave w lands at block q's 128 w + 2 l, the
+    // pair the register path stored), the state into registers for the f_i fold
+    auto load_stage = [&](int st, int buf) {
         const int ch = cb + (st >> 1), h = st & 1;
         const uint32_t ce = ctab[ch];
         const int64_t ao = (int64_t)ch * KC * TK + (int64_t)(h * HK + kl) * TK;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) ra[q] = *(const GLB d2*)(Ab + ao + 4 * q * TK);
+        for (int q = 0; q < 4; ++q)
+            dma_lds<16>(Ab + ao + 4 * q * TK, &As[buf][q * 512 + 128 * wave]);
         const GLB double* bp = (const GLB double*)v.B +
                                (int64_t)((int)(ce >> 16) + h * HK + kl) * v.ldc + v.c0 + bcell;
 #pragma unroll
         for (int q = 0; q < 4; ++q) rb[q] = *(const GLB d2*)(bp + (int64_t)(4 * q) * v.ldc);
         if (!loss) rf = *(const GLB d2*)(v.fT + (int64_t)(ce & 0xFFFFu) * v.ldc + v.c0 + bcell);
     };
+    // the B' stores wait for the state loads, issued after the operator's, so the operator
+    // block has landed too
     auto store_stage = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) *(d2*)&As[buf][q * 512 + 2 * tid] = ra[q];
 #pragma unroll
         for (int q = 0; q < 4; ++q) *(d2*)&Bs[buf][q * 512 + 2 * tid] = loss ? rb[q] : rf * rb[q];
     };
@@ -1067,9 +1070,10 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
     if (nst > 0) {
-        load_stage(0);
+        load_stage(0, 0);
         store_stage(0);
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     int buf = 0;
     // G = 1: the workgroup barrier after each stage; G > 1: each group its own stages and
@@ -1078,7 +1082,7 @@ __device__ __forceinline__ void wide64_item(const BinWork& w, SmemWide64<G>& sm)
     for (int st = 0; st < nloop; ++st) {
         const bool mine = st < nst;
         const bool more = st + 1 < nst;
-        if (more) load_stage(st + 1);
+        if (more) load_stage(st + 1, buf ^ 1);
         if (mine) {
 #pragma unroll
             for (int ks = 0; ks < HK / 4; ++ks) {
