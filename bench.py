@@ -47,7 +47,7 @@ DESCRIPTIONS = {
 # HBM bytes per launch of the FP64 contraction from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (separate runs, each scaled by the same counter over a 1 GiB device copy):
 # profiles/transport_1d_hbm_traffic.txt, profiles/precision_box_hbm_traffic.txt.
-TRAFFIC = {("transport_1d", "bin_f64_wide_kernel"): 469_590_000,
+TRAFFIC = {("transport_1d", "bin_f64_wide_kernel"): 470_180_000,
            ("precision_box", "bin_gemv_kernel<0>"): 64_916_380_000}
 
 
