@@ -145,6 +145,16 @@ __device__ inline void load_ctab(const ItemView& v, uint32_t* ctab) {
     __syncthreads();
 }
 
+// Direct-to-LDS load (global_load_lds): lane l of the wave lands at lds + BYTES l.
+#define DTX_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+
+template <int BYTES>
+__device__ __forceinline__ void dma_lds(const GLB void* src, void* lds) {
+    static_assert(BYTES == 16 || BYTES == 4, "16- or 4-byte lanes");
+    if constexpr (BYTES == 16) __builtin_amdgcn_global_load_lds((GLB void*)src, DTX_LDS(lds), 16, 0, 0);
+    else __builtin_amdgcn_global_load_lds((GLB void*)src, DTX_LDS(lds), 4, 0, 0);
+}
+
 // ------------------------------------------------------------------------------- FP64 -----
 // The row multiplier f_i is folded into the B tile while it is staged: B'[t][c] = f_i^c f_j^c
 // (LOSS channels stage f_j^c and multiply by their row's f at the end), so the accumulators
@@ -197,16 +207,19 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     load_ctab(v, ctab);
 
     double ra[AP], rb[BP], rf[BP];
-    d2 pa[2], pb[2], pf = d2{0.0, 0.0};
-    // stage st = half (st & 1) of local chunk (st >> 1)
-    auto load_stage = [&](int st) {
+    d2 pb[2], pf = d2{0.0, 0.0};
+    // stage st = half (st & 1) of local chunk (st >> 1), into buffer buf (64-cell tiles: the
+    // operator straight into LDS, lane l of wave w at piece q's 128 w + 2 l, where the register
+    // path stored it; the B' stores wait for the state loads issued after it)
+    auto load_stage = [&](int st, int buf) {
         const int ch = st >> 1, h = st & 1;
         const uint32_t ce = ctab[ch];
         const int j0 = (int)(ce >> 16) + h * HK;
         if constexpr (frag) {
             const GLB double* src = A + (int64_t)ch * KC * TK + (int64_t)(h * HK + pk) * TK + prow;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) pa[q] = *(const GLB d2*)(src + 8 * q * TK);
+            for (int q = 0; q < 2; ++q)
+                dma_lds<16>(src + 8 * q * TK, &As[buf][q * 512 + 128 * wave]);
             const GLB double* b = B + (int64_t)(j0 + pk) * v.ldc + v.c0 + prow;
 #pragma unroll
             for (int q = 0; q < 2; ++q) pb[q] = *(const GLB d2*)(b + (int64_t)(8 * q) * v.ldc);
@@ -228,8 +241,6 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
     auto store_stage = [&](int buf) {
         if constexpr (frag) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) *(d2*)&As[buf][q * 512 + 2 * tid] = pa[q];
-#pragma unroll
             for (int q = 0; q < 2; ++q) *(d2*)&Bs[buf][q * 512 + 2 * tid] = loss ? pb[q] : pf * pb[q];
         } else {
 #pragma unroll
@@ -247,14 +258,15 @@ __device__ __forceinline__ void f64_item(const BinWork& w, SmemF64<TC>& sm) {
         for (int c = 0; c < T::MC; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
     const int nst = 2 * (v.c_end - v.c_begin);
     if (nst > 0) {
-        load_stage(0);
+        load_stage(0, 0);
         store_stage(0);
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     int buf = 0;
     for (int st = 0; st < nst; ++st) {
         const bool more = st + 1 < nst;
-        if (more) load_stage(st + 1);
+        if (more) load_stage(st + 1, buf ^ 1);
 #pragma unroll
         for (int kk = 0; kk < HK; kk += 4) {
             const int kr = kk + (lane >> 4);
@@ -805,14 +817,6 @@ struct SmemWideDma {
     uint32_t ctab[MAX_RANGE];
 };
 
-#define DTX_LDS(p) ((__attribute__((address_space(3))) void*)(p))
-
-template <int BYTES>
-__device__ __forceinline__ void dma_lds(const GLB void* src, void* lds) {
-    static_assert(BYTES == 16 || BYTES == 4, "16- or 4-byte lanes");
-    if constexpr (BYTES == 16) __builtin_amdgcn_global_load_lds((GLB void*)src, DTX_LDS(lds), 16, 0, 0);
-    else __builtin_amdgcn_global_load_lds((GLB void*)src, DTX_LDS(lds), 4, 0, 0);
-}
 
 template <int NS>
 __device__ __forceinline__ void wide_item_dma(const BinWork& w, SmemWideDma<NS>& sm) {
